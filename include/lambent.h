@@ -1,0 +1,306 @@
+/*
+ * lambent.h - C ABI of the MI355X (gfx950) path-tracing library.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns an lmb_status (0 = success).  On failure the message of the last
+ * error on the calling thread is available from lmb_last_error().
+ *
+ * Each declaration names the reference interface it replaces
+ * (paths relative to the Vecvec/Lambent source tree).
+ *
+ * Threading: handles are externally synchronised.  Calls that take a
+ * `stream` (a hipStream_t; NULL = the default stream) return once the GPU
+ * work is enqueued; lmb_sync() waits.  Host arrays are copied during the
+ * call that receives them.  Device buffers belong to the library; their
+ * device addresses can be exported for zero-copy consumers.
+ */
+#ifndef LAMBENT_H
+#define LAMBENT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status codes                                                        */
+/* ------------------------------------------------------------------ */
+typedef int32_t lmb_status;
+#define LMB_OK 0
+#define LMB_ERR_INVALID_ARGUMENT (-1)
+/* camera.rs:63-66 CameraError::ProjectionUninvertable */
+#define LMB_ERR_SINGULAR_PROJECTION (-2)
+/* camera.rs:63-66 CameraError::ViewUninvertable */
+#define LMB_ERR_SINGULAR_VIEW (-3)
+/* lib.rs:165-170: `assert_ne!(u16::MAX, idx)` panics in the reference */
+#define LMB_ERR_RESERVED_INDEX (-4)
+#define LMB_ERR_OUT_OF_MEMORY (-5)
+/* a HIP runtime call failed, or no GPU is present */
+#define LMB_ERR_DEVICE (-6)
+/* the built BVH is deeper than the traversal stack supports */
+#define LMB_ERR_BVH_TOO_DEEP (-7)
+/* textures.rs:93-97 TextureError (bad dimensions / data) */
+#define LMB_ERR_TEXTURE (-8)
+
+/* Message describing the last failure on this thread ("" if none). */
+const char *lmb_last_error(void);
+/* Library version string. */
+const char *lmb_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Materials  (lib.rs:83-203, bindings.wesl:48-57)                     */
+/* ------------------------------------------------------------------ */
+/* 32-byte record, bit-identical to the reference's `Material`. */
+typedef struct lmb_material {
+    uint32_t tex_pos_1;                /* 2 x f16, low half = u       */
+    uint32_t tex_pos_2;
+    uint32_t tex_pos_3;
+    uint32_t tex_pos_recolour;         /* 2 x u16 texel coordinate     */
+    uint32_t tex_idx_diffuse_emission; /* u16 diffuse | u16 emission<<16 */
+    uint32_t tex_idx_attributes_ty;    /* u16 attributes | u16 type<<16  */
+    uint32_t emission_scale;           /* bf16 emissive | bf16 diffuse<<16 */
+    uint32_t refractive_index;         /* f16 n(760nm) | f16 n(340nm)<<16 */
+} lmb_material;
+
+/* lib.rs:194-203 MaterialType */
+#define LMB_MATERIAL_DIFFUSE 0u
+#define LMB_MATERIAL_METALLIC 1u
+#define LMB_MATERIAL_TRANSPARENT 2u
+/* bindings for `Option<u16>` arguments: negative = None */
+#define LMB_NONE (-1)
+
+/* lib.rs:103-121 EmissionScale::from_f32 + pack: two bf16 (RNE) packed. */
+lmb_status lmb_emission_scale_pack(float emissive_texture_scale,
+                                   float diffuse_texture_scale,
+                                   uint32_t *packed);
+
+/* lib.rs:132-162 Material::new.
+ * tex_idx_emission / tex_idx_attributes: LMB_NONE or 0..0xFFFE
+ * (0xFFFF is reserved and returns LMB_ERR_RESERVED_INDEX).
+ * refractive_index: NULL (= VACUUM 1..1) or {n_760nm, n_340nm}. */
+lmb_status lmb_material_new(const float tex_pos_1[2], const float tex_pos_2[2],
+                            const float tex_pos_3[2],
+                            const uint16_t tex_pos_recolour[2],
+                            uint16_t tex_idx_diffuse, int32_t tex_idx_emission,
+                            int32_t tex_idx_attributes,
+                            uint32_t emission_scale_packed,
+                            const float *refractive_index, uint32_t ty,
+                            lmb_material *out);
+
+/* ------------------------------------------------------------------ */
+/* Camera  (camera.rs:6-70)                                            */
+/* ------------------------------------------------------------------ */
+/* Column-major 4x4 inverses, 128 bytes, bit-identical layout. */
+typedef struct lmb_camera {
+    float projection_inverse[16];
+    float view_inverse[16];
+} lmb_camera;
+
+/* camera.rs:13-25 Camera::from_proj_view (column-major inputs). */
+lmb_status lmb_camera_from_proj_view(const float proj[16], const float view[16],
+                                     lmb_camera *out);
+/* camera.rs:27-42 Camera::update_from_proj_view; NULL leaves a matrix as is. */
+lmb_status lmb_camera_update_from_proj_view(lmb_camera *cam, const float *proj,
+                                            const float *view);
+
+/* lib.rs:207-219 dispatch_size: (ceil(w/64), h). */
+void lmb_dispatch_size(uint32_t width, uint32_t height, uint32_t *groups_x,
+                       uint32_t *groups_y);
+
+/* ------------------------------------------------------------------ */
+/* Textures  (textures.rs:12-243)                                      */
+/* ------------------------------------------------------------------ */
+/* A host-side texture collection, the counterpart of TextureLoader +
+ * bind_group_from_textures.  Images are RGBA8 UNORM (textures.rs:60),
+ * sampled with the default sampler: nearest, clamp-to-edge
+ * (textures.rs:196).  Decoding image files is the caller's job (the
+ * reference uses the `image` crate); this takes decoded RGBA8 texels. */
+typedef struct lmb_textures_s *lmb_textures;
+#define LMB_TEXTURE_DIFFUSE 0u
+#define LMB_TEXTURE_EMISSION 1u
+#define LMB_TEXTURE_ATTRIBUTES 2u
+
+lmb_status lmb_textures_create(lmb_textures *out);
+/* Appends an image to the diffuse / emission / attributes array; its
+ * index in that array is written to *index. */
+lmb_status lmb_textures_add(lmb_textures tex, uint32_t kind, uint32_t width,
+                            uint32_t height, const uint8_t *rgba8,
+                            uint32_t *index);
+/* textures.rs:122-149: without a recolour texture a 1x1 white one is used. */
+lmb_status lmb_textures_set_recolour(lmb_textures tex, uint32_t width,
+                                     uint32_t height, const uint8_t *rgba8);
+/* bindings.wesl:123-124 `bg: texture_cube<f32>`: 6 faces (+x,-x,+y,-y,+z,-z),
+ * each size x size RGBA8.  Default: black. */
+lmb_status lmb_textures_set_background(lmb_textures tex, uint32_t size,
+                                       const uint8_t *rgba8_faces);
+lmb_status lmb_textures_destroy(lmb_textures tex);
+
+/* ------------------------------------------------------------------ */
+/* Scene: geometry + materials + textures on the GPU                   */
+/* Replaces the wgpu BLAS/TLAS build (examples/cornell_box/src/main.rs: */
+/* 464-563) and bind group 0 (bindings.wesl:67-100) with a host SAH    */
+/* build of a compressed 8-wide BVH.                                   */
+/* ------------------------------------------------------------------ */
+typedef struct lmb_scene_s *lmb_scene;
+
+typedef struct lmb_scene_desc {
+    const float *vertices;       /* vec4 per vertex (xyz used), lib.rs:439-454 */
+    uint64_t num_vertices;
+    const uint32_t *indices;     /* 3 per triangle; NULL = non-indexed */
+    uint64_t num_indices;
+    const uint32_t *material_idx; /* 1 per triangle (bindings.wesl:59-61) */
+    const lmb_material *materials;
+    uint64_t num_materials;
+    lmb_textures textures;       /* may be NULL: no textures, black sky */
+} lmb_scene_desc;
+
+lmb_status lmb_scene_create(const lmb_scene_desc *desc, lmb_scene *out);
+lmb_status lmb_scene_destroy(lmb_scene scene);
+/* BVH statistics: node count, depth, leaf triangle count, SAH cost, build ms */
+typedef struct lmb_bvh_stats {
+    uint64_t num_triangles;
+    uint64_t num_nodes;
+    uint32_t max_depth;
+    uint32_t reserved;
+    double sah_cost;
+    double build_ms;
+    uint64_t device_bytes;
+} lmb_bvh_stats;
+lmb_status lmb_scene_bvh_stats(lmb_scene scene, lmb_bvh_stats *stats);
+
+/* Closest-hit ray query (replaces rayQueryInitialize/Proceed/GetCommitted,
+ * general.wesl:1052-1078).  A ray is 8 floats: origin.xyz, t_min,
+ * direction.xyz, t_max.  A hit is 4 x 32 bit: prim (u32, 0xFFFFFFFF = miss),
+ * t, u, v (barycentric weights of vertex 1 and 2).  Ties in t resolve to the
+ * lower primitive index.  Host arrays; blocks until done. */
+typedef struct lmb_hit {
+    uint32_t prim;
+    float t;
+    float u;
+    float v;
+} lmb_hit;
+lmb_status lmb_scene_intersect(lmb_scene scene, const float *rays, uint64_t n,
+                               lmb_hit *hits);
+/* Same on device buffers (rays_dev: n*32 bytes, hits_dev: n*16 bytes). */
+lmb_status lmb_scene_intersect_device(lmb_scene scene, const void *rays_dev,
+                                      uint64_t n, void *hits_dev, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Tracer  (lib.rs:260-376, path_tracing/mod.rs:7-147,                 */
+/* low_level.rs:77-113)                                                */
+/* ------------------------------------------------------------------ */
+#define LMB_QUALITY_LOW 0u    /* path_tracing/mod.rs:126 Low    */
+#define LMB_QUALITY_MEDIUM 1u /* path_tracing/mod.rs:98  Medium */
+#define LMB_QUALITY_HIGH 2u   /* path_tracing/mod.rs:70  High   */
+
+/* path_tracing/mod.rs:25-34 RestirTemporalMode */
+#define LMB_RESTIR_DISABLED 0u
+#define LMB_RESTIR_PER_SAMPLE 1u
+#define LMB_RESTIR_PER_PIXEL 2u
+
+/* Path guiding: the reference's Markov-chain (vMF) guiding with its world
+ * hash grid (general.wesl:148-499), or plain BSDF sampling. */
+#define LMB_GUIDING_OFF 0u
+#define LMB_GUIDING_MARKOV_CHAIN 1u
+
+typedef struct lmb_tracer_opts {
+    uint32_t quality;      /* LMB_QUALITY_*                              */
+    uint32_t samples;      /* SAMPLES override, default 4 (low_level.rs:101) */
+    float t_min;           /* default 1e-4  (low_level.rs:102)           */
+    float t_max;           /* default 1000  (low_level.rs:103)           */
+    uint32_t restir;       /* LMB_RESTIR_*, default PER_PIXEL            */
+    uint32_t bounces;      /* path segments per sample, reference const 5 (general.wesl:50) */
+    uint32_t rr_start;     /* Russian roulette from this bounce on; >= bounces = off (reference) */
+    uint32_t guiding;      /* LMB_GUIDING_*                              */
+} lmb_tracer_opts;
+
+/* Fills the reference defaults: Medium, 4 spp, 1e-4, 1000, PerPixel,
+ * 5 bounces, no Russian roulette, Markov-chain guiding. */
+void lmb_tracer_opts_default(lmb_tracer_opts *opts);
+
+typedef struct lmb_tracer_s *lmb_tracer;
+/* lib.rs:268-318 RayTracer::<Q>::new + create_pipeline */
+lmb_status lmb_tracer_create(const lmb_tracer_opts *opts, lmb_tracer *out);
+lmb_status lmb_tracer_destroy(lmb_tracer tracer);
+
+/* ------------------------------------------------------------------ */
+/* Per-frame buffers  (data_buffer/mod.rs:18-472)                      */
+/* ------------------------------------------------------------------ */
+/* data_buffer/mod.rs:18-27 BufferType bit flags */
+#define LMB_BUFFER_MARKOV_CHAIN_SCREEN_SPACE (1u << 0)
+#define LMB_BUFFER_SPATIAL_RESAMPLING (1u << 1)
+#define LMB_BUFFER_MARKOV_CHAIN_WORLD_SPACE (1u << 2)
+#define LMB_BUFFER_ALL 7u
+
+/* Outputs: bindings.wesl:117-122 (rgba32float each, row-major) */
+#define LMB_OUTPUT_COLOR 0u
+#define LMB_OUTPUT_NORMAL 1u
+#define LMB_OUTPUT_ALBEDO 2u
+
+typedef struct lmb_data_buffers_s *lmb_data_buffers;
+/* data_buffer/mod.rs:151-252 DataBuffers::new; also owns the three output
+ * images and the wavefront ray queues sized for width x height. */
+lmb_status lmb_data_buffers_create(uint32_t width, uint32_t height,
+                                   uint32_t world_size, uint32_t buffer_flags,
+                                   lmb_data_buffers *out);
+/* data_buffer/mod.rs:259-267 DataBuffers::resize */
+lmb_status lmb_data_buffers_resize(lmb_data_buffers bufs, uint32_t width,
+                                   uint32_t height);
+lmb_status lmb_data_buffers_destroy(lmb_data_buffers bufs);
+/* Device address and byte size of an output image (LMB_OUTPUT_*). */
+lmb_status lmb_data_buffers_output(lmb_data_buffers bufs, uint32_t which,
+                                   void **dev_ptr, uint64_t *bytes);
+/* Copies an output image (width*height*4 floats) to host memory (blocks). */
+lmb_status lmb_data_buffers_read_output(lmb_data_buffers bufs, uint32_t which,
+                                        float *host);
+/* Byte-level access to the persistent state for tests and checkpoints.
+ * which: 0 gi_reservoirs, 1 old_gi_reservoirs, 2 info, 3 markov_chains,
+ *        4 old_markov_chains, 5 world_markov_chains, 6 old_world_markov_chains */
+lmb_status lmb_data_buffers_state_size(lmb_data_buffers bufs, uint32_t which,
+                                       uint64_t *bytes);
+lmb_status lmb_data_buffers_read_state(lmb_data_buffers bufs, uint32_t which,
+                                       void *host, uint64_t bytes);
+lmb_status lmb_data_buffers_write_state(lmb_data_buffers bufs, uint32_t which,
+                                        const void *host, uint64_t bytes);
+/* Path vertices (closest-hit queries) issued by the last lmb_trace. Blocks. */
+lmb_status lmb_data_buffers_path_vertices(lmb_data_buffers bufs,
+                                          uint64_t *vertices);
+
+/* ------------------------------------------------------------------ */
+/* Frame passes                                                        */
+/* ------------------------------------------------------------------ */
+/* general.wesl:506-610 rt_main dispatched over dispatch_size(w, h)
+ * (examples/cornell_box/src/main.rs:723-731); seed_offset is the 4-byte
+ * immediate (general.wesl:322). */
+lmb_status lmb_trace(lmb_tracer tracer, lmb_scene scene, const lmb_camera *cam,
+                     lmb_data_buffers bufs, uint32_t seed_offset, void *stream);
+
+/* Primary-ray closest hits for every pixel (create_ray, general.wesl:
+ * 1158-1173, then the ray query): prim (u32) and t per pixel, host arrays. */
+lmb_status lmb_primary_hits(lmb_scene scene, const lmb_camera *cam,
+                            uint32_t width, uint32_t height, float t_min,
+                            float t_max, uint32_t *prim, float *t);
+
+/* importance_sampling/temporal_resampling.wesl:37-64 */
+lmb_status lmb_temporal_resample(lmb_data_buffers bufs, uint32_t seed_offset,
+                                 void *stream);
+/* importance_sampling/spatial_resampling.wesl:52-194 (IS_SAMPLES, IS_SPACE
+ * overrides, defaults 8 / 61) writes the final colour image. */
+lmb_status lmb_spatial_resample(lmb_data_buffers bufs, uint32_t is_samples,
+                                uint32_t is_space, uint32_t seed_offset,
+                                void *stream);
+/* data_buffer/mod.rs:272-349 DataBuffers::advance_frame with
+ * AdvanceOptions{confidence_cap} (data_buffer/mod.rs:442-472). */
+lmb_status lmb_advance_frame(lmb_data_buffers bufs, uint32_t buffers_to_advance,
+                             uint32_t confidence_cap, void *stream);
+
+/* Waits for all work on `stream` (NULL = device-wide). */
+lmb_status lmb_sync(void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LAMBENT_H */

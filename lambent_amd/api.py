@@ -1,0 +1,765 @@
+"""Python mirror of the reference's host API over the C ABI (include/lambent.h).
+
+Names and argument meaning follow the reference crate:
+``Material.new`` (lib.rs:132-162), ``EmissionScale`` (lib.rs:96-121),
+``MaterialType`` (lib.rs:194-203), ``refractive_indices`` (lib.rs:30-56),
+``Camera.from_proj_view`` (camera.rs:13-42), ``dispatch_size`` (lib.rs:207-219),
+``TextureLoader`` (textures.rs:12-91), ``RayTracer``/``Low``/``Medium``/``High``
+(lib.rs:260-329, path_tracing/mod.rs:66-147), ``PathTracerOpts`` /
+``RestirTemporalMode`` (path_tracing/mod.rs:7-34), ``DataBuffers`` /
+``BufferType`` / ``AdvanceOptions`` (data_buffer/mod.rs:18-472).
+
+Every call goes to liblambent_hip.so; there is no Python or CPU fallback.  A
+missing library raises at import of this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import io
+import os
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+_LIB_PATH = Path(os.environ.get("LMB_LIBRARY", _PKG / "liblambent_hip.so"))
+
+if not _LIB_PATH.exists():
+    raise ImportError(
+        f"{_LIB_PATH} is missing: build it with `python -m lambent_amd.build` "
+        "(the HIP library is required; there is no fallback)")
+
+_lib = ctypes.CDLL(str(_LIB_PATH))
+
+# ----------------------------------------------------------------- errors
+LMB_OK = 0
+ERR_INVALID_ARGUMENT = -1
+ERR_SINGULAR_PROJECTION = -2
+ERR_SINGULAR_VIEW = -3
+ERR_RESERVED_INDEX = -4
+ERR_OUT_OF_MEMORY = -5
+ERR_DEVICE = -6
+ERR_BVH_TOO_DEEP = -7
+ERR_TEXTURE = -8
+
+
+class LambentError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+class CameraError(LambentError):
+    """camera.rs:63-66 CameraError (ProjectionUninvertable / ViewUninvertable)."""
+
+
+class TextureError(LambentError):
+    """textures.rs:93-97 TextureError."""
+
+
+_lib.lmb_last_error.restype = ctypes.c_char_p
+_lib.lmb_version.restype = ctypes.c_char_p
+
+
+def _check(code: int) -> None:
+    if code == LMB_OK:
+        return
+    msg = _lib.lmb_last_error().decode(errors="replace")
+    if code in (ERR_SINGULAR_PROJECTION, ERR_SINGULAR_VIEW):
+        raise CameraError(code, msg)
+    if code == ERR_TEXTURE:
+        raise TextureError(code, msg)
+    raise LambentError(code, msg)
+
+
+def version() -> str:
+    return _lib.lmb_version().decode()
+
+
+def library_path() -> str:
+    return str(_LIB_PATH)
+
+
+# ---------------------------------------------------------------- ctypes
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u16p = ctypes.POINTER(ctypes.c_uint16)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+
+
+class _MaterialC(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "tex_pos_1", "tex_pos_2", "tex_pos_3", "tex_pos_recolour",
+        "tex_idx_diffuse_emission", "tex_idx_attributes_ty", "emission_scale",
+        "refractive_index")]
+
+
+class _CameraC(ctypes.Structure):
+    _fields_ = [("projection_inverse", ctypes.c_float * 16),
+                ("view_inverse", ctypes.c_float * 16)]
+
+
+class _SceneDescC(ctypes.Structure):
+    _fields_ = [("vertices", _f32p), ("num_vertices", ctypes.c_uint64),
+                ("indices", _u32p), ("num_indices", ctypes.c_uint64),
+                ("material_idx", _u32p), ("materials", ctypes.POINTER(_MaterialC)),
+                ("num_materials", ctypes.c_uint64), ("textures", _vp)]
+
+
+class _BvhStatsC(ctypes.Structure):
+    _fields_ = [("num_triangles", ctypes.c_uint64), ("num_nodes", ctypes.c_uint64),
+                ("max_depth", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("sah_cost", ctypes.c_double), ("build_ms", ctypes.c_double),
+                ("device_bytes", ctypes.c_uint64)]
+
+
+class _TracerOptsC(ctypes.Structure):
+    _fields_ = [("quality", ctypes.c_uint32), ("samples", ctypes.c_uint32),
+                ("t_min", ctypes.c_float), ("t_max", ctypes.c_float),
+                ("restir", ctypes.c_uint32), ("bounces", ctypes.c_uint32),
+                ("rr_start", ctypes.c_uint32), ("guiding", ctypes.c_uint32)]
+
+
+def _sig(name, argtypes, restype=ctypes.c_int32):
+    fn = getattr(_lib, name)
+    fn.argtypes = argtypes
+    fn.restype = restype
+    return fn
+
+
+_sig("lmb_emission_scale_pack", [ctypes.c_float, ctypes.c_float, _u32p])
+_sig("lmb_material_new", [_f32p, _f32p, _f32p, _u16p, ctypes.c_uint16, ctypes.c_int32,
+                          ctypes.c_int32, ctypes.c_uint32, _f32p, ctypes.c_uint32,
+                          ctypes.POINTER(_MaterialC)])
+_sig("lmb_camera_from_proj_view", [_f32p, _f32p, ctypes.POINTER(_CameraC)])
+_sig("lmb_camera_update_from_proj_view", [ctypes.POINTER(_CameraC), _f32p, _f32p])
+_sig("lmb_dispatch_size", [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p], None)
+_sig("lmb_textures_create", [ctypes.POINTER(_vp)])
+_sig("lmb_textures_add", [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u8p, _u32p])
+_sig("lmb_textures_set_recolour", [_vp, ctypes.c_uint32, ctypes.c_uint32, _u8p])
+_sig("lmb_textures_set_background", [_vp, ctypes.c_uint32, _u8p])
+_sig("lmb_textures_destroy", [_vp])
+_sig("lmb_scene_create", [ctypes.POINTER(_SceneDescC), ctypes.POINTER(_vp)])
+_sig("lmb_scene_destroy", [_vp])
+_sig("lmb_scene_bvh_stats", [_vp, ctypes.POINTER(_BvhStatsC)])
+_sig("lmb_scene_intersect", [_vp, _f32p, ctypes.c_uint64, _vp])
+_sig("lmb_scene_intersect_device", [_vp, _vp, ctypes.c_uint64, _vp, _vp])
+_sig("lmb_primary_hits", [_vp, ctypes.POINTER(_CameraC), ctypes.c_uint32, ctypes.c_uint32,
+                          ctypes.c_float, ctypes.c_float, _u32p, _f32p])
+_sig("lmb_sync", [_vp])
+
+# Entry points of the tracer and per-frame state (declared in lambent.h);
+# bound lazily so that a partial library still imports for the CPU tests.
+_OPTIONAL = {
+    "lmb_tracer_opts_default": ([ctypes.POINTER(_TracerOptsC)], None),
+    "lmb_tracer_create": ([ctypes.POINTER(_TracerOptsC), ctypes.POINTER(_vp)], ctypes.c_int32),
+    "lmb_tracer_destroy": ([_vp], ctypes.c_int32),
+    "lmb_data_buffers_create": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.c_uint32, ctypes.POINTER(_vp)], ctypes.c_int32),
+    "lmb_data_buffers_resize": ([_vp, ctypes.c_uint32, ctypes.c_uint32], ctypes.c_int32),
+    "lmb_data_buffers_destroy": ([_vp], ctypes.c_int32),
+    "lmb_data_buffers_output": ([_vp, ctypes.c_uint32, ctypes.POINTER(_vp),
+                                 ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int32),
+    "lmb_data_buffers_read_output": ([_vp, ctypes.c_uint32, _f32p], ctypes.c_int32),
+    "lmb_data_buffers_state_size": ([_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)],
+                                    ctypes.c_int32),
+    "lmb_data_buffers_read_state": ([_vp, ctypes.c_uint32, _vp, ctypes.c_uint64],
+                                    ctypes.c_int32),
+    "lmb_data_buffers_write_state": ([_vp, ctypes.c_uint32, _vp, ctypes.c_uint64],
+                                     ctypes.c_int32),
+    "lmb_data_buffers_path_vertices": ([_vp, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int32),
+    "lmb_trace": ([_vp, _vp, ctypes.POINTER(_CameraC), _vp, ctypes.c_uint32, _vp],
+                  ctypes.c_int32),
+    "lmb_temporal_resample": ([_vp, ctypes.c_uint32, _vp], ctypes.c_int32),
+    "lmb_spatial_resample": ([_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp],
+                             ctypes.c_int32),
+    "lmb_advance_frame": ([_vp, ctypes.c_uint32, ctypes.c_uint32, _vp], ctypes.c_int32),
+}
+for _name, (_args, _res) in _OPTIONAL.items():
+    if hasattr(_lib, _name):
+        _sig(_name, _args, _res)
+
+
+def exported_symbols() -> list[str]:
+    """Names declared in lambent.h that the loaded library exports."""
+    return [n for n in HEADER_FUNCTIONS if hasattr(_lib, n)]
+
+
+HEADER_FUNCTIONS = [
+    "lmb_last_error", "lmb_version", "lmb_emission_scale_pack", "lmb_material_new",
+    "lmb_camera_from_proj_view", "lmb_camera_update_from_proj_view", "lmb_dispatch_size",
+    "lmb_textures_create", "lmb_textures_add", "lmb_textures_set_recolour",
+    "lmb_textures_set_background", "lmb_textures_destroy", "lmb_scene_create",
+    "lmb_scene_destroy", "lmb_scene_bvh_stats", "lmb_scene_intersect",
+    "lmb_scene_intersect_device", "lmb_sync", "lmb_primary_hits", *list(_OPTIONAL),
+]
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(_f32p)
+
+
+def _uptr(a: np.ndarray):
+    return a.ctypes.data_as(_u32p)
+
+
+# -------------------------------------------------------------- materials
+class MaterialType(enum.IntEnum):
+    """lib.rs:194-203."""
+    Diffuse = 0
+    Metallic = 1
+    Transparent = 2
+
+
+class refractive_indices:  # noqa: N801 - mirrors the Rust module name
+    """lib.rs:30-56 (ranges are (n at 760 nm, n at 340 nm))."""
+    VACUUM = (1.0, 1.0)
+    AIR = (1.00027531, 1.000287)
+    ICE = (1.3057, 1.3267)
+    WATER = (1.33, 1.3442)
+    GLASS = (1.4540, 1.4787)
+    DIAMOND = (2.4063, 2.4922)
+
+    @staticmethod
+    def refractive_index_between(in_front, behind):
+        f32 = np.float32
+        return (float(f32(behind[0]) / f32(in_front[0])), float(f32(behind[1]) / f32(in_front[1])))
+
+
+@dataclass(frozen=True)
+class EmissionScale:
+    """lib.rs:96-121: two bf16 scales, packed emissive | diffuse << 16."""
+    emissive_texture_scale: float = 0.0
+    diffuse_texture_scale: float = 0.0
+
+    @staticmethod
+    def from_emission_scale(scale: float) -> "EmissionScale":
+        return EmissionScale(float(scale), 0.0)
+
+    @staticmethod
+    def from_f32(emissive_texture_scale: float, diffuse_texture_scale: float) -> "EmissionScale":
+        return EmissionScale(float(emissive_texture_scale), float(diffuse_texture_scale))
+
+    def pack(self) -> int:
+        out = ctypes.c_uint32()
+        _check(_lib.lmb_emission_scale_pack(self.emissive_texture_scale,
+                                            self.diffuse_texture_scale, ctypes.byref(out)))
+        return out.value
+
+
+class Material:
+    """lib.rs:83-163: the 32-byte material record."""
+    __slots__ = ("_c",)
+    DTYPE = np.dtype([(n, "<u4") for n, _ in _MaterialC._fields_])
+
+    def __init__(self, c: _MaterialC):
+        self._c = c
+
+    @staticmethod
+    def new(tex_pos_1: Sequence[float], tex_pos_2: Sequence[float], tex_pos_3: Sequence[float],
+            tex_pos_recolour: Sequence[int], tex_idx_diffuse: int,
+            tex_idx_emission: Optional[int], tex_idx_attributes: Optional[int],
+            emission_scale: EmissionScale, refractive_index: Optional[Sequence[float]],
+            ty: MaterialType) -> "Material":
+        p1 = np.asarray(tex_pos_1, np.float32)
+        p2 = np.asarray(tex_pos_2, np.float32)
+        p3 = np.asarray(tex_pos_3, np.float32)
+        rc = np.asarray(tex_pos_recolour, np.uint16)
+        ior = None if refractive_index is None else np.asarray(refractive_index, np.float32)
+        c = _MaterialC()
+        _check(_lib.lmb_material_new(
+            _fptr(p1), _fptr(p2), _fptr(p3), rc.ctypes.data_as(_u16p), int(tex_idx_diffuse),
+            -1 if tex_idx_emission is None else int(tex_idx_emission),
+            -1 if tex_idx_attributes is None else int(tex_idx_attributes),
+            emission_scale.pack(), None if ior is None else _fptr(ior), int(ty),
+            ctypes.byref(c)))
+        return Material(c)
+
+    def words(self) -> tuple:
+        return tuple(getattr(self._c, n) for n, _ in _MaterialC._fields_)
+
+    def to_bytes(self) -> bytes:
+        return bytes(self._c)
+
+    def __eq__(self, other):
+        return isinstance(other, Material) and self.words() == other.words()
+
+    def __repr__(self):
+        return "Material(" + ", ".join(f"{w:#010x}" for w in self.words()) + ")"
+
+
+def materials_buffer(materials: Sequence[Material]) -> bytes:
+    """lib.rs:184-192 `<[Material]>::buffer_descriptor` contents."""
+    return b"".join(m.to_bytes() for m in materials)
+
+
+def materials_array(materials: Sequence[Material]) -> np.ndarray:
+    return np.frombuffer(materials_buffer(materials), dtype=np.uint32).reshape(-1, 8).copy()
+
+
+# ----------------------------------------------------------------- camera
+class Camera:
+    """camera.rs:6-70: inverse projection and view matrices (column-major)."""
+
+    def __init__(self, c: _CameraC):
+        self._c = c
+
+    @staticmethod
+    def _cols(m) -> np.ndarray:
+        a = np.asarray(m, dtype=np.float32)
+        if a.shape != (4, 4):
+            raise ValueError("matrices are [[f32; 4]; 4] (four columns)")
+        return np.ascontiguousarray(a.reshape(16))
+
+    @staticmethod
+    def from_proj_view(proj, view) -> "Camera":
+        """`proj`/`view` are 4 columns of 4 floats, like cgmath's `.into()`."""
+        p, v = Camera._cols(proj), Camera._cols(view)
+        c = _CameraC()
+        _check(_lib.lmb_camera_from_proj_view(_fptr(p), _fptr(v), ctypes.byref(c)))
+        return Camera(c)
+
+    def update_from_proj_view(self, proj=None, view=None) -> None:
+        p = None if proj is None else Camera._cols(proj)
+        v = None if view is None else Camera._cols(view)
+        _check(_lib.lmb_camera_update_from_proj_view(
+            ctypes.byref(self._c), None if p is None else _fptr(p),
+            None if v is None else _fptr(v)))
+
+    @property
+    def projection_inverse(self) -> np.ndarray:
+        return np.array(self._c.projection_inverse, np.float32).reshape(4, 4)
+
+    @property
+    def view_inverse(self) -> np.ndarray:
+        return np.array(self._c.view_inverse, np.float32).reshape(4, 4)
+
+    def to_bytes(self) -> bytes:
+        return bytes(self._c)
+
+
+def perspective(fovy_deg: float, aspect: float, near: float, far: float) -> np.ndarray:
+    """cgmath::perspective (OpenGL clip space), returned as 4 columns, f32."""
+    f32 = np.float32
+    fovy = f32(np.deg2rad(f32(fovy_deg)))
+    f = f32(1.0) / f32(np.tan(fovy / f32(2.0)))
+    n, fa, a = f32(near), f32(far), f32(aspect)
+    m = np.zeros((4, 4), np.float32)
+    m[0, 0] = f / a
+    m[1, 1] = f
+    m[2, 2] = (fa + n) / (n - fa)
+    m[2, 3] = f32(-1.0)
+    m[3, 2] = (f32(2.0) * fa * n) / (n - fa)
+    return m
+
+
+def look_to_rh(eye, direction, up) -> np.ndarray:
+    """cgmath::Matrix4::look_to_rh, returned as 4 columns, f32."""
+    f32 = np.float32
+    e = np.asarray(eye, f32)
+    f = np.asarray(direction, f32)
+    f = f / f32(np.sqrt(f32(np.dot(f, f))))
+    s = np.cross(f, np.asarray(up, f32)).astype(f32)
+    s = s / f32(np.sqrt(f32(np.dot(s, s))))
+    u = np.cross(s, f).astype(f32)
+    m = np.zeros((4, 4), np.float32)
+    m[0] = (s[0], u[0], -f[0], 0)
+    m[1] = (s[1], u[1], -f[1], 0)
+    m[2] = (s[2], u[2], -f[2], 0)
+    m[3] = (-np.dot(e, s), -np.dot(e, u), np.dot(e, f), 1)
+    return m
+
+
+@dataclass(frozen=True)
+class DispatchSize:
+    width: int
+    height: int
+
+
+def dispatch_size(width: int, height: int) -> DispatchSize:
+    """lib.rs:207-219."""
+    gx, gy = ctypes.c_uint32(), ctypes.c_uint32()
+    _lib.lmb_dispatch_size(width, height, ctypes.byref(gx), ctypes.byref(gy))
+    return DispatchSize(gx.value, gy.value)
+
+
+# --------------------------------------------------------------- textures
+class TextureKind(enum.IntEnum):
+    DIFFUSE = 0
+    EMISSION = 1
+    ATTRIBUTES = 2
+
+
+class TextureLoader:
+    """textures.rs:12-91: decodes images to RGBA8 (the PNG decode uses Pillow,
+    where the reference uses the `image` crate)."""
+
+    def __init__(self):
+        self.images: list[np.ndarray] = []
+
+    def load(self, path: str) -> None:
+        with open(path, "rb") as fh:
+            self.load_from_bytes(fh.read())
+
+    def load_from_bytes(self, data: bytes) -> None:
+        try:
+            from PIL import Image
+        except ImportError as exc:  # pragma: no cover
+            raise TextureError(ERR_TEXTURE, f"no image decoder available: {exc}") from exc
+        try:
+            img = Image.open(io.BytesIO(data)).convert("RGBA")
+        except Exception as exc:
+            raise TextureError(ERR_TEXTURE, f"could not decode image: {exc}") from exc
+        self.images.append(np.asarray(img, dtype=np.uint8).copy())
+
+    def load_rgba8(self, rgba: np.ndarray) -> None:
+        a = np.ascontiguousarray(rgba, dtype=np.uint8)
+        if a.ndim != 3 or a.shape[2] != 4:
+            raise TextureError(ERR_TEXTURE, "expected an (h, w, 4) uint8 array")
+        self.images.append(a)
+
+
+class TextureSet:
+    """bind_group_from_textures (textures.rs:151-243): diffuse, emission and
+    attribute arrays, optional recolour texture and the background cube."""
+
+    def __init__(self, diffuse: Sequence[np.ndarray] = (), emission: Sequence[np.ndarray] = (),
+                 attributes: Sequence[np.ndarray] = (), recolour: Optional[np.ndarray] = None,
+                 background: Optional[np.ndarray] = None):
+        h = _vp()
+        _check(_lib.lmb_textures_create(ctypes.byref(h)))
+        self._h = h
+        for kind, imgs in ((TextureKind.DIFFUSE, diffuse), (TextureKind.EMISSION, emission),
+                           (TextureKind.ATTRIBUTES, attributes)):
+            for img in imgs:
+                self.add(kind, img)
+        if recolour is not None:
+            a = np.ascontiguousarray(recolour, np.uint8)
+            _check(_lib.lmb_textures_set_recolour(self._h, a.shape[1], a.shape[0],
+                                                  a.ctypes.data_as(_u8p)))
+        if background is not None:
+            a = np.ascontiguousarray(background, np.uint8)
+            if a.ndim != 4 or a.shape[0] != 6 or a.shape[1] != a.shape[2] or a.shape[3] != 4:
+                raise TextureError(ERR_TEXTURE, "background must be (6, n, n, 4) uint8")
+            _check(_lib.lmb_textures_set_background(self._h, a.shape[1], a.ctypes.data_as(_u8p)))
+
+    def add(self, kind: TextureKind, img: np.ndarray) -> int:
+        a = np.ascontiguousarray(img, np.uint8)
+        if a.ndim != 3 or a.shape[2] != 4:
+            raise TextureError(ERR_TEXTURE, "expected an (h, w, 4) uint8 array")
+        idx = ctypes.c_uint32()
+        _check(_lib.lmb_textures_add(self._h, int(kind), a.shape[1], a.shape[0],
+                                     a.ctypes.data_as(_u8p), ctypes.byref(idx)))
+        return idx.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            _lib.lmb_textures_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solid_rgba8(r: int, g: int, b: int, a: int = 255, size: int = 1) -> np.ndarray:
+    img = np.empty((size, size, 4), np.uint8)
+    img[...] = (r, g, b, a)
+    return img
+
+
+def uniform_background(value: float, size: int = 1) -> np.ndarray:
+    c = int(round(min(max(value, 0.0), 1.0) * 255))
+    return np.broadcast_to(np.array([c, c, c, 255], np.uint8), (6, size, size, 4)).copy()
+
+
+# ------------------------------------------------------------------ scene
+@dataclass
+class BvhStats:
+    num_triangles: int
+    num_nodes: int
+    max_depth: int
+    sah_cost: float
+    build_ms: float
+    device_bytes: int
+
+
+class Scene:
+    """Geometry + materials + textures on the GPU with a compressed 8-wide
+    BVH (replaces the BLAS/TLAS build and bind groups 0 and 2)."""
+
+    def __init__(self, vertices: np.ndarray, indices: Optional[np.ndarray],
+                 material_idx: np.ndarray, materials: Sequence[Material] | np.ndarray,
+                 textures: Optional[TextureSet] = None):
+        v = np.ascontiguousarray(vertices, np.float32)
+        if v.ndim != 2 or v.shape[1] not in (3, 4):
+            raise ValueError("vertices must be (n, 3) or (n, 4) float32")
+        if v.shape[1] == 3:
+            v = np.ascontiguousarray(np.concatenate([v, np.zeros((len(v), 1), np.float32)], 1))
+        idx = None if indices is None else np.ascontiguousarray(indices, np.uint32).reshape(-1)
+        mi = np.ascontiguousarray(material_idx, np.uint32).reshape(-1)
+        if isinstance(materials, np.ndarray):
+            mats = np.ascontiguousarray(materials, np.uint32).reshape(-1, 8)
+        else:
+            mats = materials_array(materials)
+        self._keep = (v, idx, mi, mats, textures)
+        d = _SceneDescC()
+        d.vertices = _fptr(v)
+        d.num_vertices = len(v)
+        d.indices = None if idx is None else _uptr(idx)
+        d.num_indices = 0 if idx is None else len(idx)
+        d.material_idx = _uptr(mi)
+        d.materials = mats.ctypes.data_as(ctypes.POINTER(_MaterialC))
+        d.num_materials = len(mats)
+        d.textures = None if textures is None else textures.handle
+        h = _vp()
+        _check(_lib.lmb_scene_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self.num_triangles = (len(idx) // 3) if idx is not None else len(v) // 3
+
+    @property
+    def handle(self):
+        return self._h
+
+    def bvh_stats(self) -> BvhStats:
+        s = _BvhStatsC()
+        _check(_lib.lmb_scene_bvh_stats(self._h, ctypes.byref(s)))
+        return BvhStats(s.num_triangles, s.num_nodes, s.max_depth, s.sah_cost, s.build_ms,
+                        s.device_bytes)
+
+    def intersect(self, rays: np.ndarray) -> np.ndarray:
+        """rays: (n, 8) float32 (origin, tmin, dir, tmax) -> (n,) structured hits."""
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.empty(len(r), dtype=HIT_DTYPE)
+        _check(_lib.lmb_scene_intersect(self._h, _fptr(r), len(r), hits.ctypes.data_as(_vp)))
+        return hits
+
+    def primary_hits(self, camera: Camera, width: int, height: int, t_min: float = 1e-4,
+                     t_max: float = 1000.0):
+        prim = np.empty(width * height, np.uint32)
+        t = np.empty(width * height, np.float32)
+        _check(_lib.lmb_primary_hits(self._h, ctypes.byref(camera._c), width, height, t_min,
+                                     t_max, _uptr(prim), _fptr(t)))
+        return prim.reshape(height, width), t.reshape(height, width)
+
+    def close(self):
+        if self._h:
+            _lib.lmb_scene_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+HIT_DTYPE = np.dtype([("prim", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+MISS = 0xFFFFFFFF
+
+
+def sync(stream=None) -> None:
+    _check(_lib.lmb_sync(stream))
+
+
+# ----------------------------------------------------------------- tracer
+class Quality(enum.IntEnum):
+    """path_tracing/mod.rs:66-147: High (spectral), Medium, Low."""
+    LOW = 0
+    MEDIUM = 1
+    HIGH = 2
+
+
+Low, Medium, High = Quality.LOW, Quality.MEDIUM, Quality.HIGH
+
+
+class RestirTemporalMode(enum.IntEnum):
+    """path_tracing/mod.rs:25-34."""
+    Disabled = 0
+    PerSample = 1
+    PerPixel = 2
+
+
+class Guiding(enum.IntEnum):
+    OFF = 0
+    MARKOV_CHAIN = 1
+
+
+@dataclass
+class PathTracerOpts:
+    """path_tracing/mod.rs:7-23 (+ the bounce count, Russian roulette start
+    and guiding switch that the reference fixes at compile time)."""
+    samples: int = 4
+    tmin: float = 0.0001
+    tmax: float = 1000.0
+    restir_temporal: RestirTemporalMode = RestirTemporalMode.PerPixel
+    bounces: int = 5
+    rr_start: Optional[int] = None
+    guiding: Guiding = Guiding.MARKOV_CHAIN
+
+
+class Pipeline:
+    """The ray-tracing pipeline (lib.rs:331-376 create_pipeline result)."""
+
+    def __init__(self, quality: Quality, opts: PathTracerOpts):
+        c = _TracerOptsC()
+        c.quality = int(quality)
+        c.samples = int(opts.samples)
+        c.t_min = float(opts.tmin)
+        c.t_max = float(opts.tmax)
+        c.restir = int(opts.restir_temporal)
+        c.bounces = int(opts.bounces)
+        c.rr_start = 0xFFFFFFFF if opts.rr_start is None else int(opts.rr_start)
+        c.guiding = int(opts.guiding)
+        h = _vp()
+        _check(_lib.lmb_tracer_create(ctypes.byref(c), ctypes.byref(h)))
+        self._h = h
+        self.quality = quality
+        self.opts = opts
+
+    def dispatch(self, scene: Scene, camera: Camera, buffers: "DataBuffers", seed_offset: int = 0,
+                 stream=None) -> None:
+        """general.wesl:506 rt_main over dispatch_size(w, h) with the
+        seed_offset immediate (examples/cornell_box/src/main.rs:723-731)."""
+        _check(_lib.lmb_trace(self._h, scene.handle, ctypes.byref(camera._c), buffers.handle,
+                              int(seed_offset) & 0xFFFFFFFF, stream))
+
+    def close(self):
+        if self._h:
+            _lib.lmb_tracer_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RayTracer:
+    """lib.rs:260-329 RayTracer<S>; `RayTracer(Medium).create_pipeline(opts)`."""
+
+    def __init__(self, quality: Quality = Quality.MEDIUM):
+        self.quality = Quality(quality)
+
+    def create_pipeline(self, opts: Optional[PathTracerOpts] = None) -> Pipeline:
+        return Pipeline(self.quality, opts or PathTracerOpts())
+
+
+class BufferType(enum.IntFlag):
+    """data_buffer/mod.rs:18-27."""
+    MARKOV_CHAIN_SCREEN_SPACE = 1 << 0
+    SPATIAL_RESAMPLING = 1 << 1
+    MARKOV_CHAIN_WORLD_SPACE = 1 << 2
+
+    @classmethod
+    def all(cls) -> "BufferType":
+        return cls(7)
+
+
+@dataclass(frozen=True)
+class AdvanceOptions:
+    """data_buffer/mod.rs:442-472."""
+    confidence_cap: int
+
+
+AdvanceOptions.RESPONSIVE = AdvanceOptions(2)
+AdvanceOptions.SEMI_INTERACTIVE = AdvanceOptions(30)
+AdvanceOptions.SLOW = AdvanceOptions(100)
+
+
+class StateBuffer(enum.IntEnum):
+    GI_RESERVOIRS = 0
+    OLD_GI_RESERVOIRS = 1
+    INFO = 2
+    MARKOV_CHAINS = 3
+    OLD_MARKOV_CHAINS = 4
+    WORLD_MARKOV_CHAINS = 5
+    OLD_WORLD_MARKOV_CHAINS = 6
+
+
+class Output(enum.IntEnum):
+    COLOR = 0
+    NORMAL = 1
+    ALBEDO = 2
+
+
+class DataBuffers:
+    """data_buffer/mod.rs:103-440 DataBuffers (plus the output images)."""
+
+    def __init__(self, width: int, height: int, world_space_buffer_size: int = 1,
+                 buffers: BufferType = BufferType.all()):
+        h = _vp()
+        _check(_lib.lmb_data_buffers_create(width, height, world_space_buffer_size, int(buffers),
+                                            ctypes.byref(h)))
+        self._h = h
+        self.width, self.height = max(width, 1), max(height, 1)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def resize(self, width: int, height: int) -> None:
+        _check(_lib.lmb_data_buffers_resize(self._h, width, height))
+        self.width, self.height = max(width, 1), max(height, 1)
+
+    def advance_frame(self, buffers_to_temporally_advance: BufferType,
+                      options: AdvanceOptions = AdvanceOptions.SEMI_INTERACTIVE,
+                      stream=None) -> None:
+        _check(_lib.lmb_advance_frame(self._h, int(buffers_to_temporally_advance),
+                                      int(options.confidence_cap), stream))
+
+    def temporal_resample(self, seed_offset: int = 0, stream=None) -> None:
+        _check(_lib.lmb_temporal_resample(self._h, int(seed_offset) & 0xFFFFFFFF, stream))
+
+    def spatial_resample(self, is_samples: int = 8, is_space: int = 61, seed_offset: int = 0,
+                         stream=None) -> None:
+        _check(_lib.lmb_spatial_resample(self._h, is_samples, is_space,
+                                         int(seed_offset) & 0xFFFFFFFF, stream))
+
+    def output(self, which: Output = Output.COLOR) -> np.ndarray:
+        out = np.empty((self.height, self.width, 4), np.float32)
+        _check(_lib.lmb_data_buffers_read_output(self._h, int(which), _fptr(out)))
+        return out
+
+    def output_device_ptr(self, which: Output = Output.COLOR) -> tuple[int, int]:
+        p, n = _vp(), ctypes.c_uint64()
+        _check(_lib.lmb_data_buffers_output(self._h, int(which), ctypes.byref(p), ctypes.byref(n)))
+        return p.value, n.value
+
+    def state_bytes(self, which: StateBuffer) -> bytes:
+        n = ctypes.c_uint64()
+        _check(_lib.lmb_data_buffers_state_size(self._h, int(which), ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(_lib.lmb_data_buffers_read_state(self._h, int(which), buf, n.value))
+        return buf.raw
+
+    def write_state(self, which: StateBuffer, data: bytes) -> None:
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        _check(_lib.lmb_data_buffers_write_state(self._h, int(which), buf, len(data)))
+
+    def path_vertices(self) -> int:
+        n = ctypes.c_uint64()
+        _check(_lib.lmb_data_buffers_path_vertices(self._h, ctypes.byref(n)))
+        return n.value
+
+    def close(self):
+        if self._h:
+            _lib.lmb_data_buffers_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

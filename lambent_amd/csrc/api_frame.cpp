@@ -1,0 +1,309 @@
+// api_frame.cpp - C ABI: tracer options, per-frame buffers (DataBuffers) and
+// the frame passes.
+#include <hip/hip_runtime_api.h>
+
+#include <cstring>
+#include <string>
+
+#include "../../include/lambent.h"
+#include "lmb_frame_dev.h"
+#include "lmb_internal.h"
+
+using namespace lmb;
+
+struct lmb_tracer_s {
+    lmb_tracer_opts opts;
+};
+
+struct lmb_data_buffers_s {
+    int device = 0;
+    uint32_t width = 0, height = 0, world_size = 0, flags = 0;
+    DeviceBuffer out_color, out_normal, out_albedo;
+    DeviceBuffer own_seed, pix_sum, ps_seed, ps_through, ps_accum;
+    DeviceBuffer queue0, queue1, hits, counters;
+    DeviceBuffer gi, gi_old, info, mc, mc_old, wmc, wmc_old;
+    SpillBuffer spill;
+    uint64_t gi_count = 0, mc_count = 0, wmc_count = 0;
+    FrameDev dev{};
+};
+
+namespace {
+
+lmb_status alloc_zero(DeviceBuffer& b, size_t bytes) {
+    lmb_status st = b.alloc(bytes);
+    if (st) return st;
+    hipError_t e = hipMemset(b.ptr, 0, bytes);
+    if (e != hipSuccess) return fail_hip(e, "hipMemset");
+    return LMB_OK;
+}
+
+// data_buffer/mod.rs:116-148 create_window_required_buffers (+ the output
+// images and wavefront queues that the caller owns in the reference).
+lmb_status alloc_window(lmb_data_buffers b, uint32_t width, uint32_t height) {
+    width = width ? width : 1;
+    height = height ? height : 1;
+    const uint64_t n = (uint64_t)width * height;
+    lmb_status st;
+    if ((st = alloc_zero(b->out_color, n * 16))) return st;
+    if ((st = alloc_zero(b->out_normal, n * 16))) return st;
+    if ((st = alloc_zero(b->out_albedo, n * 16))) return st;
+    if ((st = alloc_zero(b->own_seed, n * 4))) return st;
+    if ((st = alloc_zero(b->pix_sum, n * 16))) return st;
+    if ((st = alloc_zero(b->ps_seed, n * 4))) return st;
+    if ((st = alloc_zero(b->ps_through, n * 16))) return st;
+    if ((st = alloc_zero(b->ps_accum, n * 16))) return st;
+    if ((st = alloc_zero(b->queue0, n * 32))) return st;
+    if ((st = alloc_zero(b->queue1, n * 32))) return st;
+    if ((st = alloc_zero(b->hits, n * 16))) return st;
+    if ((st = alloc_zero(b->counters, kNumCounters * 4))) return st;
+    b->gi_count = (b->flags & LMB_BUFFER_SPATIAL_RESAMPLING) ? n : 1;
+    b->mc_count = (b->flags & LMB_BUFFER_MARKOV_CHAIN_SCREEN_SPACE) ? n : 1;
+    if ((st = alloc_zero(b->gi, b->gi_count * sizeof(Reservoir)))) return st;
+    if ((st = alloc_zero(b->gi_old, b->gi_count * sizeof(Reservoir)))) return st;
+    if ((st = alloc_zero(b->info, n * sizeof(CompressedInfo)))) return st;
+    if ((st = alloc_zero(b->mc, b->mc_count * sizeof(MarkovChain)))) return st;
+    if ((st = alloc_zero(b->mc_old, b->mc_count * sizeof(MarkovChain)))) return st;
+    if ((st = ensure_spill(&b->spill, frame_threads(n)))) return st;
+    b->width = width;
+    b->height = height;
+    return LMB_OK;
+}
+
+void refresh_view(lmb_data_buffers b) {
+    FrameDev& f = b->dev;
+    f.width = b->width;
+    f.height = b->height;
+    f.npix = (uint64_t)b->width * b->height;
+    f.out_color = b->out_color.as<float4>();
+    f.out_normal = b->out_normal.as<float4>();
+    f.out_albedo = b->out_albedo.as<float4>();
+    f.own_seed = b->own_seed.as<uint32_t>();
+    f.pix_sum = b->pix_sum.as<float4>();
+    f.ps_seed = b->ps_seed.as<uint32_t>();
+    f.ps_through = b->ps_through.as<float4>();
+    f.ps_accum = b->ps_accum.as<float4>();
+    f.queue[0] = b->queue0.as<float4>();
+    f.queue[1] = b->queue1.as<float4>();
+    f.hits = b->hits.as<uint4>();
+    f.counters = b->counters.as<uint32_t>();
+    f.spill = b->spill.ptr;
+    f.gi = b->gi.as<Reservoir>();
+    f.gi_old = b->gi_old.as<Reservoir>();
+    f.info = b->info.as<CompressedInfo>();
+    f.mc = b->mc.as<MarkovChain>();
+    f.mc_old = b->mc_old.as<MarkovChain>();
+    f.wmc = b->wmc.as<WorldMarkovChain>();
+    f.wmc_old = b->wmc_old.as<WorldMarkovChain>();
+    f.world_size = (uint32_t)b->wmc_count;
+    f.flags = b->flags;
+}
+
+DeviceBuffer* state_buffer(lmb_data_buffers b, uint32_t which) {
+    switch (which) {
+        case 0: return &b->gi;
+        case 1: return &b->gi_old;
+        case 2: return &b->info;
+        case 3: return &b->mc;
+        case 4: return &b->mc_old;
+        case 5: return &b->wmc;
+        case 6: return &b->wmc_old;
+        default: return nullptr;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void lmb_tracer_opts_default(lmb_tracer_opts* o) {
+    if (!o) return;
+    o->quality = LMB_QUALITY_MEDIUM;
+    o->samples = 4;      // low_level.rs:101
+    o->t_min = 0.0001f;  // low_level.rs:102
+    o->t_max = 1000.0f;  // low_level.rs:103
+    o->restir = LMB_RESTIR_PER_PIXEL;  // path_tracing/mod.rs:32
+    o->bounces = 5;      // general.wesl:50
+    o->rr_start = 0xFFFFFFFFu;
+    o->guiding = LMB_GUIDING_MARKOV_CHAIN;
+}
+
+lmb_status lmb_tracer_create(const lmb_tracer_opts* opts, lmb_tracer* out) {
+    if (!opts || !out) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (opts->quality > LMB_QUALITY_HIGH) return fail(LMB_ERR_INVALID_ARGUMENT, "bad quality");
+    if (opts->samples == 0)  // NonZeroU32 in the reference
+        return fail(LMB_ERR_INVALID_ARGUMENT, "samples must be non-zero");
+    if (opts->bounces == 0 || opts->bounces > kMaxBounces)
+        return fail(LMB_ERR_INVALID_ARGUMENT,
+                    "bounces must be in 1.." + std::to_string(kMaxBounces));
+    if (opts->restir > LMB_RESTIR_PER_PIXEL) return fail(LMB_ERR_INVALID_ARGUMENT, "bad restir");
+    if (opts->guiding > LMB_GUIDING_MARKOV_CHAIN)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "bad guiding mode");
+    if (!(opts->t_min >= 0.0f) || !(opts->t_max > opts->t_min))
+        return fail(LMB_ERR_INVALID_ARGUMENT, "need 0 <= t_min < t_max");
+    lmb_tracer t = new lmb_tracer_s();
+    t->opts = *opts;
+    *out = t;
+    return LMB_OK;
+}
+
+lmb_status lmb_tracer_destroy(lmb_tracer tracer) {
+    delete tracer;
+    return LMB_OK;
+}
+
+lmb_status lmb_data_buffers_create(uint32_t width, uint32_t height, uint32_t world_size,
+                                   uint32_t buffer_flags, lmb_data_buffers* out) {
+    if (!out) return fail(LMB_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (buffer_flags & ~LMB_BUFFER_ALL) return fail(LMB_ERR_INVALID_ARGUMENT, "bad buffer flags");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return fail(LMB_ERR_DEVICE, "no HIP device available");
+    }
+    lmb_data_buffers b = new lmb_data_buffers_s();
+    (void)hipGetDevice(&b->device);
+    b->flags = buffer_flags;
+    b->world_size = world_size;
+    // data_buffer/mod.rs:162-167: world buffers hold max(world_size, 1)
+    // entries when requested, else 1.
+    b->wmc_count = (buffer_flags & LMB_BUFFER_MARKOV_CHAIN_WORLD_SPACE)
+                       ? (world_size ? world_size : 1)
+                       : 1;
+    lmb_status st = alloc_zero(b->wmc, b->wmc_count * sizeof(WorldMarkovChain));
+    if (!st) st = alloc_zero(b->wmc_old, b->wmc_count * sizeof(WorldMarkovChain));
+    if (!st) st = alloc_window(b, width, height);
+    if (st) {
+        delete b;
+        return st;
+    }
+    refresh_view(b);
+    *out = b;
+    return LMB_OK;
+}
+
+lmb_status lmb_data_buffers_resize(lmb_data_buffers b, uint32_t width, uint32_t height) {
+    if (!b) return fail(LMB_ERR_INVALID_ARGUMENT, "buffers is NULL");
+    lmb_status st = alloc_window(b, width, height);
+    refresh_view(b);
+    return st;
+}
+
+lmb_status lmb_data_buffers_destroy(lmb_data_buffers b) {
+    delete b;
+    return LMB_OK;
+}
+
+lmb_status lmb_data_buffers_output(lmb_data_buffers b, uint32_t which, void** dev_ptr,
+                                   uint64_t* bytes) {
+    if (!b || !dev_ptr) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    DeviceBuffer* buf = which == LMB_OUTPUT_COLOR    ? &b->out_color
+                        : which == LMB_OUTPUT_NORMAL ? &b->out_normal
+                        : which == LMB_OUTPUT_ALBEDO ? &b->out_albedo
+                                                     : nullptr;
+    if (!buf) return fail(LMB_ERR_INVALID_ARGUMENT, "bad output index");
+    *dev_ptr = buf->ptr;
+    if (bytes) *bytes = buf->bytes;
+    return LMB_OK;
+}
+
+lmb_status lmb_data_buffers_read_output(lmb_data_buffers b, uint32_t which, float* host) {
+    void* ptr = nullptr;
+    uint64_t bytes = 0;
+    lmb_status st = lmb_data_buffers_output(b, which, &ptr, &bytes);
+    if (st) return st;
+    if (!host) return fail(LMB_ERR_INVALID_ARGUMENT, "host is NULL");
+    hipError_t e = hipMemcpy(host, ptr, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy output");
+    return LMB_OK;
+}
+
+lmb_status lmb_data_buffers_state_size(lmb_data_buffers b, uint32_t which, uint64_t* bytes) {
+    if (!b || !bytes) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    DeviceBuffer* buf = state_buffer(b, which);
+    if (!buf) return fail(LMB_ERR_INVALID_ARGUMENT, "bad state index");
+    *bytes = buf->bytes;
+    return LMB_OK;
+}
+
+lmb_status lmb_data_buffers_read_state(lmb_data_buffers b, uint32_t which, void* host,
+                                       uint64_t bytes) {
+    if (!b || !host) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    DeviceBuffer* buf = state_buffer(b, which);
+    if (!buf) return fail(LMB_ERR_INVALID_ARGUMENT, "bad state index");
+    if (bytes != buf->bytes) return fail(LMB_ERR_INVALID_ARGUMENT, "size mismatch");
+    hipError_t e = hipMemcpy(host, buf->ptr, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy state");
+    return LMB_OK;
+}
+
+lmb_status lmb_data_buffers_write_state(lmb_data_buffers b, uint32_t which, const void* host,
+                                        uint64_t bytes) {
+    if (!b || !host) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    DeviceBuffer* buf = state_buffer(b, which);
+    if (!buf) return fail(LMB_ERR_INVALID_ARGUMENT, "bad state index");
+    if (bytes != buf->bytes) return fail(LMB_ERR_INVALID_ARGUMENT, "size mismatch");
+    hipError_t e = hipMemcpy(buf->ptr, host, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy state");
+    return LMB_OK;
+}
+
+lmb_status lmb_data_buffers_path_vertices(lmb_data_buffers b, uint64_t* vertices) {
+    if (!b || !vertices) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    hipError_t e = hipMemcpy(vertices, b->dev.counters + kCounterVertices, 8,
+                             hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy counter");
+    return LMB_OK;
+}
+
+lmb_status lmb_trace(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
+                     lmb_data_buffers bufs, uint32_t seed_offset, void* stream) {
+    if (!tracer || !scene || !cam || !bufs) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    const lmb_tracer_opts& o = tracer->opts;
+    if (o.guiding != LMB_GUIDING_OFF)
+        return fail(LMB_ERR_INVALID_ARGUMENT,
+                    "Markov-chain guiding is not available in this build; use LMB_GUIDING_OFF");
+    TraceParams p;
+    p.quality = o.quality;
+    p.samples = o.samples;
+    p.tmin = o.t_min;
+    p.tmax = o.t_max;
+    p.restir = o.restir;
+    p.bounces = o.bounces;
+    p.rr_start = o.rr_start;
+    p.guiding = o.guiding;
+    p.seed_offset = seed_offset;
+    CameraDev cd;
+    std::memcpy(&cd, cam, sizeof cd);
+    hipError_t e = launch_trace(scene->dev, cd, bufs->dev, p, (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "trace launch");
+    return LMB_OK;
+}
+
+lmb_status lmb_temporal_resample(lmb_data_buffers bufs, uint32_t seed_offset, void* stream) {
+    (void)seed_offset;
+    (void)stream;
+    if (!bufs) return fail(LMB_ERR_INVALID_ARGUMENT, "buffers is NULL");
+    return fail(LMB_ERR_INVALID_ARGUMENT, "temporal resampling is not available in this build");
+}
+
+lmb_status lmb_spatial_resample(lmb_data_buffers bufs, uint32_t is_samples, uint32_t is_space,
+                                uint32_t seed_offset, void* stream) {
+    (void)is_samples;
+    (void)is_space;
+    (void)seed_offset;
+    (void)stream;
+    if (!bufs) return fail(LMB_ERR_INVALID_ARGUMENT, "buffers is NULL");
+    return fail(LMB_ERR_INVALID_ARGUMENT, "spatial resampling is not available in this build");
+}
+
+lmb_status lmb_advance_frame(lmb_data_buffers bufs, uint32_t buffers_to_advance,
+                             uint32_t confidence_cap, void* stream) {
+    (void)buffers_to_advance;
+    (void)confidence_cap;
+    (void)stream;
+    if (!bufs) return fail(LMB_ERR_INVALID_ARGUMENT, "buffers is NULL");
+    return fail(LMB_ERR_INVALID_ARGUMENT, "advance_frame is not available in this build");
+}
+
+}  // extern "C"

@@ -1,0 +1,420 @@
+// bvh_build.cpp - host builder of the compressed 8-wide BVH that replaces the
+// driver-built BLAS of the reference (examples/cornell_box/src/main.rs:464-563;
+// the reference's ray queries traverse it at general.wesl:1052-1078).
+//
+// 1. Binned-SAH binary BVH (16 bins per axis, leaves of <= 3 triangles),
+//    subtrees above 64k primitives built on separate threads.
+// 2. Collapse to 8-wide: repeatedly open the child of largest surface area
+//    until the node has 8 children or only leaves remain.
+// 3. Slot assignment by direction octant (greedy maximum alignment of the
+//    child centroid offset with the slot's octant direction).
+// 4. 8-bit quantization of the child boxes against a per-node origin that is
+//    a multiple of the power-of-two scale, so that the dequantized planes
+//    fmaf(q, scale, p) are exact and enclose the true boxes.
+#include "bvh_build.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <future>
+#include <limits>
+#include <string>
+#include <vector>
+
+namespace lmb {
+
+namespace {
+
+struct PrimRef {
+    float lo[3];
+    float hi[3];
+    float c[3];
+    uint32_t id;
+};
+
+struct BNode {
+    float lo[3];
+    float hi[3];
+    uint32_t left;   // index of the left child; right = left + 1
+    uint32_t first;  // leaf: first PrimRef
+    uint32_t count;  // leaf: number of primitives (0 for internal nodes)
+};
+
+constexpr int kBins = 16;
+constexpr uint32_t kMaxLeaf = 3;
+constexpr uint32_t kParallelThreshold = 1u << 16;
+
+inline float half_area(const float lo[3], const float hi[3]) {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    if (dx < 0.f || dy < 0.f || dz < 0.f) return 0.f;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+struct Box {
+    float lo[3], hi[3];
+    void reset() {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::numeric_limits<float>::infinity();
+            hi[a] = -std::numeric_limits<float>::infinity();
+        }
+    }
+    void grow(const float l[3], const float h[3]) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], l[a]);
+            hi[a] = std::max(hi[a], h[a]);
+        }
+    }
+    void grow_point(const float p[3]) { grow(p, p); }
+};
+
+class BinaryBuilder {
+  public:
+    BinaryBuilder(std::vector<PrimRef>& refs) : refs_(refs) {
+        nodes_.resize(std::max<size_t>(1, 2 * refs.size()));
+        next_.store(1);
+    }
+
+    void build() { build_node(0, 0, (uint32_t)refs_.size()); }
+    uint32_t node_count() const { return next_.load(); }
+    std::vector<BNode>& nodes() { return nodes_; }
+
+  private:
+    void build_node(uint32_t ni, uint32_t first, uint32_t count) {
+        Box b, cb;
+        b.reset();
+        cb.reset();
+        for (uint32_t i = first; i < first + count; ++i) {
+            b.grow(refs_[i].lo, refs_[i].hi);
+            cb.grow_point(refs_[i].c);
+        }
+        BNode& n = nodes_[ni];
+        std::memcpy(n.lo, b.lo, sizeof n.lo);
+        std::memcpy(n.hi, b.hi, sizeof n.hi);
+        if (count <= kMaxLeaf) {
+            n.first = first;
+            n.count = count;
+            n.left = 0;
+            return;
+        }
+        // Binned SAH over the three axes.
+        float best_cost = std::numeric_limits<float>::infinity();
+        int best_axis = -1, best_split = -1;
+        for (int a = 0; a < 3; ++a) {
+            float ext = cb.hi[a] - cb.lo[a];
+            if (!(ext > 0.f)) continue;
+            float scale = kBins / ext;
+            uint32_t cnt[kBins] = {};
+            Box bb[kBins];
+            for (auto& x : bb) x.reset();
+            for (uint32_t i = first; i < first + count; ++i) {
+                int k = (int)((refs_[i].c[a] - cb.lo[a]) * scale);
+                k = std::min(std::max(k, 0), kBins - 1);
+                cnt[k]++;
+                bb[k].grow(refs_[i].lo, refs_[i].hi);
+            }
+            float right_area[kBins];
+            uint32_t right_cnt[kBins];
+            Box acc;
+            acc.reset();
+            uint32_t c = 0;
+            for (int k = kBins - 1; k > 0; --k) {
+                acc.grow(bb[k].lo, bb[k].hi);
+                c += cnt[k];
+                right_area[k] = half_area(acc.lo, acc.hi);
+                right_cnt[k] = c;
+            }
+            acc.reset();
+            c = 0;
+            for (int k = 0; k < kBins - 1; ++k) {
+                acc.grow(bb[k].lo, bb[k].hi);
+                c += cnt[k];
+                if (c == 0 || right_cnt[k + 1] == 0) continue;
+                float cost = half_area(acc.lo, acc.hi) * (float)c +
+                             right_area[k + 1] * (float)right_cnt[k + 1];
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = a;
+                    best_split = k + 1;
+                }
+            }
+        }
+        uint32_t mid = first + count / 2;
+        if (best_axis >= 0) {
+            float ext = cb.hi[best_axis] - cb.lo[best_axis];
+            float scale = kBins / ext;
+            float lo = cb.lo[best_axis];
+            auto it = std::partition(
+                refs_.begin() + first, refs_.begin() + first + count,
+                [&](const PrimRef& r) {
+                    int k = (int)((r.c[best_axis] - lo) * scale);
+                    k = std::min(std::max(k, 0), kBins - 1);
+                    return k < best_split;
+                });
+            mid = (uint32_t)(it - refs_.begin());
+            if (mid == first || mid == first + count) mid = first + count / 2;
+        } else {
+            // All centroids coincide: split by position in the list.
+            mid = first + count / 2;
+        }
+        uint32_t left = next_.fetch_add(2);
+        n.left = left;
+        n.first = 0;
+        n.count = 0;
+        uint32_t lc = mid - first, rc = first + count - mid;
+        if (count >= kParallelThreshold) {
+            auto fut = std::async(std::launch::async,
+                                  [=] { build_node(left, first, lc); });
+            build_node(left + 1, mid, rc);
+            fut.get();
+        } else {
+            build_node(left, first, lc);
+            build_node(left + 1, mid, rc);
+        }
+    }
+
+    std::vector<PrimRef>& refs_;
+    std::vector<BNode> nodes_;
+    std::atomic<uint32_t> next_;
+};
+
+inline bool finite3(const float* v) {
+    return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
+}
+
+// Quantization scale exponent for one axis of a node covering [lo, hi]:
+// 2^e >= extent / 254 so that q_hi <= 255, and 2^e >= max|coord| / 2^22 so
+// that every dequantized plane (k + q) * 2^e is an exact float.
+int pick_exponent(float lo, float hi) {
+    double extent = (double)hi - (double)lo;
+    double need = std::max(extent / 254.0,
+                           std::max(std::fabs((double)lo), std::fabs((double)hi)) /
+                               4194304.0);
+    need = std::max(need, 1e-30);
+    int e = (int)std::ceil(std::log2(need));
+    while (std::ldexp(1.0, e) < need) ++e;
+    e = std::max(e, -126);
+    e = std::min(e, 127);
+    return e;
+}
+
+}  // namespace
+
+int build_bvh8(const float* vertices, uint64_t num_vertices, const uint32_t* indices,
+               uint64_t num_triangles, BvhBuildResult* out, std::string* err) {
+    auto t0 = std::chrono::steady_clock::now();
+    out->nodes.clear();
+    out->tris.clear();
+    out->max_depth = 0;
+    out->sah_cost = 0.0;
+
+    // Primitive references (non-finite triangles can never be hit and are
+    // left out).
+    std::vector<PrimRef> refs;
+    refs.reserve(num_triangles);
+    for (uint64_t t = 0; t < num_triangles; ++t) {
+        uint64_t i0 = indices ? indices[3 * t + 0] : 3 * t + 0;
+        uint64_t i1 = indices ? indices[3 * t + 1] : 3 * t + 1;
+        uint64_t i2 = indices ? indices[3 * t + 2] : 3 * t + 2;
+        if (i0 >= num_vertices || i1 >= num_vertices || i2 >= num_vertices) {
+            *err = "triangle " + std::to_string(t) + " indexes a vertex out of range";
+            return -1;
+        }
+        const float* a = vertices + 4 * i0;
+        const float* b = vertices + 4 * i1;
+        const float* c = vertices + 4 * i2;
+        if (!finite3(a) || !finite3(b) || !finite3(c)) continue;
+        PrimRef r;
+        for (int k = 0; k < 3; ++k) {
+            r.lo[k] = std::min(a[k], std::min(b[k], c[k]));
+            r.hi[k] = std::max(a[k], std::max(b[k], c[k]));
+            r.c[k] = 0.5f * r.lo[k] + 0.5f * r.hi[k];
+        }
+        r.id = (uint32_t)t;
+        refs.push_back(r);
+    }
+    if (refs.empty()) {
+        out->build_ms = std::chrono::duration<double, std::milli>(
+                            std::chrono::steady_clock::now() - t0)
+                            .count();
+        return 0;
+    }
+
+    BinaryBuilder bb(refs);
+    bb.build();
+    std::vector<BNode>& bn = bb.nodes();
+
+    // Collapse into 8-wide nodes, breadth first so that the internal children
+    // of every node are contiguous.
+    struct Pending {
+        uint32_t bnode;
+        uint32_t node8;
+        uint32_t depth;
+    };
+    std::vector<Pending> queue;
+    queue.push_back({0, 0, 1});
+    out->nodes.resize(1);
+    double root_area = std::max(1e-30, (double)half_area(bn[0].lo, bn[0].hi));
+    double sah = 0.0;
+    size_t head = 0;
+    while (head < queue.size()) {
+        Pending pd = queue[head++];
+        out->max_depth = std::max(out->max_depth, pd.depth);
+        const BNode& b = bn[pd.bnode];
+        sah += 1.0 * half_area(b.lo, b.hi) / root_area;
+
+        uint32_t ch[8];
+        int nch = 0;
+        if (b.count > 0) {
+            ch[nch++] = pd.bnode;  // only a leaf root gets here
+        } else {
+            ch[nch++] = b.left;
+            ch[nch++] = b.left + 1;
+            while (nch < 8) {
+                int pick = -1;
+                float best = -1.f;
+                for (int i = 0; i < nch; ++i) {
+                    const BNode& c = bn[ch[i]];
+                    if (c.count > 0) continue;
+                    float a = half_area(c.lo, c.hi);
+                    if (a > best) {
+                        best = a;
+                        pick = i;
+                    }
+                }
+                if (pick < 0) break;
+                uint32_t l = bn[ch[pick]].left;
+                ch[pick] = l;
+                ch[nch++] = l + 1;
+            }
+        }
+
+        // Octant slot assignment.
+        float center[3];
+        for (int a = 0; a < 3; ++a) center[a] = 0.5f * (b.lo[a] + b.hi[a]);
+        struct Cand {
+            float score;
+            int child, slot;
+        };
+        std::vector<Cand> cands;
+        cands.reserve(64);
+        for (int i = 0; i < nch; ++i) {
+            const BNode& c = bn[ch[i]];
+            float d[3];
+            for (int a = 0; a < 3; ++a) d[a] = 0.5f * (c.lo[a] + c.hi[a]) - center[a];
+            for (int s = 0; s < 8; ++s) {
+                float sc = ((s & 1) ? d[0] : -d[0]) + ((s & 2) ? d[1] : -d[1]) +
+                           ((s & 4) ? d[2] : -d[2]);
+                cands.push_back({sc, i, s});
+            }
+        }
+        std::stable_sort(cands.begin(), cands.end(),
+                         [](const Cand& x, const Cand& y) { return x.score > y.score; });
+        int slot_of[8];
+        int child_in_slot[8];
+        for (int i = 0; i < 8; ++i) {
+            slot_of[i] = -1;
+            child_in_slot[i] = -1;
+        }
+        for (const Cand& c : cands) {
+            if (slot_of[c.child] >= 0 || child_in_slot[c.slot] >= 0) continue;
+            slot_of[c.child] = c.slot;
+            child_in_slot[c.slot] = c.child;
+        }
+
+        Bvh8Node node;
+        std::memset(&node, 0, sizeof node);
+        // Quantization frame.
+        float scale[3];
+        for (int a = 0; a < 3; ++a) {
+            int e = pick_exponent(b.lo[a], b.hi[a]);
+            double s = std::ldexp(1.0, e);
+            double p = std::floor((double)b.lo[a] / s) * s;
+            node.p[a] = (float)p;
+            node.e[a] = (uint8_t)(e + 127);
+            scale[a] = (float)s;
+        }
+        // Children in slot order: internal nodes get consecutive indices,
+        // leaf triangles consecutive positions.
+        uint32_t child_base = (uint32_t)out->nodes.size();
+        uint32_t tri_base = (uint32_t)out->tris.size();
+        node.child_base = child_base;
+        node.tri_base = tri_base;
+        uint32_t n_internal = 0, tri_off = 0;
+        for (int s = 0; s < 8; ++s) {
+            int ci = child_in_slot[s];
+            if (ci < 0) {
+                node.meta[s] = kMetaEmpty;
+                for (int a = 0; a < 3; ++a) {
+                    node.qlo[a][s] = 0;
+                    node.qhi[a][s] = 0;
+                }
+                continue;
+            }
+            const BNode& c = bn[ch[ci]];
+            for (int a = 0; a < 3; ++a) {
+                double s_a = scale[a];
+                double ql = std::floor(((double)c.lo[a] - (double)node.p[a]) / s_a);
+                double qh = std::ceil(((double)c.hi[a] - (double)node.p[a]) / s_a);
+                ql = std::min(std::max(ql, 0.0), 255.0);
+                qh = std::min(std::max(qh, 0.0), 255.0);
+                float dl = std::fmaf((float)ql, scale[a], node.p[a]);
+                float dh = std::fmaf((float)qh, scale[a], node.p[a]);
+                if (!(dl <= c.lo[a]) || !(dh >= c.hi[a])) {
+                    *err = "BVH quantization is not conservative (internal error)";
+                    return -1;
+                }
+                node.qlo[a][s] = (uint8_t)ql;
+                node.qhi[a][s] = (uint8_t)qh;
+            }
+            if (c.count == 0) {
+                node.meta[s] = kMetaInternal;
+                node.imask |= (uint8_t)(1u << s);
+                n_internal++;
+            } else {
+                uint32_t unary = (1u << c.count) - 1u;
+                node.meta[s] = (uint8_t)((unary << 5) | tri_off);
+                for (uint32_t k = 0; k < c.count; ++k) {
+                    const PrimRef& r = refs[c.first + k];
+                    uint64_t t = r.id;
+                    uint64_t i0 = indices ? indices[3 * t + 0] : 3 * t + 0;
+                    uint64_t i1 = indices ? indices[3 * t + 1] : 3 * t + 1;
+                    uint64_t i2 = indices ? indices[3 * t + 2] : 3 * t + 2;
+                    LeafTri lt;
+                    for (int q = 0; q < 3; ++q) {
+                        lt.v0[q] = vertices[4 * i0 + q];
+                        lt.v1[q] = vertices[4 * i1 + q];
+                        lt.v2[q] = vertices[4 * i2 + q];
+                    }
+                    uint32_t id = r.id;
+                    std::memcpy(&lt.v0[3], &id, 4);
+                    lt.v1[3] = 0.f;
+                    lt.v2[3] = 0.f;
+                    out->tris.push_back(lt);
+                }
+                tri_off += c.count;
+            }
+        }
+        if (tri_off > 24) {
+            *err = "BVH node references more than 24 triangles (internal error)";
+            return -1;
+        }
+        out->nodes.resize(out->nodes.size() + n_internal);
+        uint32_t k = 0;
+        for (int s = 0; s < 8; ++s) {
+            int ci = child_in_slot[s];
+            if (ci < 0 || node.meta[s] != kMetaInternal) continue;
+            queue.push_back({ch[ci], child_base + k, pd.depth + 1});
+            ++k;
+        }
+        out->nodes[pd.node8] = node;
+    }
+    out->sah_cost = sah;
+    out->build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+            .count();
+    return 0;
+}
+
+}  // namespace lmb

@@ -1,0 +1,259 @@
+// dev_traverse.h - gfx950 device code for closest-hit ray queries over the
+// compressed 8-wide BVH (replaces the reference's hardware ray query,
+// general.wesl:1052-1078, whose semantics are: closest hit with parametric
+// t in [T_MIN, T_MAX], barycentrics (u, v) weighting vertices 1 and 2,
+// opaque geometry auto-committed).
+//
+// All code here is compiled with -ffp-contract=off; every fused multiply-add
+// is an explicit fmaf, so the arithmetic is identical on any IEEE target.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lmb_layout.h"
+
+namespace lmb {
+
+struct Ray {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+};
+
+struct Hit {
+    uint32_t prim;  // original primitive index (kMissPrim = no hit)
+    uint32_t leaf;  // position of the triangle in leaf order
+    float t, u, v;
+};
+
+// Conservative widening of the far slab distance (1 + 2*gamma(3), Ize 2013,
+// "Robust BVH Ray Traversal").
+constexpr float kRobustFar = 1.0000004f;
+constexpr float kZeroDirInv = 1e30f;
+
+__device__ __forceinline__ float comp3(float x, float y, float z, int k) {
+    return k == 0 ? x : (k == 1 ? y : z);
+}
+
+// Per-ray constants of the watertight test (Woop, Benthin, Wald 2013,
+// "Watertight Ray/Triangle Intersection", JCGT 2(1)).
+struct Shear {
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+
+__device__ __forceinline__ Shear make_shear(float dx, float dy, float dz) {
+    float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    Shear s;
+    s.kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
+    s.kx = s.kz + 1;
+    if (s.kx == 3) s.kx = 0;
+    s.ky = s.kx + 1;
+    if (s.ky == 3) s.ky = 0;
+    float dkz = comp3(dx, dy, dz, s.kz);
+    if (dkz < 0.0f) {
+        int t = s.kx;
+        s.kx = s.ky;
+        s.ky = t;
+    }
+    float dkx = comp3(dx, dy, dz, s.kx);
+    float dky = comp3(dx, dy, dz, s.ky);
+    s.sx = dkx / dkz;
+    s.sy = dky / dkz;
+    s.sz = 1.0f / dkz;
+    return s;
+}
+
+// Two-sided watertight intersection.  Returns true and (t, u, v) when the
+// hit lies in [tmin, tmax]; u, v weight vertices 1 and 2.
+__device__ __forceinline__ bool intersect_triangle(const Shear& sh, float ox, float oy,
+                                                   float oz, const float4 a4,
+                                                   const float4 b4, const float4 c4,
+                                                   float tmin, float tmax, float* t_out,
+                                                   float* u_out, float* v_out) {
+    float Ax0 = a4.x - ox, Ay0 = a4.y - oy, Az0 = a4.z - oz;
+    float Bx0 = b4.x - ox, By0 = b4.y - oy, Bz0 = b4.z - oz;
+    float Cx0 = c4.x - ox, Cy0 = c4.y - oy, Cz0 = c4.z - oz;
+    float Akx = comp3(Ax0, Ay0, Az0, sh.kx), Aky = comp3(Ax0, Ay0, Az0, sh.ky),
+          Akz = comp3(Ax0, Ay0, Az0, sh.kz);
+    float Bkx = comp3(Bx0, By0, Bz0, sh.kx), Bky = comp3(Bx0, By0, Bz0, sh.ky),
+          Bkz = comp3(Bx0, By0, Bz0, sh.kz);
+    float Ckx = comp3(Cx0, Cy0, Cz0, sh.kx), Cky = comp3(Cx0, Cy0, Cz0, sh.ky),
+          Ckz = comp3(Cx0, Cy0, Cz0, sh.kz);
+    float Ax = Akx - sh.sx * Akz;
+    float Ay = Aky - sh.sy * Akz;
+    float Bx = Bkx - sh.sx * Bkz;
+    float By = Bky - sh.sy * Bkz;
+    float Cx = Ckx - sh.sx * Ckz;
+    float Cy = Cky - sh.sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        double CxBy = (double)Cx * (double)By;
+        double CyBx = (double)Cy * (double)Bx;
+        U = (float)(CxBy - CyBx);
+        double AxCy = (double)Ax * (double)Cy;
+        double AyCx = (double)Ay * (double)Cx;
+        V = (float)(AxCy - AyCx);
+        double BxAy = (double)Bx * (double)Ay;
+        double ByAx = (double)By * (double)Ax;
+        W = (float)(BxAy - ByAx);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f))
+        return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = sh.sz * Akz;
+    float Bz = sh.sz * Bkz;
+    float Cz = sh.sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    float rcp = 1.0f / det;
+    float t = T * rcp;
+    if (!(t >= tmin && t <= tmax)) return false;
+    *t_out = t;
+    *u_out = V * rcp;
+    *v_out = W * rcp;
+    return true;
+}
+
+__device__ __forceinline__ uint32_t byte_of(uint32_t lo4, uint32_t hi4, int s) {
+    uint32_t w = s < 4 ? lo4 : hi4;
+    return (w >> ((s & 3) * 8)) & 0xFFu;
+}
+
+// Traversal stack: kStackLds entries per lane in LDS (`lds` points at this
+// lane's column, stride BLOCK), deeper entries in `spill`.
+template <int BLOCK>
+struct ShortStack {
+    uint2* lds;
+    uint2* spill;
+    int sp;
+    __device__ __forceinline__ void push(uint2 e) {
+        uint2* slot = lds + (sp & (kStackLds - 1)) * BLOCK;
+        if (sp >= kStackLds) spill[sp - kStackLds] = *slot;
+        *slot = e;
+        ++sp;
+    }
+    __device__ __forceinline__ uint2 pop() {
+        --sp;
+        uint2* slot = lds + (sp & (kStackLds - 1)) * BLOCK;
+        uint2 e = *slot;
+        if (sp >= kStackLds) *slot = spill[sp - kStackLds];
+        return e;
+    }
+};
+
+// Closest hit over the BVH.  Ties in t resolve to the lower primitive index,
+// so the result does not depend on the traversal order.
+template <int BLOCK>
+__device__ __forceinline__ Hit trace_closest(const Bvh8Node* __restrict__ nodes,
+                                             const LeafTri* __restrict__ tris,
+                                             uint32_t num_nodes, const Ray& r,
+                                             ShortStack<BLOCK>& st) {
+    Hit h;
+    h.prim = kMissPrim;
+    h.leaf = kMissPrim;
+    h.t = r.tmax;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    if (num_nodes == 0) return h;
+    // Near-zero direction components get a large positive inverse: the slab
+    // test then accepts exactly the boxes whose slab contains the origin.
+    const float ix = fabsf(r.dx) < 1e-30f ? kZeroDirInv : 1.0f / r.dx;
+    const float iy = fabsf(r.dy) < 1e-30f ? kZeroDirInv : 1.0f / r.dy;
+    const float iz = fabsf(r.dz) < 1e-30f ? kZeroDirInv : 1.0f / r.dz;
+    const uint32_t oct = (ix < 0.0f ? 1u : 0u) | (iy < 0.0f ? 2u : 0u) | (iz < 0.0f ? 4u : 0u);
+    const Shear sh = make_shear(r.dx, r.dy, r.dz);
+    st.sp = 0;
+    uint32_t node = 0;
+    while (true) {
+        const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
+        const uint4 w0 = np[0];
+        const uint4 w1 = np[1];
+        const uint4 w2 = np[2];
+        const uint4 w3 = np[3];
+        const uint4 w4 = np[4];
+        const float px = __uint_as_float(w0.x);
+        const float py = __uint_as_float(w0.y);
+        const float pz = __uint_as_float(w0.z);
+        const float sx = __uint_as_float((w0.w & 0xFFu) << 23);
+        const float sy = __uint_as_float(((w0.w >> 8) & 0xFFu) << 23);
+        const float sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23);
+        const uint32_t imask = w0.w >> 24;
+        const uint32_t child_base = w1.x;
+        const uint32_t tri_base = w1.y;
+        // near / far plane bytes per axis, chosen by the ray's sign
+        const uint32_t nx0 = (oct & 1u) ? w3.z : w2.x, nx1 = (oct & 1u) ? w3.w : w2.y;
+        const uint32_t fx0 = (oct & 1u) ? w2.x : w3.z, fx1 = (oct & 1u) ? w2.y : w3.w;
+        const uint32_t ny0 = (oct & 2u) ? w4.x : w2.z, ny1 = (oct & 2u) ? w4.y : w2.w;
+        const uint32_t fy0 = (oct & 2u) ? w2.z : w4.x, fy1 = (oct & 2u) ? w2.w : w4.y;
+        const uint32_t nz0 = (oct & 4u) ? w4.z : w3.x, nz1 = (oct & 4u) ? w4.w : w3.y;
+        const uint32_t fz0 = (oct & 4u) ? w3.x : w4.z, fz1 = (oct & 4u) ? w3.y : w4.w;
+        const float best_far = h.t * kRobustFar;
+        uint32_t node_hits = 0, tri_hits = 0;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const uint32_t meta = byte_of(w1.z, w1.w, s);
+            const float qnx = (float)byte_of(nx0, nx1, s);
+            const float qfx = (float)byte_of(fx0, fx1, s);
+            const float qny = (float)byte_of(ny0, ny1, s);
+            const float qfy = (float)byte_of(fy0, fy1, s);
+            const float qnz = (float)byte_of(nz0, nz1, s);
+            const float qfz = (float)byte_of(fz0, fz1, s);
+            const float tnx = (fmaf(qnx, sx, px) - r.ox) * ix;
+            const float tfx = (fmaf(qfx, sx, px) - r.ox) * ix;
+            const float tny = (fmaf(qny, sy, py) - r.oy) * iy;
+            const float tfy = (fmaf(qfy, sy, py) - r.oy) * iy;
+            const float tnz = (fmaf(qnz, sz, pz) - r.oz) * iz;
+            const float tfz = (fmaf(qfz, sz, pz) - r.oz) * iz;
+            const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tmin));
+            const float tf = fminf(fminf(tfx, tfy), tfz) * kRobustFar;
+            const bool hit = meta != kMetaEmpty && tn <= tf && tn <= best_far;
+            if (hit) {
+                if (meta == kMetaInternal) {
+                    node_hits |= 1u << (s ^ oct);
+                } else {
+                    tri_hits |= (meta >> 5) << (meta & 31u);
+                }
+            }
+        }
+        while (tri_hits) {
+            const uint32_t k = __builtin_ctz(tri_hits);
+            tri_hits &= tri_hits - 1u;
+            const float4* tp = reinterpret_cast<const float4*>(tris + tri_base + k);
+            const float4 a = tp[0], b = tp[1], c = tp[2];
+            float t, u, v;
+            if (intersect_triangle(sh, r.ox, r.oy, r.oz, a, b, c, r.tmin, h.t, &t, &u, &v)) {
+                const uint32_t prim = __float_as_uint(a.w);
+                if (t < h.t || prim < h.prim) {
+                    h.t = t;
+                    h.u = u;
+                    h.v = v;
+                    h.prim = prim;
+                    h.leaf = tri_base + k;
+                }
+            }
+        }
+        uint32_t gbase, ghits, gimask;
+        if (node_hits) {
+            gbase = child_base;
+            ghits = node_hits;
+            gimask = imask;
+        } else {
+            if (st.sp == 0) break;
+            const uint2 e = st.pop();
+            gbase = e.x;
+            ghits = e.y & 0xFFu;
+            gimask = e.y >> 8;
+        }
+        const uint32_t key = __builtin_ctz(ghits);
+        ghits &= ghits - 1u;
+        if (ghits) st.push(make_uint2(gbase, ghits | (gimask << 8)));
+        const uint32_t slot = key ^ oct;
+        node = gbase + __builtin_popcount(gimask & ((1u << slot) - 1u));
+    }
+    return h;
+}
+
+}  // namespace lmb

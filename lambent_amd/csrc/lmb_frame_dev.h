@@ -1,0 +1,68 @@
+// lmb_frame_dev.h - device view of the per-frame buffers (DataBuffers) and the
+// launch entry points of the wavefront path tracer.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include "lmb_layout.h"
+#include "lmb_scene_dev.h"
+
+namespace lmb {
+
+constexpr uint32_t kMaxBounces = 64;
+// counters[]: [0, kMaxBounces] queue length per bounce;
+// [kCounterVertices] (64-bit, two words) path vertices of the current trace.
+constexpr uint32_t kCounterVertices = kMaxBounces + 2;
+constexpr uint32_t kNumCounters = kMaxBounces + 4;
+
+struct FrameDev {
+    uint32_t width, height;
+    uint64_t npix;
+    // outputs: rgba32float images (bindings.wesl:117-122)
+    float4* out_color;
+    float4* out_normal;
+    float4* out_albedo;
+    // per pixel
+    uint32_t* own_seed;
+    float4* pix_sum;
+    // per path (path id = pixel index)
+    uint32_t* ps_seed;
+    float4* ps_through;  // xyz throughput, w picked refractive-index parameter (<0: none)
+    float4* ps_accum;    // xyz sample colour
+    // wavefront queues: slot -> 2 float4 (origin | path id bits, direction)
+    float4* queue[2];
+    uint4* hits;  // slot -> (leaf, t, u, v)
+    uint32_t* counters;
+    uint2* spill;
+    // persistent state (data_buffer/mod.rs:103-114)
+    Reservoir* gi;
+    Reservoir* gi_old;
+    CompressedInfo* info;
+    MarkovChain* mc;
+    MarkovChain* mc_old;
+    WorldMarkovChain* wmc;
+    WorldMarkovChain* wmc_old;
+    uint32_t world_size;
+    uint32_t flags;
+};
+
+struct TraceParams {
+    uint32_t quality;
+    uint32_t samples;
+    float tmin, tmax;
+    uint32_t restir;
+    uint32_t bounces;
+    uint32_t rr_start;
+    uint32_t guiding;
+    uint32_t seed_offset;
+};
+
+// Threads of the persistent wavefront kernels for a frame of npix paths.
+uint64_t frame_threads(uint64_t npix);
+
+// One full trace (all samples, all bounces) on `stream`.
+hipError_t launch_trace(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
+                        const TraceParams& p, hipStream_t stream);
+
+}  // namespace lmb

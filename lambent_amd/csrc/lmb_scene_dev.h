@@ -1,0 +1,43 @@
+// lmb_scene_dev.h - device-pointer view of a scene, passed by value to the
+// kernels, and the launch entry points of the ray-query kernels.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include "lmb_layout.h"
+
+namespace lmb {
+
+struct SceneDev {
+    const Bvh8Node* nodes;
+    const LeafTri* tris;
+    uint32_t num_nodes;
+    uint32_t num_materials;
+    const uint32_t* prim_material;  // material index per original primitive
+    const Material* materials;
+    const uint32_t* texels;         // RGBA8 texel pool
+    const TexDesc* tex;             // all textures
+    uint32_t diffuse_base, num_diffuse;
+    uint32_t emission_base, num_emission;
+    uint32_t attr_base, num_attr;
+    uint32_t recolour;              // TexDesc index of the recolour texture
+    uint32_t bg_base;               // six TexDesc entries: +x -x +y -y +z -z
+};
+
+// Number of compute units of the current device (cached).
+int device_cu_count();
+
+// Threads launched by the persistent ray-query kernels for n rays.
+uint64_t query_threads(uint64_t n);
+
+// rays: n x 32 bytes (origin, tmin, direction, tmax); hits: n x 16 bytes
+// (prim, t, u, v).  spill: query_threads(n) * (kStackMax - kStackLds) uint2.
+hipError_t launch_intersect(const SceneDev& s, const void* rays, uint64_t n, void* hits,
+                            uint2* spill, hipStream_t stream);
+
+hipError_t launch_primary(const SceneDev& s, const CameraDev& cam, uint32_t width,
+                          uint32_t height, float tmin, float tmax, uint32_t* prim,
+                          float* t, uint2* spill, hipStream_t stream);
+
+}  // namespace lmb

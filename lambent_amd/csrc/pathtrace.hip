@@ -1,0 +1,276 @@
+// pathtrace.hip - wavefront path tracer for gfx950.
+//
+// The reference runs one megakernel thread per pixel (rt_main,
+// general.wesl:506-610) that loops over SAMPLES paths of <= BOUNCES segments
+// (rt_sample, general.wesl:643-975).  Here each sample is a wavefront:
+//
+//   generate  per pixel: seed, primary ray (create_ray) into queue 0
+//   extend    per queued ray: closest hit over the 8-wide BVH
+//   shade     per queued ray: hit record (ray_hit), emission, quality hooks,
+//             Fresnel / diffuse / metal / dielectric scattering, optional
+//             Russian roulette; survivors are compacted into the other queue
+//             with one atomic per wavefront (ballot + prefix popcount)
+//   resolve   colour = max(sum / SAMPLES, 0)
+//
+// This file implements guiding mode LMB_GUIDING_OFF ("plain"): the BSDF
+// sampling of rt_sample with every Markov chain invalid, i.e. the cosine
+// lobe with probability 1 (general.wesl:812-815).  Its random-number stream
+// is the reference's with the guiding draws left out.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "dev_common.h"
+#include "dev_shade.h"
+#include "dev_traverse.h"
+#include "lmb_frame_dev.h"
+
+namespace lmb {
+
+constexpr int kPtBlock = 256;
+
+uint64_t frame_threads(uint64_t npix) {
+    uint64_t blocks = (npix + kPtBlock - 1) / kPtBlock;
+    uint64_t cap = (uint64_t)device_cu_count() * 8;
+    blocks = std::max<uint64_t>(1, std::min(blocks, cap));
+    return blocks * kPtBlock;
+}
+
+__device__ __forceinline__ uint64_t* vertex_counter(const FrameDev& f) {
+    return reinterpret_cast<uint64_t*>(f.counters + kCounterVertices);
+}
+
+// general.wesl:516: own_seed = rand_u32(x * y) + seed_offset + y; later
+// samples advance it with rand_u32 (general.wesl:544).
+__global__ __launch_bounds__(kPtBlock) void k_generate(CameraDev cam, FrameDev f, TraceParams p,
+                                                       uint32_t sample) {
+    const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
+    if (gtid == 0) {
+        f.counters[0] = (uint32_t)f.npix;
+        for (uint32_t i = 1; i <= kMaxBounces; ++i) f.counters[i] = 0;
+        if (sample == 0) *vertex_counter(f) = 0;
+    }
+    for (uint64_t pix = gtid; pix < f.npix; pix += stride) {
+        const uint32_t x = (uint32_t)(pix % f.width), y = (uint32_t)(pix / f.width);
+        uint32_t seed;
+        if (sample == 0) {
+            seed = rand_u32(x * y) + p.seed_offset + y;
+            f.pix_sum[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+            seed = rand_u32(f.own_seed[pix]);
+        }
+        f.own_seed[pix] = seed;
+        float o[3], d[3];
+        create_ray(cam, x, y, f.width, f.height, o, d);
+        f.queue[0][2 * pix] = make_float4(o[0], o[1], o[2], __uint_as_float((uint32_t)pix));
+        f.queue[0][2 * pix + 1] = make_float4(d[0], d[1], d[2], 0.f);
+        f.ps_seed[pix] = seed;
+        f.ps_through[pix] = make_float4(1.f, 1.f, 1.f, -1.f);
+        f.ps_accum[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_extend(SceneDev s, FrameDev f, TraceParams p,
+                                                     uint32_t bounce) {
+    __shared__ uint2 lds_stack[kStackLds * kPtBlock];
+    const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
+    const uint32_t count = f.counters[bounce];
+    if (gtid == 0) atomicAdd((unsigned long long*)vertex_counter(f), (unsigned long long)count);
+    const float4* __restrict__ q = f.queue[bounce & 1];
+    ShortStack<kPtBlock> st;
+    st.lds = lds_stack + threadIdx.x;
+    st.spill = f.spill + gtid * (kStackMax - kStackLds);
+    for (uint64_t slot = gtid; slot < count; slot += stride) {
+        const float4 o = q[2 * slot];
+        const float4 d = q[2 * slot + 1];
+        Ray r;
+        r.ox = o.x;
+        r.oy = o.y;
+        r.oz = o.z;
+        r.tmin = p.tmin;
+        r.dx = d.x;
+        r.dy = d.y;
+        r.dz = d.z;
+        r.tmax = p.tmax;
+        const Hit h = trace_closest<kPtBlock>(s.nodes, s.tris, s.num_nodes, r, st);
+        f.hits[slot] = make_uint4(h.leaf, __float_as_uint(h.t), __float_as_uint(h.u),
+                                  __float_as_uint(h.v));
+    }
+}
+
+template <uint32_t Q>
+__global__ __launch_bounds__(kPtBlock) void k_shade_plain(SceneDev s, FrameDev f, TraceParams p,
+                                                          uint32_t bounce) {
+    const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
+    const uint32_t count = f.counters[bounce];
+    const uint64_t count_r = ((uint64_t)count + 63) & ~63ull;  // wave-uniform trip count
+    const float4* __restrict__ qin = f.queue[bounce & 1];
+    float4* __restrict__ qout = f.queue[(bounce + 1) & 1];
+    const uint32_t lane = __lane_id();
+    for (uint64_t slot = gtid; slot < count_r; slot += stride) {
+        bool cont = false;
+        float4 out_o = make_float4(0.f, 0.f, 0.f, 0.f), out_d = out_o;
+        if (slot < count) {
+            const float4 o4 = qin[2 * slot];
+            const float4 d4 = qin[2 * slot + 1];
+            const uint32_t pid = __float_as_uint(o4.w);
+            const uint4 h = f.hits[slot];
+            uint32_t seed = f.ps_seed[pid];
+            const float4 th = f.ps_through[pid];
+            const float4 ac = f.ps_accum[pid];
+            V3 color = v3(th.x, th.y, th.z);
+            float picked = th.w;
+            V3 scol = v3(ac.x, ac.y, ac.z);
+            const V3 dir = v3(d4.x, d4.y, d4.z);
+            const V3 org = v3(o4.x, o4.y, o4.z);
+            if (h.x != kMissPrim) {
+                const float t = __uint_as_float(h.y);
+                const Isect it = make_isect(s, h.x, t, __uint_as_float(h.z),
+                                            __uint_as_float(h.w), dir);
+                const V3 new_pos = fma3(dir, v3s(t), org);
+                if (bounce == 0) {
+                    f.out_normal[pid] = make_float4(it.normal.x, it.normal.y, it.normal.z, 1.f);
+                    f.out_albedo[pid] = make_float4(it.color.x, it.color.y, it.color.z, 1.f);
+                }
+                // general.wesl:714-762
+                scol = fma3(color, it.emission, scol);
+                const V3 assign_normal =
+                    get_normal<Q>(dir, it.normal, it.tangent, seed, it.rough_x, it.rough_y);
+                const V3 transparent_normal =
+                    get_normal<Q>(dir, it.normal, it.tangent, seed, it.rough_tx, it.rough_ty);
+                V3 normal = transparent_normal;
+                bool should_reflect = false;
+                seed += 1u;
+                V3 ccol;
+                float ri = pick_ref_idx<Q>(&picked, it.ior_low, it.ior_high, seed, &ccol);
+                seed += 1u;
+                if (!(!it.front_face && it.ty == kMatTransparent)) ri = 1.0f / ri;
+                color = color * ccol;
+                if (ri != 1.0f) {
+                    const float cos_theta = fminf(dot3(-dir, normal), 1.0f);
+                    const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+                    const bool cannot_refract = ri * sin_theta > 1.0f;
+                    const float pdf = reflectance(cos_theta, ri);
+                    const float rnd = rand_f32(seed);
+                    if (cannot_refract || pdf > rnd) should_reflect = true;
+                    seed += 1u;
+                }
+                if (!should_reflect || it.ty == kMatTransparent) color = color * it.color;
+                if (!should_reflect) {
+                    normal = assign_normal;
+                    seed += 1u;
+                }
+                // general.wesl:771-901 with every guiding candidate invalid
+                const bool reflecting = should_reflect || it.ty == kMatMetallic;
+                V3 ndir = dir;
+                if (!reflecting) {
+                    if (it.ty == kMatDiffuse) {
+                        const V3 dir_cos = cosine_weighted_hemisphere(seed, it.normal, it.tangent);
+                        const float cos_pdf = fmaxf(dot3(dir_cos, it.normal) / kPi, 0.0f);
+                        seed += 3u;  // theta, xi and the lobe choice
+                        const float avg_pdf = 1.0f * cos_pdf;
+                        const V3 diffuse =
+                            make_diffuse<Q>(it.color, dir, it.normal, it.tangent, it.rough_x, dir_cos);
+                        const float cached_brdf = fmaxf(dot3(dir_cos, it.normal) / kPi, 0.0f);
+                        const float nf = fabsf(cached_brdf) / fmaxf(avg_pdf, 0.001f);
+                        color = (color * nf) * diffuse;
+                        ndir = dir_cos;
+                    } else if (it.ty == kMatTransparent) {
+                        ndir = refract3(dir, normal, ri);
+                    }
+                } else {
+                    ndir = reflect3(dir, normal);
+                    if (dot3(ndir, it.normal) < 0.0f) ndir = reflect3(ndir, it.normal);
+                }
+                bool alive = bounce + 1 < p.bounces;
+                if (alive && bounce >= p.rr_start) {
+                    const float q = fminf(fmaxf(fmaxf(color.x, color.y), color.z), 1.0f);
+                    const float rnd = rand_f32(seed);
+                    seed += 1u;
+                    if (rnd >= q)
+                        alive = false;
+                    else
+                        color = color / q;
+                }
+                if (alive) {
+                    cont = true;
+                    out_o = make_float4(new_pos.x, new_pos.y, new_pos.z, __uint_as_float(pid));
+                    out_d = make_float4(ndir.x, ndir.y, ndir.z, 0.f);
+                    f.ps_seed[pid] = seed;
+                    f.ps_through[pid] = make_float4(color.x, color.y, color.z, picked);
+                    f.ps_accum[pid] = make_float4(scol.x, scol.y, scol.z, 0.f);
+                } else {
+                    float4 ps = f.pix_sum[pid];
+                    f.pix_sum[pid] = make_float4(ps.x + scol.x, ps.y + scol.y, ps.z + scol.z, 0.f);
+                }
+            } else {
+                if (bounce == 0) {
+                    f.out_normal[pid] = make_float4(0.f, 0.f, 0.f, 1.f);
+                    f.out_albedo[pid] = make_float4(0.f, 0.f, 0.f, 1.f);
+                }
+                // general.wesl:915-938
+                const V3 bg = background(s, dir);
+                scol = fma3(color, bg, scol);
+                float4 ps = f.pix_sum[pid];
+                f.pix_sum[pid] = make_float4(ps.x + scol.x, ps.y + scol.y, ps.z + scol.z, 0.f);
+            }
+        }
+        // Compact the survivors into the next queue: one atomic per wave.
+        const uint64_t mask = __ballot(cont);
+        if (mask) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&f.counters[bounce + 1], (uint32_t)__popcll(mask));
+            base = __shfl(base, (int)leader);
+            if (cont) {
+                const uint32_t idx = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+                qout[2 * (uint64_t)idx] = out_o;
+                qout[2 * (uint64_t)idx + 1] = out_d;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_resolve(FrameDev f, TraceParams p) {
+    const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
+    const float ns = (float)p.samples;
+    for (uint64_t pix = gtid; pix < f.npix; pix += stride) {
+        const float4 s4 = f.pix_sum[pix];
+        f.out_color[pix] = make_float4(fmaxf(s4.x / ns, 0.f), fmaxf(s4.y / ns, 0.f),
+                                       fmaxf(s4.z / ns, 0.f), 1.f);
+    }
+}
+
+template <uint32_t Q>
+static hipError_t launch_plain(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
+                               const TraceParams& p, hipStream_t stream) {
+    const dim3 grid((uint32_t)(frame_threads(f.npix) / kPtBlock)), block(kPtBlock);
+    for (uint32_t sample = 0; sample < p.samples; ++sample) {
+        hipLaunchKernelGGL(k_generate, grid, block, 0, stream, cam, f, p, sample);
+        for (uint32_t b = 0; b < p.bounces; ++b) {
+            hipLaunchKernelGGL(k_extend, grid, block, 0, stream, s, f, p, b);
+            hipLaunchKernelGGL(k_shade_plain<Q>, grid, block, 0, stream, s, f, p, b);
+        }
+    }
+    hipLaunchKernelGGL(k_resolve, grid, block, 0, stream, f, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
+                        const TraceParams& p, hipStream_t stream) {
+    if (f.npix == 0) return hipSuccess;
+    switch (p.quality) {
+        case kQualityLow:
+            return launch_plain<kQualityLow>(s, cam, f, p, stream);
+        case kQualityHigh:
+            return launch_plain<kQualityHigh>(s, cam, f, p, stream);
+        default:
+            return launch_plain<kQualityMedium>(s, cam, f, p, stream);
+    }
+}
+
+}  // namespace lmb

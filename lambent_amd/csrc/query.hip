@@ -1,0 +1,107 @@
+// query.hip - ray-query kernels: arbitrary rays and camera primary rays.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "dev_common.h"
+#include "dev_traverse.h"
+#include "lmb_scene_dev.h"
+
+namespace lmb {
+
+constexpr int kQueryBlock = 256;
+
+int device_cu_count() {
+    static int cached_dev = -1;
+    static int cached_cu = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (dev != cached_dev) {
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) !=
+                hipSuccess ||
+            cu <= 0)
+            cu = 256;
+        cached_cu = cu;
+        cached_dev = dev;
+    }
+    return cached_cu;
+}
+
+uint64_t query_threads(uint64_t n) {
+    uint64_t blocks = (n + kQueryBlock - 1) / kQueryBlock;
+    uint64_t cap = (uint64_t)device_cu_count() * 8;
+    blocks = std::max<uint64_t>(1, std::min(blocks, cap));
+    return blocks * kQueryBlock;
+}
+
+__global__ __launch_bounds__(kQueryBlock) void k_intersect(SceneDev s, const Ray* __restrict__ rays,
+                                                           uint64_t n, uint4* __restrict__ hits,
+                                                           uint2* __restrict__ spill) {
+    __shared__ uint2 lds_stack[kStackLds * kQueryBlock];
+    const uint64_t gtid = (uint64_t)blockIdx.x * kQueryBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kQueryBlock;
+    ShortStack<kQueryBlock> st;
+    st.lds = lds_stack + threadIdx.x;
+    st.spill = spill + gtid * (kStackMax - kStackLds);
+    for (uint64_t i = gtid; i < n; i += stride) {
+        Ray r = rays[i];
+        Hit h = trace_closest<kQueryBlock>(s.nodes, s.tris, s.num_nodes, r, st);
+        hits[i] = make_uint4(h.prim, __float_as_uint(h.t), __float_as_uint(h.u),
+                             __float_as_uint(h.v));
+    }
+}
+
+__global__ __launch_bounds__(kQueryBlock) void k_primary(SceneDev s, CameraDev cam,
+                                                         uint32_t width, uint32_t height,
+                                                         float tmin, float tmax,
+                                                         uint32_t* __restrict__ prim_out,
+                                                         float* __restrict__ t_out,
+                                                         uint2* __restrict__ spill) {
+    __shared__ uint2 lds_stack[kStackLds * kQueryBlock];
+    const uint64_t gtid = (uint64_t)blockIdx.x * kQueryBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kQueryBlock;
+    const uint64_t n = (uint64_t)width * height;
+    ShortStack<kQueryBlock> st;
+    st.lds = lds_stack + threadIdx.x;
+    st.spill = spill + gtid * (kStackMax - kStackLds);
+    for (uint64_t i = gtid; i < n; i += stride) {
+        uint32_t x = (uint32_t)(i % width), y = (uint32_t)(i / width);
+        float o[3], d[3];
+        create_ray(cam, x, y, width, height, o, d);
+        Ray r;
+        r.ox = o[0];
+        r.oy = o[1];
+        r.oz = o[2];
+        r.tmin = tmin;
+        r.dx = d[0];
+        r.dy = d[1];
+        r.dz = d[2];
+        r.tmax = tmax;
+        Hit h = trace_closest<kQueryBlock>(s.nodes, s.tris, s.num_nodes, r, st);
+        prim_out[i] = h.prim;
+        t_out[i] = h.t;
+    }
+}
+
+hipError_t launch_intersect(const SceneDev& s, const void* rays, uint64_t n, void* hits,
+                            uint2* spill, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    uint64_t threads = query_threads(n);
+    hipLaunchKernelGGL(k_intersect, dim3((uint32_t)(threads / kQueryBlock)), dim3(kQueryBlock),
+                       0, stream, s, (const Ray*)rays, n, (uint4*)hits, spill);
+    return hipGetLastError();
+}
+
+hipError_t launch_primary(const SceneDev& s, const CameraDev& cam, uint32_t width,
+                          uint32_t height, float tmin, float tmax, uint32_t* prim, float* t,
+                          uint2* spill, hipStream_t stream) {
+    uint64_t n = (uint64_t)width * height;
+    if (n == 0) return hipSuccess;
+    uint64_t threads = query_threads(n);
+    hipLaunchKernelGGL(k_primary, dim3((uint32_t)(threads / kQueryBlock)), dim3(kQueryBlock), 0,
+                       stream, s, cam, width, height, tmin, tmax, prim, t, spill);
+    return hipGetLastError();
+}
+
+}  // namespace lmb

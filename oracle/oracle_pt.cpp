@@ -1,0 +1,564 @@
+// oracle_pt.cpp - CPU ORACLE (test infrastructure only) for the path tracer.
+//
+// A per-pixel restatement of the reference megakernel
+//   rt_main    path_tracing/general.wesl:506-610 (seeds, sample loop, outputs)
+//   rt_sample  general.wesl:643-975 (bounce loop, emission, Fresnel, scatter)
+//   ray_hit    general.wesl:1080-1136 (hit record: tangent, normal, textures)
+//   helpers    general.wesl:1139-1281, importance_sampling/shared.wesl:38-46,
+//              215-231, quality modules low.wesl / medium.wesl / high.wesl
+// in "plain" mode: Markov-chain guiding off, i.e. every guiding candidate
+// invalid, so the diffuse lobe is the cosine lobe with probability 1
+// (general.wesl:812-815) and the guiding random draws are left out.
+// Optional Russian roulette from bounce `rr_start` (not in the reference).
+//
+// Closest hits come from the oracle BVH in oracle_geom.cpp, which returns
+// exactly the brute-force watertight result.
+//
+// Pinned by (tests/test_oracle_pt.py): closed-box furnace identities
+// (diffuse / metal / Low quality: radiance E * sum_{k<D} rho^k, exact per
+// path), emission-only and background-only images, Russian-roulette
+// unbiasedness, the Oren-Nayar weight at roughness 0, VNDF / cosine sample
+// distributions (chi-square), the Schlick closed form, and the CIE colour
+// weights of the High quality against a float64 evaluation.
+#include <omp.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+extern "C" {
+uint32_t orc_rand_u32(uint32_t s);
+float orc_rand_f32(uint32_t s);
+void orc_create_ray(const float* cam, uint32_t px, uint32_t py, uint32_t width, uint32_t height,
+                    float* o, float* d);
+void* orc_bvh_create(const float* verts, uint64_t nverts, const uint32_t* idx, uint64_t ntri);
+void orc_bvh_destroy(void* h);
+struct OrcHit {
+    uint32_t prim;
+    float t, u, v;
+};
+void orc_bvh_intersect1(void* hb, const float* ray, OrcHit* hit);
+}
+
+namespace {
+
+const float PI = 3.14159265358979323846f;
+const float HALF_PI = PI / 2.0f;
+const float TAU = 2.0f * PI;
+
+struct F3 {
+    float x, y, z;
+};
+inline F3 mk(float x, float y, float z) { return F3{x, y, z}; }
+inline F3 splat(float s) { return F3{s, s, s}; }
+inline F3 add(F3 a, F3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline F3 sub(F3 a, F3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline F3 neg(F3 a) { return mk(-a.x, -a.y, -a.z); }
+inline F3 mul(F3 a, F3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline F3 scale(F3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+inline F3 divs(F3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+inline F3 fma3(F3 a, F3 b, F3 c) {
+    return mk(std::fma(a.x, b.x, c.x), std::fma(a.y, b.y, c.y), std::fma(a.z, b.z, c.z));
+}
+inline float dot(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+inline F3 cross(F3 a, F3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+inline float length(F3 a) { return std::sqrt(dot(a, a)); }
+inline F3 normalize(F3 a) {
+    float l = length(a);
+    return mk(a.x / l, a.y / l, a.z / l);
+}
+inline F3 reflect(F3 e1, F3 e2) { return sub(e1, scale(e2, 2.0f * dot(e2, e1))); }
+inline F3 refract(F3 e1, F3 e2, float eta) {
+    float d = dot(e2, e1);
+    float k = 1.0f - eta * eta * (1.0f - d * d);
+    if (k < 0.0f) return splat(0.0f);
+    return sub(scale(e1, eta), scale(e2, eta * d + std::sqrt(k)));
+}
+inline bool near_zero(float f) { return f < 0.0001f && f > -0.0001f; }
+inline bool vec3_near_zero(F3 v) { return near_zero(v.x) && near_zero(v.y) && near_zero(v.z); }
+inline float saturate(float x) { return std::fmin(std::fmax(x, 0.0f), 1.0f); }
+
+float half_to_float(uint32_t h) {
+    uint32_t s = (h >> 15) & 1u, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+    float v;
+    if (e == 0)
+        v = std::ldexp((float)m, -24);
+    else if (e == 31)
+        v = m ? NAN : INFINITY;
+    else
+        v = std::ldexp((float)(m | 0x400u), (int)e - 25);
+    return s ? -v : v;
+}
+float bfloat_to_float(uint32_t b) {
+    uint32_t x = b << 16;
+    float f;
+    std::memcpy(&f, &x, 4);
+    return f;
+}
+
+struct Scene {
+    void* bvh;
+    std::vector<float> verts;
+    std::vector<uint32_t> idx;
+    uint64_t ntri;
+    std::vector<uint32_t> matidx;
+    std::vector<uint32_t> mats;  // 8 words per material
+    std::vector<uint32_t> texels;
+    std::vector<uint32_t> descs;  // offset, width, height per texture
+    uint32_t diffuse_base, emission_base, attr_base, recolour, bg_base;
+};
+
+struct Tex4 {
+    float r, g, b, a;
+};
+
+Tex4 texel_value(const Scene& s, uint32_t ti, uint32_t x, uint32_t y) {
+    uint32_t off = s.descs[3 * ti], w = s.descs[3 * ti + 1], h = s.descs[3 * ti + 2];
+    if (x >= w) x = w - 1;
+    if (y >= h) y = h - 1;
+    uint32_t t = s.texels[off + y * w + x];
+    return Tex4{(float)(t & 255u) / 255.0f, (float)((t >> 8) & 255u) / 255.0f,
+                (float)((t >> 16) & 255u) / 255.0f, (float)(t >> 24) / 255.0f};
+}
+
+// Nearest filtering, clamp to edge (the default wgpu sampler).
+Tex4 sample_tex(const Scene& s, uint32_t ti, float u, float v) {
+    uint32_t w = s.descs[3 * ti + 1], h = s.descs[3 * ti + 2];
+    float fx = std::floor(u * (float)w), fy = std::floor(v * (float)h);
+    if (!(fx >= 0.0f)) fx = 0.0f;
+    if (!(fy >= 0.0f)) fy = 0.0f;
+    if (fx > (float)(w - 1)) fx = (float)(w - 1);
+    if (fy > (float)(h - 1)) fy = (float)(h - 1);
+    return texel_value(s, ti, (uint32_t)fx, (uint32_t)fy);
+}
+
+// Cube map: major axis picks the face (+x,-x,+y,-y,+z,-z), then (s, t) as in
+// the Vulkan cube-map face table.
+F3 background(const Scene& s, F3 d) {
+    float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+    uint32_t face;
+    float sc, tc, ma;
+    if (ax >= ay && ax >= az) {
+        ma = ax;
+        if (d.x >= 0.0f) {
+            face = 0;
+            sc = -d.z;
+        } else {
+            face = 1;
+            sc = d.z;
+        }
+        tc = -d.y;
+    } else if (ay >= az) {
+        ma = ay;
+        sc = d.x;
+        if (d.y >= 0.0f) {
+            face = 2;
+            tc = d.z;
+        } else {
+            face = 3;
+            tc = -d.z;
+        }
+    } else {
+        ma = az;
+        if (d.z >= 0.0f) {
+            face = 4;
+            sc = d.x;
+        } else {
+            face = 5;
+            sc = -d.x;
+        }
+        tc = -d.y;
+    }
+    Tex4 c = sample_tex(s, s.bg_base + face, 0.5f * (sc / ma + 1.0f), 0.5f * (tc / ma + 1.0f));
+    return mk(c.r, c.g, c.b);
+}
+
+struct Hit {
+    bool hit;
+    float t;
+    F3 color, emission, normal, tangent;
+    uint32_t ty;
+    bool front_face;
+    float rx, ry, rtx, rty, n_lo, n_hi;
+};
+
+F3 correct_tangent(F3 dir) {
+    F3 a = mk(std::fabs(dir.x), std::fabs(dir.y), std::fabs(dir.z));
+    bool x_largest = a.x > a.y && a.x > a.z;
+    bool y_largest = a.y > a.x && a.y > a.z;
+    bool rev;
+    if (x_largest)
+        rev = dir.x < 0.0f;
+    else if (y_largest)
+        rev = dir.y < 0.0f;
+    else
+        rev = dir.z < 0.0f;
+    return rev ? neg(dir) : dir;
+}
+
+Hit ray_hit(const Scene& s, F3 org, F3 dir, float tmin, float tmax) {
+    float ray[8] = {org.x, org.y, org.z, tmin, dir.x, dir.y, dir.z, tmax};
+    OrcHit oh;
+    orc_bvh_intersect1(s.bvh, ray, &oh);
+    Hit h;
+    std::memset(&h, 0, sizeof h);
+    if (oh.prim == 0xFFFFFFFFu) return h;
+    h.hit = true;
+    h.t = oh.t;
+    uint64_t p = oh.prim;
+    uint64_t i0 = s.idx.empty() ? 3 * p : s.idx[3 * p], i1 = s.idx.empty() ? 3 * p + 1 : s.idx[3 * p + 1],
+             i2 = s.idx.empty() ? 3 * p + 2 : s.idx[3 * p + 2];
+    F3 v0 = mk(s.verts[4 * i0], s.verts[4 * i0 + 1], s.verts[4 * i0 + 2]);
+    F3 v1 = mk(s.verts[4 * i1], s.verts[4 * i1 + 1], s.verts[4 * i1 + 2]);
+    F3 v2 = mk(s.verts[4 * i2], s.verts[4 * i2 + 1], s.verts[4 * i2 + 2]);
+    // front face: counter-clockwise seen from the ray origin
+    h.front_face = dot(dir, cross(sub(v1, v0), sub(v2, v0))) < 0.0f;
+    const uint32_t* m = &s.mats[8 * s.matidx[p]];
+    uint32_t idx_diffuse = m[4] & 0xFFFFu, idx_emission = m[4] >> 16;
+    uint32_t idx_attr = m[5] & 0xFFFFu;
+    h.ty = m[5] >> 16;
+    h.n_lo = half_to_float(m[7] & 0xFFFFu);
+    h.n_hi = half_to_float(m[7] >> 16);
+    F3 tangent = correct_tangent(normalize(sub(v0, v1)));
+    F3 normal = normalize(cross(tangent, sub(v0, v2)));
+    h.normal = dot(dir, normal) > 0.0f ? neg(normal) : normal;
+    h.tangent = tangent;
+    float bx = oh.u, by = oh.v, bz = (1.0f - oh.u) - oh.v;
+    float tu = std::fma(half_to_float(m[0] & 0xFFFFu), bz,
+                        std::fma(half_to_float(m[1] & 0xFFFFu), bx, half_to_float(m[2] & 0xFFFFu) * by));
+    float tv = std::fma(half_to_float(m[0] >> 16), bz,
+                        std::fma(half_to_float(m[1] >> 16), bx, half_to_float(m[2] >> 16) * by));
+    Tex4 dt = sample_tex(s, s.diffuse_base + idx_diffuse, tu, tv);
+    Tex4 rc = texel_value(s, s.recolour, m[3] & 0xFFFFu, m[3] >> 16);
+    h.color = mk(dt.r * rc.r, dt.g * rc.g, dt.b * rc.b);
+    float es0 = bfloat_to_float(m[6] & 0xFFFFu), es1 = bfloat_to_float(m[6] >> 16);
+    F3 em = splat(0.0f);
+    if (idx_emission != 0xFFFFu) {
+        Tex4 et = sample_tex(s, s.emission_base + idx_emission, tu, tv);
+        em = scale(mk(et.r, et.g, et.b), es0);
+    }
+    h.emission = add(em, scale(h.color, es1));
+    if (idx_attr != 0xFFFFu) {
+        Tex4 at = sample_tex(s, s.attr_base + idx_attr, tu, tv);
+        h.rx = at.r;
+        h.ry = at.g;
+        h.rtx = at.b;
+        h.rty = at.a;
+    }
+    return h;
+}
+
+F3 from_onb(F3 src, F3 up, F3 tangent) {
+    F3 t = cross(up, tangent);
+    return fma3(splat(src.x), tangent, fma3(splat(src.y), up, scale(t, src.z)));
+}
+F3 to_onb(F3 src, F3 up, F3 tangent) {
+    F3 t = cross(up, tangent);
+    F3 inv_s = cross(up, t), inv_up = cross(t, tangent), inv_t = cross(tangent, up);
+    return fma3(splat(src.x), inv_s, fma3(splat(src.y), inv_up, scale(inv_t, src.z)));
+}
+F3 cosine_weighted_hemisphere(uint32_t seed, F3 normal, F3 tangent) {
+    float r1 = orc_rand_f32(seed), r2 = orc_rand_f32(seed + 1u);
+    float phi = TAU * r1;
+    F3 hemi = mk(std::cos(phi) * std::sqrt(r2), std::sqrt(1.0f - r2), std::sin(phi) * std::sqrt(r2));
+    return from_onb(hemi, normal, tangent);
+}
+F3 sample_vndf_ggx(uint32_t seed, F3 view, float rx, float ry) {
+    float u1 = orc_rand_f32(seed), u2 = orc_rand_f32(seed + 1u);
+    F3 in_dir = mk(view.x * rx, view.y, view.z * ry);
+    if (vec3_near_zero(in_dir)) in_dir = mk(0.0f, -1.0f, 0.0f);
+    F3 wi = normalize(in_dir);
+    float phi = TAU * u1;
+    float y = std::fma(1.0f - u2, 1.0f + wi.y, -wi.y);
+    float sin_theta = std::sqrt(std::fmin(std::fmax(std::fma(y, -y, 1.0f), 0.0f), 1.0f));
+    F3 wm = add(mk(sin_theta * std::cos(phi), y, sin_theta * std::sin(phi)), wi);
+    F3 out = mk(wm.x * rx, wm.y, wm.z * ry);
+    if (vec3_near_zero(out)) out = mk(0.0f, 1.0f, 0.0f);
+    return normalize(out);
+}
+float reflectance(float cosine, float ref_idx) {
+    float r0 = (1.0f - ref_idx) / (1.0f + ref_idx);
+    float r0s = r0 * r0;
+    return std::fma(1.0f - r0s, std::pow(1.0f - cosine, 5.0f), r0s);
+}
+F3 oren_nayar(F3 incoming, F3 reflected, float roughness, F3 color) {
+    float rr = roughness * HALF_PI;
+    float rr2 = rr * rr;
+    F3 a = fma3(scale(color, 0.17f), splat(rr2 / (rr2 + 0.13f)),
+                splat(1.0f - 0.5f * (rr2 / (rr2 + 0.33f))));
+    float b = 0.45f * (rr2 / (rr2 + 0.09f));
+    float s = std::fma(-reflected.y, incoming.y, dot(reflected, incoming));
+    float t = 1.0f;
+    if (s > 0.0f) {
+        t = std::fmax(reflected.y, incoming.y);
+        if (near_zero(t)) t = 0.001f;
+    }
+    return fma3(splat(b), splat(s / t), a);
+}
+
+F3 get_normal(uint32_t q, F3 ray_dir, F3 normal, F3 tangent, uint32_t seed, float rx, float ry) {
+    if (q == 0) return normal;
+    F3 n_onb = sample_vndf_ggx(seed, to_onb(ray_dir, normal, tangent), rx, ry);
+    return from_onb(n_onb, normal, tangent);
+}
+F3 make_diffuse(uint32_t q, F3 hit_color, F3 ray_dir, F3 normal, F3 tangent, float rough,
+                F3 new_dir) {
+    if (q == 0) return splat(1.0f);
+    return oren_nayar(to_onb(ray_dir, normal, tangent), new_dir, rough, hit_color);
+}
+float cie_g(float x, float u, float t1, float t2) {
+    float tau = x < u ? t1 : t2;
+    float d = x - u;
+    return std::exp(-((d * d) * (tau * tau)) * 0.5f);
+}
+// returns IOR; *colour = wavelength weight; *picked < 0 = not picked yet
+float pick_ref_idx(uint32_t q, float* picked, float lo, float hi, uint32_t seed, F3* colour) {
+    *colour = splat(1.0f);
+    if (q == 0) return lo;
+    bool fresh = *picked < 0.0f;
+    if (fresh) *picked = orc_rand_f32(seed);
+    float u = *picked;
+    if (q == 2 && fresh) {
+        float lambda = 760.0f * (1.0f - u) + 340.0f * u;
+        float x = std::fma(1.056f, cie_g(lambda, 599.8f, 0.0264f, 0.0323f),
+                           std::fma(0.362f, cie_g(lambda, 442.0f, 0.0624f, 0.0374f),
+                                    -(0.065f * cie_g(lambda, 501.1f, 0.049f, 0.0382f))));
+        float y = std::fma(0.821f, cie_g(lambda, 568.8f, 0.0213f, 0.0248f),
+                           0.286f * cie_g(lambda, 530.9f, 0.0613f, 0.0322f));
+        float z = std::fma(1.217f, cie_g(lambda, 437.0f, 0.0845f, 0.0278f),
+                           0.681f * cie_g(lambda, 459.0f, 0.0385f, 0.0725f));
+        float r = (3.2406f * x + -1.5372f * y) + -0.4986f * z;
+        float g = (-0.9689f * x + 1.8758f * y) + 0.0415f * z;
+        float b = (0.0557f * x + -0.204f * y) + 1.057f * z;
+        *colour = mk(saturate(r) * 3.910f, saturate(g) * 4.411f, saturate(b) * 5.511f);
+    }
+    return lo * (1.0f - u) + hi * u;
+}
+
+struct PlainOpts {
+    uint32_t quality, bounces, rr_start;
+    float tmin, tmax;
+};
+
+struct SampleOut {
+    F3 color, normal, albedo;
+    uint32_t vertices;
+};
+
+SampleOut rt_sample_plain(const Scene& s, const PlainOpts& o, F3 org, F3 dir, uint32_t own_seed) {
+    SampleOut out;
+    out.color = splat(0.0f);
+    out.normal = splat(0.0f);
+    out.albedo = splat(0.0f);
+    out.vertices = 0;
+    uint32_t self_seed = own_seed;
+    F3 color = splat(1.0f);
+    float picked = -1.0f;
+    for (uint32_t i = 0; i < o.bounces; ++i) {
+        out.vertices++;
+        Hit it = ray_hit(s, org, dir, o.tmin, o.tmax);
+        if (!it.hit) {
+            out.color = fma3(color, background(s, dir), out.color);
+            break;
+        }
+        F3 new_pos = fma3(dir, splat(it.t), org);
+        org = new_pos;
+        if (i == 0) {
+            out.normal = it.normal;
+            out.albedo = it.color;
+        }
+        out.color = fma3(color, it.emission, out.color);
+        F3 assign_normal = get_normal(o.quality, dir, it.normal, it.tangent, self_seed, it.rx, it.ry);
+        F3 transparent_normal =
+            get_normal(o.quality, dir, it.normal, it.tangent, self_seed, it.rtx, it.rty);
+        F3 normal = transparent_normal;
+        bool should_reflect = false;
+        self_seed += 1u;
+        F3 ccol;
+        float ri = pick_ref_idx(o.quality, &picked, it.n_lo, it.n_hi, self_seed, &ccol);
+        self_seed += 1u;
+        if (!(!it.front_face && it.ty == 2u)) ri = 1.0f / ri;
+        color = mul(color, ccol);
+        if (ri != 1.0f) {
+            float cos_theta = std::fmin(dot(neg(dir), normal), 1.0f);
+            float sin_theta = std::sqrt(1.0f - cos_theta * cos_theta);
+            bool cannot = ri * sin_theta > 1.0f;
+            float pdf = reflectance(cos_theta, ri);
+            if (cannot || pdf > orc_rand_f32(self_seed)) should_reflect = true;
+            self_seed += 1u;
+        }
+        if (!should_reflect || it.ty == 2u) color = mul(color, it.color);
+        if (!should_reflect) {
+            normal = assign_normal;
+            self_seed += 1u;
+        }
+        bool reflecting = should_reflect || it.ty == 1u;
+        if (!reflecting) {
+            if (it.ty == 0u) {
+                F3 dir_cos = cosine_weighted_hemisphere(self_seed, it.normal, it.tangent);
+                float cosine_pdf = std::fmax(dot(dir_cos, it.normal) / PI, 0.0f);
+                self_seed += 3u;
+                float avg_pdf = 1.0f * cosine_pdf;
+                F3 diffuse = make_diffuse(o.quality, it.color, dir, it.normal, it.tangent, it.rx, dir_cos);
+                float cached_brdf = std::fmax(dot(dir_cos, it.normal) / PI, 0.0f);
+                float nf = std::fabs(cached_brdf) / std::fmax(avg_pdf, 0.001f);
+                color = mul(scale(color, nf), diffuse);
+                dir = dir_cos;
+            } else if (it.ty == 2u) {
+                dir = refract(dir, normal, ri);
+            }
+        } else {
+            dir = reflect(dir, normal);
+            if (dot(dir, it.normal) < 0.0f) dir = reflect(dir, it.normal);
+        }
+        if (i + 1 < o.bounces && i >= o.rr_start) {
+            float q = std::fmin(std::fmax(std::fmax(color.x, color.y), color.z), 1.0f);
+            float r = orc_rand_f32(self_seed);
+            self_seed += 1u;
+            if (r >= q) break;
+            color = divs(color, q);
+        }
+    }
+    return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+// descs: offset, width, height per texture.
+void* orc_scene_create(const float* verts, uint64_t nverts, const uint32_t* idx, uint64_t ntri,
+                       const uint32_t* matidx, const uint32_t* mats, uint64_t nmat,
+                       const uint32_t* texels, uint64_t ntexels, const uint32_t* descs,
+                       uint32_t ntex, uint32_t diffuse_base, uint32_t emission_base,
+                       uint32_t attr_base, uint32_t recolour, uint32_t bg_base) {
+    Scene* s = new Scene();
+    s->verts.assign(verts, verts + 4 * nverts);
+    if (idx) s->idx.assign(idx, idx + 3 * ntri);
+    s->ntri = ntri;
+    s->matidx.assign(matidx, matidx + ntri);
+    s->mats.assign(mats, mats + 8 * nmat);
+    s->texels.assign(texels, texels + ntexels);
+    s->descs.assign(descs, descs + 3 * ntex);
+    s->diffuse_base = diffuse_base;
+    s->emission_base = emission_base;
+    s->attr_base = attr_base;
+    s->recolour = recolour;
+    s->bg_base = bg_base;
+    s->bvh = orc_bvh_create(s->verts.data(), nverts, idx ? s->idx.data() : nullptr, ntri);
+    return s;
+}
+
+void orc_scene_destroy(void* h) {
+    Scene* s = (Scene*)h;
+    orc_bvh_destroy(s->bvh);
+    delete s;
+}
+
+// Plain-mode trace of a width x height image (or of the pixel rows
+// [row0, row0 + rows)).  Outputs rgba per pixel of the traced rows; colour is
+// max(mean, 0) (general.wesl:592), accumulated in float64 when fp64 != 0.
+// Returns the number of path vertices (closest-hit queries).
+uint64_t orc_trace_plain(void* h, const float* cam, uint32_t width, uint32_t height,
+                         uint32_t row0, uint32_t rows, uint32_t quality, uint32_t samples,
+                         float tmin, float tmax, uint32_t bounces, uint32_t rr_start,
+                         uint32_t seed_offset, int fp64, float* out_color, float* out_normal,
+                         float* out_albedo) {
+    const Scene& s = *(const Scene*)h;
+    PlainOpts o{quality, bounces, rr_start, tmin, tmax};
+    uint64_t total = 0;
+    const int64_t n = (int64_t)width * rows;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+    for (int64_t k = 0; k < n; ++k) {
+        uint32_t x = (uint32_t)(k % width), y = row0 + (uint32_t)(k / width);
+        float o3[3], d3[3];
+        orc_create_ray(cam, x, y, width, height, o3, d3);
+        uint32_t own_seed = orc_rand_u32(x * y) + seed_offset + y;
+        double acc[3] = {0, 0, 0};
+        F3 accf = splat(0.0f);
+        SampleOut last{};
+        for (uint32_t i = 0; i < samples; ++i) {
+            SampleOut so = rt_sample_plain(s, o, mk(o3[0], o3[1], o3[2]), mk(d3[0], d3[1], d3[2]),
+                                           own_seed);
+            total += so.vertices;
+            if (fp64) {
+                acc[0] += so.color.x;
+                acc[1] += so.color.y;
+                acc[2] += so.color.z;
+            } else {
+                accf = add(accf, so.color);
+            }
+            last = so;
+            own_seed = orc_rand_u32(own_seed);
+        }
+        float c[3];
+        if (fp64) {
+            for (int j = 0; j < 3; ++j) c[j] = (float)std::fmax(acc[j] / (double)samples, 0.0);
+        } else {
+            c[0] = std::fmax(accf.x / (float)samples, 0.0f);
+            c[1] = std::fmax(accf.y / (float)samples, 0.0f);
+            c[2] = std::fmax(accf.z / (float)samples, 0.0f);
+        }
+        float* oc = out_color + 4 * k;
+        oc[0] = c[0];
+        oc[1] = c[1];
+        oc[2] = c[2];
+        oc[3] = 1.0f;
+        float* on = out_normal + 4 * k;
+        on[0] = last.normal.x;
+        on[1] = last.normal.y;
+        on[2] = last.normal.z;
+        on[3] = 1.0f;
+        float* oa = out_albedo + 4 * k;
+        oa[0] = last.albedo.x;
+        oa[1] = last.albedo.y;
+        oa[2] = last.albedo.z;
+        oa[3] = 1.0f;
+    }
+    return total;
+}
+
+// Test hooks for single functions.
+void orc_oren_nayar(const float* incoming, const float* reflected, float roughness,
+                    const float* color, float* out) {
+    F3 r = oren_nayar(mk(incoming[0], incoming[1], incoming[2]),
+                      mk(reflected[0], reflected[1], reflected[2]), roughness,
+                      mk(color[0], color[1], color[2]));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+float orc_reflectance(float cosine, float ref_idx) { return reflectance(cosine, ref_idx); }
+void orc_sample_vndf_ggx(uint32_t seed, const float* view, float rx, float ry, float* out) {
+    F3 r = sample_vndf_ggx(seed, mk(view[0], view[1], view[2]), rx, ry);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+void orc_cosine_hemisphere(uint32_t seed, const float* normal, const float* tangent, float* out) {
+    F3 r = cosine_weighted_hemisphere(seed, mk(normal[0], normal[1], normal[2]),
+                                      mk(tangent[0], tangent[1], tangent[2]));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+void orc_high_colour(uint32_t seed, float* out) {
+    float picked = -1.0f;
+    F3 c;
+    pick_ref_idx(2, &picked, 1.0f, 1.0f, seed, &c);
+    out[0] = c.x;
+    out[1] = c.y;
+    out[2] = c.z;
+    out[3] = picked;
+}
+void orc_background(void* h, const float* d, float* out) {
+    F3 c = background(*(const Scene*)h, mk(d[0], d[1], d[2]));
+    out[0] = c.x;
+    out[1] = c.y;
+    out[2] = c.z;
+}
+
+}  // extern "C"
