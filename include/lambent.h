@@ -268,6 +268,22 @@ lmb_status lmb_data_buffers_write_state(lmb_data_buffers bufs, uint32_t which,
 lmb_status lmb_data_buffers_path_vertices(lmb_data_buffers bufs,
                                           uint64_t *vertices);
 
+/* Kernel timing (no counterpart in the reference, which has no GPU timing;
+ * this is its profiling hook).  When enabled, every traversal ("extend") and
+ * shading launch of lmb_trace is bracketed by HIP events on its stream.
+ * lmb_data_buffers_kernel_stats waits for them, reports the totals since the
+ * previous call (or since enabling) and resets them. */
+typedef struct lmb_kernel_stats {
+    double extend_ms;          /* summed duration of the traversal launches */
+    double shade_ms;           /* summed duration of the shading launches   */
+    uint64_t extend_launches;
+    uint64_t shade_launches;
+    uint64_t path_vertices;    /* closest-hit queries issued meanwhile      */
+} lmb_kernel_stats;
+lmb_status lmb_data_buffers_set_profiling(lmb_data_buffers bufs, int32_t enable);
+lmb_status lmb_data_buffers_kernel_stats(lmb_data_buffers bufs,
+                                         lmb_kernel_stats *stats);
+
 /* ------------------------------------------------------------------ */
 /* Frame passes                                                        */
 /* ------------------------------------------------------------------ */

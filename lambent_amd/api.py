@@ -117,6 +117,12 @@ class _BvhStatsC(ctypes.Structure):
                 ("device_bytes", ctypes.c_uint64)]
 
 
+class _KernelStatsC(ctypes.Structure):
+    _fields_ = [("extend_ms", ctypes.c_double), ("shade_ms", ctypes.c_double),
+                ("extend_launches", ctypes.c_uint64), ("shade_launches", ctypes.c_uint64),
+                ("path_vertices", ctypes.c_uint64)]
+
+
 class _TracerOptsC(ctypes.Structure):
     _fields_ = [("quality", ctypes.c_uint32), ("samples", ctypes.c_uint32),
                 ("t_min", ctypes.c_float), ("t_max", ctypes.c_float),
@@ -172,6 +178,8 @@ _OPTIONAL = {
     "lmb_data_buffers_write_state": ([_vp, ctypes.c_uint32, _vp, ctypes.c_uint64],
                                      ctypes.c_int32),
     "lmb_data_buffers_path_vertices": ([_vp, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int32),
+    "lmb_data_buffers_set_profiling": ([_vp, ctypes.c_int32], ctypes.c_int32),
+    "lmb_data_buffers_kernel_stats": ([_vp, ctypes.POINTER(_KernelStatsC)], ctypes.c_int32),
     "lmb_trace": ([_vp, _vp, ctypes.POINTER(_CameraC), _vp, ctypes.c_uint32, _vp],
                   ctypes.c_int32),
     "lmb_temporal_resample": ([_vp, ctypes.c_uint32, _vp], ctypes.c_int32),
@@ -747,6 +755,16 @@ class DataBuffers:
     def write_state(self, which: StateBuffer, data: bytes) -> None:
         buf = ctypes.create_string_buffer(bytes(data), len(data))
         _check(_lib.lmb_data_buffers_write_state(self._h, int(which), buf, len(data)))
+
+    def set_profiling(self, enable: bool = True) -> None:
+        _check(_lib.lmb_data_buffers_set_profiling(self._h, 1 if enable else 0))
+
+    def kernel_stats(self) -> dict:
+        """Summed extend / shade launch durations (HIP events) since the last
+        call, and the path vertices issued meanwhile."""
+        st = _KernelStatsC()
+        _check(_lib.lmb_data_buffers_kernel_stats(self._h, ctypes.byref(st)))
+        return {n: getattr(st, n) for n, _ in _KernelStatsC._fields_}
 
     def path_vertices(self) -> int:
         n = ctypes.c_uint64()

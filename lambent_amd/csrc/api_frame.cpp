@@ -3,7 +3,9 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstring>
+#include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/lambent.h"
 #include "lmb_frame_dev.h"
@@ -15,7 +17,72 @@ struct lmb_tracer_s {
     lmb_tracer_opts opts;
 };
 
+namespace {
+
+// Records a start/stop event pair around each timed launch; events are
+// pooled and resolved when the stats are read.
+class EventTimer final : public KernelTimer {
+  public:
+    ~EventTimer() override {
+        for (auto& s : pool_) {
+            (void)hipEventDestroy(s.start);
+            (void)hipEventDestroy(s.stop);
+        }
+    }
+    void begin(int kind, hipStream_t stream) override {
+        if (used_ == pool_.size()) {
+            Slot s{};
+            if (hipEventCreate(&s.start) != hipSuccess || hipEventCreate(&s.stop) != hipSuccess) {
+                failed_ = true;
+                return;
+            }
+            pool_.push_back(s);
+        }
+        pool_[used_].kind = kind;
+        if (hipEventRecord(pool_[used_].start, stream) != hipSuccess) failed_ = true;
+    }
+    void end(int kind, hipStream_t stream) override {
+        (void)kind;
+        if (used_ >= pool_.size()) return;
+        if (hipEventRecord(pool_[used_].stop, stream) != hipSuccess) failed_ = true;
+        ++used_;
+    }
+    // Sums the recorded durations per kind and resets.
+    bool collect(double ms[kNumKernelKinds], uint64_t launches[kNumKernelKinds]) {
+        for (int k = 0; k < kNumKernelKinds; ++k) {
+            ms[k] = 0.0;
+            launches[k] = 0;
+        }
+        bool ok = !failed_;
+        for (size_t i = 0; i < used_; ++i) {
+            float t = 0.f;
+            if (hipEventSynchronize(pool_[i].stop) != hipSuccess ||
+                hipEventElapsedTime(&t, pool_[i].start, pool_[i].stop) != hipSuccess) {
+                ok = false;
+                continue;
+            }
+            ms[pool_[i].kind] += t;
+            launches[pool_[i].kind] += 1;
+        }
+        used_ = 0;
+        failed_ = false;
+        return ok;
+    }
+
+  private:
+    struct Slot {
+        hipEvent_t start, stop;
+        int kind;
+    };
+    std::vector<Slot> pool_;
+    size_t used_ = 0;
+    bool failed_ = false;
+};
+
+}  // namespace
+
 struct lmb_data_buffers_s {
+    std::unique_ptr<EventTimer> timer;
     int device = 0;
     uint32_t width = 0, height = 0, world_size = 0, flags = 0;
     DeviceBuffer out_color, out_normal, out_albedo;
@@ -256,6 +323,39 @@ lmb_status lmb_data_buffers_path_vertices(lmb_data_buffers b, uint64_t* vertices
     return LMB_OK;
 }
 
+lmb_status lmb_data_buffers_set_profiling(lmb_data_buffers b, int32_t enable) {
+    if (!b) return fail(LMB_ERR_INVALID_ARGUMENT, "buffers is NULL");
+    if (enable && !b->timer) {
+        b->timer.reset(new EventTimer());
+        hipError_t e = hipMemset(b->dev.counters + kCounterVerticesTotal, 0, 8);
+        if (e != hipSuccess) return fail_hip(e, "hipMemset");
+    } else if (!enable) {
+        b->timer.reset();
+    }
+    return LMB_OK;
+}
+
+lmb_status lmb_data_buffers_kernel_stats(lmb_data_buffers b, lmb_kernel_stats* stats) {
+    if (!b || !stats) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!b->timer) return fail(LMB_ERR_INVALID_ARGUMENT, "profiling is not enabled");
+    double ms[kNumKernelKinds];
+    uint64_t n[kNumKernelKinds];
+    bool ok = b->timer->collect(ms, n);
+    uint64_t verts = 0;
+    hipError_t e = hipMemcpy(&verts, b->dev.counters + kCounterVerticesTotal, 8,
+                             hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy counter");
+    e = hipMemset(b->dev.counters + kCounterVerticesTotal, 0, 8);
+    if (e != hipSuccess) return fail_hip(e, "hipMemset");
+    if (!ok) return fail(LMB_ERR_DEVICE, "a timing event failed");
+    stats->extend_ms = ms[kKernelExtend];
+    stats->shade_ms = ms[kKernelShade];
+    stats->extend_launches = n[kKernelExtend];
+    stats->shade_launches = n[kKernelShade];
+    stats->path_vertices = verts;
+    return LMB_OK;
+}
+
 lmb_status lmb_trace(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
                      lmb_data_buffers bufs, uint32_t seed_offset, void* stream) {
     if (!tracer || !scene || !cam || !bufs) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
@@ -275,7 +375,8 @@ lmb_status lmb_trace(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
     p.seed_offset = seed_offset;
     CameraDev cd;
     std::memcpy(&cd, cam, sizeof cd);
-    hipError_t e = launch_trace(scene->dev, cd, bufs->dev, p, (hipStream_t)stream);
+    hipError_t e =
+        launch_trace(scene->dev, cd, bufs->dev, p, (hipStream_t)stream, bufs->timer.get());
     if (e != hipSuccess) return fail_hip(e, "trace launch");
     return LMB_OK;
 }

@@ -12,9 +12,11 @@ namespace lmb {
 
 constexpr uint32_t kMaxBounces = 64;
 // counters[]: [0, kMaxBounces] queue length per bounce;
-// [kCounterVertices] (64-bit, two words) path vertices of the current trace.
+// [kCounterVertices] (64-bit, two words) path vertices of the current trace;
+// [kCounterVerticesTotal] (64-bit) path vertices since the last stats read.
 constexpr uint32_t kCounterVertices = kMaxBounces + 2;
-constexpr uint32_t kNumCounters = kMaxBounces + 4;
+constexpr uint32_t kCounterVerticesTotal = kMaxBounces + 4;
+constexpr uint32_t kNumCounters = kMaxBounces + 6;
 
 struct FrameDev {
     uint32_t width, height;
@@ -61,8 +63,16 @@ struct TraceParams {
 // Threads of the persistent wavefront kernels for a frame of npix paths.
 uint64_t frame_threads(uint64_t npix);
 
+// Optional per-launch timing hooks (HIP events recorded on the launch stream).
+enum KernelKind { kKernelExtend = 0, kKernelShade = 1, kKernelOther = 2, kNumKernelKinds = 3 };
+struct KernelTimer {
+    virtual void begin(int kind, hipStream_t stream) = 0;
+    virtual void end(int kind, hipStream_t stream) = 0;
+    virtual ~KernelTimer() = default;
+};
+
 // One full trace (all samples, all bounces) on `stream`.
 hipError_t launch_trace(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
-                        const TraceParams& p, hipStream_t stream);
+                        const TraceParams& p, hipStream_t stream, KernelTimer* timer);
 
 }  // namespace lmb

@@ -77,7 +77,11 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(SceneDev s, FrameDev f, Tra
     const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
     const uint32_t count = f.counters[bounce];
-    if (gtid == 0) atomicAdd((unsigned long long*)vertex_counter(f), (unsigned long long)count);
+    if (gtid == 0) {
+        atomicAdd((unsigned long long*)vertex_counter(f), (unsigned long long)count);
+        atomicAdd((unsigned long long*)(f.counters + kCounterVerticesTotal),
+                  (unsigned long long)count);
+    }
     const float4* __restrict__ q = f.queue[bounce & 1];
     ShortStack<kPtBlock> st;
     st.lds = lds_stack + threadIdx.x;
@@ -247,13 +251,17 @@ __global__ __launch_bounds__(kPtBlock) void k_resolve(FrameDev f, TraceParams p)
 
 template <uint32_t Q>
 static hipError_t launch_plain(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
-                               const TraceParams& p, hipStream_t stream) {
+                               const TraceParams& p, hipStream_t stream, KernelTimer* tm) {
     const dim3 grid((uint32_t)(frame_threads(f.npix) / kPtBlock)), block(kPtBlock);
     for (uint32_t sample = 0; sample < p.samples; ++sample) {
         hipLaunchKernelGGL(k_generate, grid, block, 0, stream, cam, f, p, sample);
         for (uint32_t b = 0; b < p.bounces; ++b) {
+            if (tm) tm->begin(kKernelExtend, stream);
             hipLaunchKernelGGL(k_extend, grid, block, 0, stream, s, f, p, b);
+            if (tm) tm->end(kKernelExtend, stream);
+            if (tm) tm->begin(kKernelShade, stream);
             hipLaunchKernelGGL(k_shade_plain<Q>, grid, block, 0, stream, s, f, p, b);
+            if (tm) tm->end(kKernelShade, stream);
         }
     }
     hipLaunchKernelGGL(k_resolve, grid, block, 0, stream, f, p);
@@ -261,15 +269,15 @@ static hipError_t launch_plain(const SceneDev& s, const CameraDev& cam, const Fr
 }
 
 hipError_t launch_trace(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
-                        const TraceParams& p, hipStream_t stream) {
+                        const TraceParams& p, hipStream_t stream, KernelTimer* timer) {
     if (f.npix == 0) return hipSuccess;
     switch (p.quality) {
         case kQualityLow:
-            return launch_plain<kQualityLow>(s, cam, f, p, stream);
+            return launch_plain<kQualityLow>(s, cam, f, p, stream, timer);
         case kQualityHigh:
-            return launch_plain<kQualityHigh>(s, cam, f, p, stream);
+            return launch_plain<kQualityHigh>(s, cam, f, p, stream, timer);
         default:
-            return launch_plain<kQualityMedium>(s, cam, f, p, stream);
+            return launch_plain<kQualityMedium>(s, cam, f, p, stream, timer);
     }
 }
 

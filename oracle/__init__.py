@@ -1,0 +1,245 @@
+"""CPU ORACLE - test infrastructure only.
+
+ctypes bindings of liboracle.so (oracle_geom.cpp, oracle_pt.cpp), the CPU
+restatement of the reference's path-tracing shaders.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as the checker
+or the CPU baseline; the product library never does.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+
+def _load():
+    if not LIB_PATH.exists():
+        from .build import build
+        build(verbose=False)
+    return ctypes.CDLL(str(LIB_PATH))
+
+
+_lib = _load()
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+
+
+def _sig(name, args, res=None):
+    fn = getattr(_lib, name)
+    fn.argtypes = args
+    fn.restype = res
+    return fn
+
+
+_sig("orc_rand_u32", [ctypes.c_uint32], ctypes.c_uint32)
+_sig("orc_rand_f32", [ctypes.c_uint32], ctypes.c_float)
+_sig("orc_rand_u32_many", [_u32p, ctypes.c_uint64, _u32p])
+_sig("orc_rand_f32_many", [_u32p, ctypes.c_uint64, _f32p])
+_sig("orc_create_ray", [_f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                        _f32p, _f32p])
+_sig("orc_create_rays", [_f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                         _f32p])
+_sig("orc_intersect_brute", [_f32p, _u32p, ctypes.c_uint64, _f32p, ctypes.c_uint64, _vp])
+_sig("orc_bvh_create", [_f32p, ctypes.c_uint64, _u32p, ctypes.c_uint64], _vp)
+_sig("orc_bvh_destroy", [_vp])
+_sig("orc_bvh_intersect", [_vp, _f32p, ctypes.c_uint64, _vp])
+_sig("orc_num_threads", [], ctypes.c_int)
+_sig("orc_scene_create", [_f32p, ctypes.c_uint64, _u32p, ctypes.c_uint64, _u32p, _u32p,
+                          ctypes.c_uint64, _u32p, ctypes.c_uint64, _u32p, ctypes.c_uint32,
+                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                          ctypes.c_uint32], _vp)
+_sig("orc_scene_destroy", [_vp])
+_sig("orc_trace_plain", [_vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                         ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                         ctypes.c_int, _f32p, _f32p, _f32p], ctypes.c_uint64)
+_sig("orc_oren_nayar", [_f32p, _f32p, ctypes.c_float, _f32p, _f32p])
+_sig("orc_reflectance", [ctypes.c_float, ctypes.c_float], ctypes.c_float)
+_sig("orc_sample_vndf_ggx", [ctypes.c_uint32, _f32p, ctypes.c_float, ctypes.c_float, _f32p])
+_sig("orc_cosine_hemisphere", [ctypes.c_uint32, _f32p, _f32p, _f32p])
+_sig("orc_high_colour", [ctypes.c_uint32, _f32p])
+_sig("orc_background", [_vp, _f32p, _f32p])
+
+HIT_DTYPE = np.dtype([("prim", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+MISS = 0xFFFFFFFF
+
+
+def _f(a):
+    return a.ctypes.data_as(_f32p)
+
+
+def _u(a):
+    return a.ctypes.data_as(_u32p)
+
+
+def num_threads() -> int:
+    return _lib.orc_num_threads()
+
+
+def rand_u32(seeds) -> np.ndarray:
+    s = np.ascontiguousarray(seeds, np.uint32).reshape(-1)
+    out = np.empty_like(s)
+    _lib.orc_rand_u32_many(_u(s), len(s), _u(out))
+    return out
+
+
+def rand_f32(seeds) -> np.ndarray:
+    s = np.ascontiguousarray(seeds, np.uint32).reshape(-1)
+    out = np.empty(len(s), np.float32)
+    _lib.orc_rand_f32_many(_u(s), len(s), _f(out))
+    return out
+
+
+def camera_floats(camera_bytes: bytes) -> np.ndarray:
+    a = np.frombuffer(camera_bytes, np.float32)
+    assert a.size == 32
+    return np.ascontiguousarray(a)
+
+
+def create_rays(camera_bytes: bytes, width: int, height: int, tmin=1e-4, tmax=1000.0):
+    cam = camera_floats(camera_bytes)
+    rays = np.empty((width * height, 8), np.float32)
+    _lib.orc_create_rays(_f(cam), width, height, tmin, tmax, _f(rays))
+    return rays
+
+
+def _vidx(vertices, indices):
+    v = np.ascontiguousarray(vertices, np.float32)
+    if v.shape[1] == 3:
+        v = np.ascontiguousarray(np.concatenate([v, np.zeros((len(v), 1), np.float32)], 1))
+    idx = None if indices is None else np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    ntri = len(idx) // 3 if idx is not None else len(v) // 3
+    return v, idx, ntri
+
+
+def intersect_brute(vertices, indices, rays) -> np.ndarray:
+    v, idx, ntri = _vidx(vertices, indices)
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    hits = np.empty(len(r), HIT_DTYPE)
+    _lib.orc_intersect_brute(_f(v), None if idx is None else _u(idx), ntri, _f(r), len(r),
+                             hits.ctypes.data_as(_vp))
+    return hits
+
+
+class Bvh:
+    def __init__(self, vertices, indices):
+        v, idx, ntri = _vidx(vertices, indices)
+        self._keep = (v, idx)
+        self._h = _lib.orc_bvh_create(_f(v), len(v), None if idx is None else _u(idx), ntri)
+
+    def intersect(self, rays) -> np.ndarray:
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.empty(len(r), HIT_DTYPE)
+        _lib.orc_bvh_intersect(self._h, _f(r), len(r), hits.ctypes.data_as(_vp))
+        return hits
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.orc_bvh_destroy(self._h)
+            self._h = None
+
+
+def _solid(rgba):
+    return np.asarray(rgba, np.uint8).reshape(1, 1, 4)
+
+
+class Scene:
+    """Oracle copy of a scene: same vertices, materials and texel values.
+
+    Texture arrays follow the order diffuse[], emission[], attributes[],
+    recolour (default 1x1 white), 6 background faces (default black)."""
+
+    def __init__(self, spec):
+        v, idx, ntri = _vidx(spec.vertices, spec.indices)
+        images = list(spec.diffuse) + list(spec.emission) + list(spec.attributes)
+        diffuse_base = 0
+        emission_base = len(spec.diffuse)
+        attr_base = emission_base + len(spec.emission)
+        recolour = len(images)
+        images.append(spec.recolour if spec.recolour is not None else _solid((255, 255, 255, 255)))
+        bg_base = len(images)
+        if spec.background is not None:
+            images.extend(list(spec.background))
+        else:
+            images.extend([_solid((0, 0, 0, 255))] * 6)
+        descs, pools, off = [], [], 0
+        for im in images:
+            a = np.ascontiguousarray(im, np.uint8)
+            h, w = a.shape[:2]
+            descs += [off, w, h]
+            pools.append(a.reshape(-1, 4).view(np.uint32).reshape(-1))
+            off += w * h
+        texels = np.ascontiguousarray(np.concatenate(pools))
+        d = np.asarray(descs, np.uint32)
+        mi = np.ascontiguousarray(spec.material_idx, np.uint32)
+        mats = np.ascontiguousarray(spec.materials, np.uint32).reshape(-1, 8)
+        self._keep = (v, idx, mi, mats, texels, d)
+        self._h = _lib.orc_scene_create(_f(v), len(v), None if idx is None else _u(idx), ntri,
+                                        _u(mi), _u(mats), len(mats), _u(texels), len(texels),
+                                        _u(d), len(images), diffuse_base, emission_base,
+                                        attr_base, recolour, bg_base)
+        self.spec = spec
+
+    def trace_plain(self, camera_bytes: bytes, width: int, height: int, quality: int = 1,
+                    samples: int = 1, tmin: float = 1e-4, tmax: float = 1000.0, bounces: int = 5,
+                    rr_start: Optional[int] = None, seed_offset: int = 0, fp64: bool = False,
+                    row0: int = 0, rows: Optional[int] = None):
+        """Returns (colour, normal, albedo, path_vertices) for rows
+        [row0, row0 + rows) of a width x height image."""
+        rows = height - row0 if rows is None else rows
+        cam = camera_floats(camera_bytes)
+        c = np.empty((rows, width, 4), np.float32)
+        n = np.empty_like(c)
+        a = np.empty_like(c)
+        verts = _lib.orc_trace_plain(self._h, _f(cam), width, height, row0, rows, quality, samples,
+                                     tmin, tmax, bounces,
+                                     0xFFFFFFFF if rr_start is None else rr_start, seed_offset,
+                                     1 if fp64 else 0, _f(c), _f(n), _f(a))
+        return c, n, a, verts
+
+    def background(self, direction) -> np.ndarray:
+        d = np.ascontiguousarray(direction, np.float32)
+        out = np.empty(3, np.float32)
+        _lib.orc_background(self._h, _f(d), _f(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.orc_scene_destroy(self._h)
+            self._h = None
+
+
+def oren_nayar(incoming, reflected, roughness, color) -> np.ndarray:
+    out = np.empty(3, np.float32)
+    _lib.orc_oren_nayar(_f(np.asarray(incoming, np.float32)), _f(np.asarray(reflected, np.float32)),
+                        roughness, _f(np.asarray(color, np.float32)), _f(out))
+    return out
+
+
+def reflectance(cosine: float, ref_idx: float) -> float:
+    return _lib.orc_reflectance(cosine, ref_idx)
+
+
+def sample_vndf_ggx(seed: int, view, rx: float, ry: float) -> np.ndarray:
+    out = np.empty(3, np.float32)
+    _lib.orc_sample_vndf_ggx(seed, _f(np.asarray(view, np.float32)), rx, ry, _f(out))
+    return out
+
+
+def cosine_hemisphere(seed: int, normal, tangent) -> np.ndarray:
+    out = np.empty(3, np.float32)
+    _lib.orc_cosine_hemisphere(seed, _f(np.asarray(normal, np.float32)),
+                               _f(np.asarray(tangent, np.float32)), _f(out))
+    return out
+
+
+def high_colour(seed: int):
+    out = np.empty(4, np.float32)
+    _lib.orc_high_colour(seed, _f(out))
+    return out[:3], float(out[3])

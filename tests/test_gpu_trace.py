@@ -1,0 +1,104 @@
+"""GPU parity of the wavefront path tracer (plain mode) against the oracle.
+
+* closed furnaces: every path's radiance is E * sum_{k<D} rho^k exactly, so
+  each pixel must match the closed form to float rounding;
+* same seeds: the GPU image and the oracle image of the Cornell box agree per
+  pixel except where a transcendental-function ulp flips a branch;
+* converged images: relMSE against the oracle's float64-accumulated image;
+* Russian roulette is unbiased (mean radiance within the Monte-Carlo error).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from lambent_amd import api, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _trace(spec, quality=api.Medium, samples=1, bounces=5, rr_start=None, seed_offset=0,
+           scene=None):
+    scene = scene or spec.build()
+    pipe = api.RayTracer(quality).create_pipeline(
+        api.PathTracerOpts(samples=samples, bounces=bounces, rr_start=rr_start,
+                           guiding=api.Guiding.OFF))
+    bufs = api.DataBuffers(spec.width, spec.height, 1)
+    pipe.dispatch(scene, spec.camera(), bufs, seed_offset)
+    return (bufs.output(api.Output.COLOR), bufs.output(api.Output.NORMAL),
+            bufs.output(api.Output.ALBEDO), bufs.path_vertices())
+
+
+def rel_mse(img, ref):
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+
+
+@pytest.mark.parametrize("quality", [api.Low, api.Medium, api.High])
+@pytest.mark.parametrize("ty", [api.MaterialType.Diffuse, api.MaterialType.Metallic])
+def test_furnace_closed_form(quality, ty):
+    rho = 0.6
+    spec = scenes.furnace_box(rho, 1.5, ty=ty, width=48, height=40)
+    color, normal, albedo, verts = _trace(spec, quality, samples=3, bounces=6)
+    r = float(np.float32(round(rho * 255)) / np.float32(255))
+    expect = 1.5 * sum(r ** k for k in range(6))
+    if quality == api.High:
+        # the High quality weights each path by its wavelength's colour
+        c_o, _, _, _ = oracle.Scene(spec).trace_plain(spec.camera().to_bytes(), spec.width,
+                                                      spec.height, quality=2, samples=3,
+                                                      bounces=6)
+        np.testing.assert_allclose(color, c_o, rtol=2e-5, atol=1e-6)
+    else:
+        exact = np.all(np.abs(color[..., :3] - expect) <= 2e-6 * expect, axis=-1)
+        # a path whose vertex lands within T_MIN of a box edge can escape
+        # through the adjacent wall; it only loses energy
+        assert exact.mean() > 0.99, exact.mean()
+        assert (color[..., :3][~exact] < expect).all()
+    assert verts >= 0.99 * spec.width * spec.height * 3 * 6
+    np.testing.assert_allclose(albedo[..., :3], r, rtol=1e-6)
+    np.testing.assert_allclose(np.linalg.norm(normal[..., :3], axis=-1), 1.0, rtol=1e-5)
+
+
+@pytest.mark.parametrize("quality", [api.Low, api.Medium, api.High])
+def test_cornell_same_seeds_as_oracle(quality):
+    spec = scenes.cornell_box(128, 128)
+    color, normal, albedo, verts = _trace(spec, quality, samples=4, bounces=8, seed_offset=3)
+    c_o, n_o, a_o, v_o = oracle.Scene(spec).trace_plain(
+        spec.camera().to_bytes(), 128, 128, quality=int(quality), samples=4, bounces=8,
+        seed_offset=3)
+    np.testing.assert_array_equal(albedo, a_o)
+    np.testing.assert_allclose(normal, n_o, atol=1e-6)
+    close = np.all(np.abs(color - c_o) <= 1e-4 * (1 + np.abs(c_o)), axis=-1)
+    assert close.mean() > 0.97, f"only {close.mean():.4f} of pixels match the oracle"
+    assert abs(verts - v_o) <= 0.01 * v_o
+    assert rel_mse(color[..., :3], c_o[..., :3]) < 1e-3
+
+
+def test_procedural_same_seeds_as_oracle():
+    spec = scenes.procedural(50_000, seed=5, width=160, height=90)
+    color, normal, albedo, verts = _trace(spec, api.Medium, samples=2, bounces=5, rr_start=3)
+    c_o, n_o, a_o, v_o = oracle.Scene(spec).trace_plain(
+        spec.camera().to_bytes(), 160, 90, quality=1, samples=2, bounces=5, rr_start=3)
+    np.testing.assert_array_equal(albedo, a_o)
+    close = np.all(np.abs(color - c_o) <= 1e-4 * (1 + np.abs(c_o)), axis=-1)
+    assert close.mean() > 0.97, f"only {close.mean():.4f} of pixels match the oracle"
+    assert abs(verts - v_o) <= 0.02 * v_o
+
+
+def test_cornell_converges_to_fp64_oracle():
+    spec = scenes.cornell_box(32, 32)
+    c_ref, _, _, _ = oracle.Scene(spec).trace_plain(spec.camera().to_bytes(), 32, 32, quality=1,
+                                                    samples=4096, bounces=8, seed_offset=1000,
+                                                    fp64=True)
+    color, _, _, _ = _trace(spec, api.Medium, samples=1024, bounces=8)
+    err = rel_mse(color[..., :3], c_ref[..., :3])
+    assert err < 0.01, f"relMSE {err}"
+
+
+def test_russian_roulette_unbiased():
+    spec = scenes.cornell_box(32, 32)
+    scene = spec.build()
+    plain, _, _, v0 = _trace(spec, api.Medium, samples=2048, bounces=8, scene=scene)
+    rr, _, _, v1 = _trace(spec, api.Medium, samples=2048, bounces=8, rr_start=2,
+                          seed_offset=77, scene=scene)
+    assert v1 < v0
+    m0, m1 = plain[..., :3].mean(), rr[..., :3].mean()
+    assert abs(m1 - m0) / m0 < 0.02, (m0, m1)
