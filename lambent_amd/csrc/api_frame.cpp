@@ -10,6 +10,7 @@
 #include "../../include/lambent.h"
 #include "lmb_frame_dev.h"
 #include "lmb_internal.h"
+#include "lmb_restir.h"
 
 using namespace lmb;
 
@@ -162,6 +163,8 @@ void refresh_view(lmb_data_buffers b) {
     f.wmc = b->wmc.as<WorldMarkovChain>();
     f.wmc_old = b->wmc_old.as<WorldMarkovChain>();
     f.world_size = (uint32_t)b->wmc_count;
+    f.gi_count = b->gi_count;
+    f.mc_count = b->mc_count;
     f.flags = b->flags;
 }
 
@@ -381,30 +384,80 @@ lmb_status lmb_trace(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
     return LMB_OK;
 }
 
+static lmb_status need_spatial_buffers(lmb_data_buffers b) {
+    if (b->gi_count < (uint64_t)b->width * b->height)
+        return fail(LMB_ERR_INVALID_ARGUMENT,
+                    "resampling needs per-pixel reservoirs: create the buffers with "
+                    "LMB_BUFFER_SPATIAL_RESAMPLING");
+    return LMB_OK;
+}
+
 lmb_status lmb_temporal_resample(lmb_data_buffers bufs, uint32_t seed_offset, void* stream) {
-    (void)seed_offset;
-    (void)stream;
     if (!bufs) return fail(LMB_ERR_INVALID_ARGUMENT, "buffers is NULL");
-    return fail(LMB_ERR_INVALID_ARGUMENT, "temporal resampling is not available in this build");
+    lmb_status st = need_spatial_buffers(bufs);
+    if (st) return st;
+    hipError_t e = launch_temporal(bufs->dev, seed_offset, (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "temporal resampling launch");
+    return LMB_OK;
 }
 
 lmb_status lmb_spatial_resample(lmb_data_buffers bufs, uint32_t is_samples, uint32_t is_space,
                                 uint32_t seed_offset, void* stream) {
-    (void)is_samples;
-    (void)is_space;
-    (void)seed_offset;
-    (void)stream;
     if (!bufs) return fail(LMB_ERR_INVALID_ARGUMENT, "buffers is NULL");
-    return fail(LMB_ERR_INVALID_ARGUMENT, "spatial resampling is not available in this build");
+    if (is_space < 2 || is_space > 4096)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "IS_SPACE must be in 2..4096");
+    lmb_status st = need_spatial_buffers(bufs);
+    if (st) return st;
+    hipError_t e = launch_spatial(bufs->dev, is_samples, is_space, seed_offset, (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "spatial resampling launch");
+    return LMB_OK;
 }
 
-lmb_status lmb_advance_frame(lmb_data_buffers bufs, uint32_t buffers_to_advance,
+// data_buffer/mod.rs:272-349.  "Copy current to old, clear current" is a
+// pointer swap plus a clear; the world chains are copied (they persist).
+lmb_status lmb_advance_frame(lmb_data_buffers b, uint32_t buffers_to_advance,
                              uint32_t confidence_cap, void* stream) {
-    (void)buffers_to_advance;
-    (void)confidence_cap;
-    (void)stream;
-    if (!bufs) return fail(LMB_ERR_INVALID_ARGUMENT, "buffers is NULL");
-    return fail(LMB_ERR_INVALID_ARGUMENT, "advance_frame is not available in this build");
+    if (!b) return fail(LMB_ERR_INVALID_ARGUMENT, "buffers is NULL");
+    if (confidence_cap == 0 || confidence_cap > 0xFFFF)  // NonZeroU16
+        return fail(LMB_ERR_INVALID_ARGUMENT, "confidence_cap must be in 1..65535");
+    if (buffers_to_advance & ~LMB_BUFFER_ALL)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "bad buffer flags");
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = hipSuccess;
+    auto swap_buf = [](DeviceBuffer& a, DeviceBuffer& c) {
+        std::swap(a.ptr, c.ptr);
+        std::swap(a.bytes, c.bytes);
+    };
+    if (buffers_to_advance & LMB_BUFFER_SPATIAL_RESAMPLING) {
+        e = launch_eof_gi(b->gi.as<Reservoir>(), b->gi_count, confidence_cap, s);
+        if (e == hipSuccess) {
+            swap_buf(b->gi, b->gi_old);
+            e = launch_fill_zero(b->gi.ptr, b->gi.bytes, s);
+        }
+    } else {
+        e = launch_fill_zero(b->gi.ptr, b->gi.bytes, s);
+        if (e == hipSuccess) e = launch_fill_zero(b->gi_old.ptr, b->gi_old.bytes, s);
+    }
+    if (e != hipSuccess) return fail_hip(e, "advance_frame (reservoirs)");
+    if (buffers_to_advance & LMB_BUFFER_MARKOV_CHAIN_SCREEN_SPACE) {
+        swap_buf(b->mc, b->mc_old);
+        e = launch_fill_zero(b->mc.ptr, b->mc.bytes, s);
+    } else {
+        e = launch_fill_zero(b->mc.ptr, b->mc.bytes, s);
+        if (e == hipSuccess) e = launch_fill_zero(b->mc_old.ptr, b->mc_old.bytes, s);
+    }
+    if (e != hipSuccess) return fail_hip(e, "advance_frame (screen chains)");
+    if (buffers_to_advance & LMB_BUFFER_MARKOV_CHAIN_WORLD_SPACE) {
+        e = launch_eof_world(b->wmc.as<WorldMarkovChain>(), b->wmc_count, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->wmc_old.ptr, b->wmc.ptr, b->wmc.bytes, hipMemcpyDeviceToDevice, s);
+    } else {
+        e = launch_fill_zero(b->wmc.ptr, b->wmc.bytes, s);
+        if (e == hipSuccess) e = launch_fill_zero(b->wmc_old.ptr, b->wmc_old.bytes, s);
+    }
+    if (e != hipSuccess) return fail_hip(e, "advance_frame (world chains)");
+    refresh_view(b);
+    return LMB_OK;
 }
 
 }  // extern "C"

@@ -184,11 +184,12 @@ inline bool finite3(const float* v) {
 }
 
 // Quantization scale exponent for one axis of a node covering [lo, hi]:
-// 2^e >= extent / 254 so that q_hi <= 255, and 2^e >= max|coord| / 2^22 so
-// that every dequantized plane (k + q) * 2^e is an exact float.
+// 2^e >= extent / 253 so that q_hi <= 255 with a plane to spare, and
+// 2^e >= max|coord| / 2^22 so that every dequantized plane (k + q) * 2^e is
+// an exact float.
 int pick_exponent(float lo, float hi) {
     double extent = (double)hi - (double)lo;
-    double need = std::max(extent / 254.0,
+    double need = std::max(extent / 253.0,
                            std::max(std::fabs((double)lo), std::fabs((double)hi)) /
                                4194304.0);
     need = std::max(need, 1e-30);
@@ -359,10 +360,21 @@ int build_bvh8(const float* vertices, uint64_t num_vertices, const uint32_t* ind
                 double qh = std::ceil(((double)c.hi[a] - (double)node.p[a]) / s_a);
                 ql = std::min(std::max(ql, 0.0), 255.0);
                 qh = std::min(std::max(qh, 0.0), 255.0);
+                // (c - p) is not exact in double when |c| << |p|: step the
+                // planes outward until the kernel's expression encloses c.
+                while (ql > 0.0 && !(std::fmaf((float)ql, scale[a], node.p[a]) <= c.lo[a]))
+                    ql -= 1.0;
+                while (qh < 255.0 && !(std::fmaf((float)qh, scale[a], node.p[a]) >= c.hi[a]))
+                    qh += 1.0;
                 float dl = std::fmaf((float)ql, scale[a], node.p[a]);
                 float dh = std::fmaf((float)qh, scale[a], node.p[a]);
                 if (!(dl <= c.lo[a]) || !(dh >= c.hi[a])) {
-                    *err = "BVH quantization is not conservative (internal error)";
+                    *err = "BVH quantization is not conservative (internal error): axis " +
+                           std::to_string(a) + " child [" + std::to_string(c.lo[a]) + ", " +
+                           std::to_string(c.hi[a]) + "] dequantized [" + std::to_string(dl) +
+                           ", " + std::to_string(dh) + "] p " + std::to_string(node.p[a]) +
+                           " scale " + std::to_string(scale[a]) + " q " + std::to_string(ql) +
+                           " " + std::to_string(qh);
                     return -1;
                 }
                 node.qlo[a][s] = (uint8_t)ql;

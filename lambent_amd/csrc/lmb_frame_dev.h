@@ -45,6 +45,8 @@ struct FrameDev {
     MarkovChain* mc_old;
     WorldMarkovChain* wmc;
     WorldMarkovChain* wmc_old;
+    uint64_t gi_count;   // entries of gi / gi_old (1 unless spatial buffers exist)
+    uint64_t mc_count;   // entries of mc / mc_old
     uint32_t world_size;
     uint32_t flags;
 };

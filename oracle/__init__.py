@@ -243,3 +243,78 @@ def high_colour(seed: int):
     out = np.empty(4, np.float32)
     _lib.orc_high_colour(seed, _f(out))
     return out[:3], float(out[3])
+
+
+# --------------------------------------------------------- ReSTIR / EOF
+_sig("orc_normalised_to_u32", [_f32p], ctypes.c_uint32)
+_sig("orc_u32_to_normalised", [ctypes.c_uint32, _f32p])
+_sig("orc_merge", [_vp, _vp, ctypes.c_float, ctypes.c_uint32], ctypes.c_int)
+_sig("orc_update", [_vp, _vp, ctypes.c_float, ctypes.c_uint32])
+_sig("orc_temporal", [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32])
+_sig("orc_spatial", [_vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                     ctypes.c_uint32, ctypes.c_uint32, _f32p, ctypes.POINTER(ctypes.c_int32)])
+_sig("orc_eof_gi", [_vp, ctypes.c_uint64, ctypes.c_uint32])
+_sig("orc_eof_world", [_u32p, ctypes.c_uint64])
+
+# bindings.wesl:158-169 Reservoir (64 bytes)
+RESERVOIR_DTYPE = np.dtype([("visible_point", "<f4", 3), ("visible_normal", "<u4"),
+                            ("sample_point", "<f4", 3), ("sample_normal", "<u4"),
+                            ("out_radiance", "<f4", 3), ("ty", "<u4"), ("roughness", "<f4"),
+                            ("w", "<f4"), ("packed_confidence_valid", "<u4"), ("W", "<f4")])
+assert RESERVOIR_DTYPE.itemsize == 64
+# bindings.wesl:33-40 WorldMarcovChainState (56 bytes)
+WORLD_DTYPE = np.dtype([("secondary_hash", "<u4"), ("num_samples", "<u4"), ("luminance", "<f4", 3),
+                        ("lock", "<u4"), ("chain_light_source", "<f4", 3),
+                        ("chain_mean_cosine", "<f4"), ("chain_weight_sum", "<f4"),
+                        ("chain_num_samples", "<u4"), ("chain_score", "<f4"), ("timeout", "<u4")])
+assert WORLD_DTYPE.itemsize == 56
+
+
+def normalised_to_u32(n) -> int:
+    return int(_lib.orc_normalised_to_u32(_f(np.ascontiguousarray(n, np.float32))))
+
+
+def u32_to_normalised(u: int) -> np.ndarray:
+    out = np.empty(3, np.float32)
+    _lib.orc_u32_to_normalised(int(u), _f(out))
+    return out
+
+
+def merge(res: np.ndarray, r: np.ndarray, p: float, seed: int) -> bool:
+    """In place on a 1-element RESERVOIR_DTYPE array `res`."""
+    return bool(_lib.orc_merge(res.ctypes.data_as(_vp), r.ctypes.data_as(_vp), p, seed))
+
+
+def update(res: np.ndarray, s: np.ndarray, w: float, seed: int) -> None:
+    _lib.orc_update(res.ctypes.data_as(_vp), s.ctypes.data_as(_vp), w, seed)
+
+
+def temporal(gi: np.ndarray, gi_old: np.ndarray, width: int, height: int, seed_offset: int):
+    out = np.ascontiguousarray(gi.copy())
+    old = np.ascontiguousarray(gi_old)
+    _lib.orc_temporal(out.ctypes.data_as(_vp), old.ctypes.data_as(_vp), width, height, seed_offset)
+    return out
+
+
+def spatial(gi: np.ndarray, info: np.ndarray, width: int, height: int, is_samples: int,
+            is_space: int, seed_offset: int):
+    """Returns (rgba image, written mask)."""
+    g = np.ascontiguousarray(gi)
+    inf = np.ascontiguousarray(info, np.float32).reshape(-1)
+    out = np.zeros((height, width, 4), np.float32)
+    written = np.zeros((height, width), np.int32)
+    _lib.orc_spatial(g.ctypes.data_as(_vp), _f(inf), width, height, is_samples, is_space,
+                     seed_offset, _f(out), written.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return out, written.astype(bool)
+
+
+def eof_gi(res: np.ndarray, cap: int) -> np.ndarray:
+    out = np.ascontiguousarray(res.copy())
+    _lib.orc_eof_gi(out.ctypes.data_as(_vp), len(out), cap)
+    return out
+
+
+def eof_world(w: np.ndarray) -> np.ndarray:
+    out = np.ascontiguousarray(w.copy())
+    _lib.orc_eof_world(out.view(np.uint32).ctypes.data_as(_u32p), len(out))
+    return out

@@ -84,13 +84,24 @@ def test_procedural_same_seeds_as_oracle():
 
 
 def test_cornell_converges_to_fp64_oracle():
+    """GPU at 4096 spp vs the 4096-spp float64-accumulated oracle (other
+    seeds).  Both carry Monte-Carlo noise of the same size, so their relMSE
+    must match that of two independent GPU renders (ratio near 1; a bias
+    would add to it) and the image means must agree."""
     spec = scenes.cornell_box(32, 32)
+    scene = spec.build()
     c_ref, _, _, _ = oracle.Scene(spec).trace_plain(spec.camera().to_bytes(), 32, 32, quality=1,
                                                     samples=4096, bounces=8, seed_offset=1000,
                                                     fp64=True)
-    color, _, _, _ = _trace(spec, api.Medium, samples=1024, bounces=8)
-    err = rel_mse(color[..., :3], c_ref[..., :3])
-    assert err < 0.01, f"relMSE {err}"
+    a, _, _, _ = _trace(spec, api.Medium, samples=4096, bounces=8, seed_offset=0, scene=scene)
+    b, _, _, _ = _trace(spec, api.Medium, samples=4096, bounces=8, seed_offset=50_000,
+                        scene=scene)
+    e_ref = rel_mse(a[..., :3], c_ref[..., :3])
+    e_gpu = rel_mse(a[..., :3], b[..., :3])
+    assert e_ref < 1.5 * e_gpu + 1e-4, (e_ref, e_gpu)
+    assert e_ref < 0.02, e_ref
+    m_ref = float(c_ref[..., :3].mean())
+    assert abs(float(a[..., :3].mean()) - m_ref) / m_ref < 0.01
 
 
 def test_russian_roulette_unbiased():
