@@ -147,6 +147,15 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    # HBM-side bytes per k_extend launch from the last calibrated rocprofv3
+    # --pmc FETCH_SIZE / WRITE_SIZE runs of this workload (profiles/).
+    tfile = ROOT / "profiles" / "k_extend_traffic.json"
+    if tfile.exists() and args.triangles == 1_000_000 and (args.width, args.height) == (1920, 1080):
+        t = json.loads(tfile.read_text())
+        out["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
+        out["roofline"]["traffic_source"] = t["source"]
+        out["roofline"]["algorithmic_bytes_per_launch"] = (verts / ks["extend_launches"]
+                                                           * bytes_per_vertex)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(spec)

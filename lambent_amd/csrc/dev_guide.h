@@ -1,0 +1,304 @@
+// dev_guide.h - device restatement of the reference's Markov-chain (vMF)
+// path guiding and its world-space hash grid / radiance cache:
+//
+//   strip_NaN, hashes, grid_loc     general.wesl:117-188
+//   grid_load / store / radiance    general.wesl:192-311 (relaxed atomics,
+//                                   CAS key claim, try-lock: racy by design)
+//   update_marcov_state             general.wesl:464-492
+//   get_light_dir                   general.wesl:495-498
+//   get_kappa / vmf_pdf / expm1     general.wesl:1000-1047
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dev_common.h"
+#include "dev_restir.h"
+#include "dev_shade.h"
+#include "lmb_layout.h"
+
+namespace lmb {
+
+// bindings.wesl:3-9 MarcovChainState
+struct Chain {
+    V3 ls;
+    float mean_cosine;
+    float weight_sum;
+    uint32_t num_samples;
+    float score;
+};
+
+__device__ __forceinline__ Chain chain_zero() {
+    Chain c;
+    c.ls = v3s(0.0f);
+    c.mean_cosine = 0.0f;
+    c.weight_sum = 0.0f;
+    c.num_samples = 0u;
+    c.score = 0.0f;
+    return c;
+}
+__device__ __forceinline__ Chain chain_load(const MarkovChain& m) {
+    Chain c;
+    c.ls = v3(m.light_source[0], m.light_source[1], m.light_source[2]);
+    c.mean_cosine = m.mean_cosine;
+    c.weight_sum = m.weight_sum;
+    c.num_samples = m.num_samples;
+    c.score = m.score;
+    return c;
+}
+__device__ __forceinline__ MarkovChain chain_store(const Chain& c) {
+    MarkovChain m;
+    m.light_source[0] = c.ls.x;
+    m.light_source[1] = c.ls.y;
+    m.light_source[2] = c.ls.z;
+    m.mean_cosine = c.mean_cosine;
+    m.weight_sum = c.weight_sum;
+    m.num_samples = c.num_samples;
+    m.score = c.score;
+    m.pad = 0u;
+    return m;
+}
+
+constexpr uint32_t kNumResamples = 21;     // general.wesl:341-345
+constexpr uint32_t kScreenResamples = 4;
+constexpr uint32_t kScreenLastFrame = 12;
+constexpr uint32_t kWorldResamples = 4;
+constexpr uint32_t kResampleSize = 91;     // general.wesl:363
+constexpr uint32_t kChainSampleClamp = 1024;
+constexpr uint32_t kCacheTimeout = 30;
+constexpr uint32_t kMaxNumSamples = 0x0FFFFFFFu;
+constexpr float kGridSize = 0.1f;
+
+// general.wesl:120-131: +-Inf -> 10000, NaN -> 0
+__device__ __forceinline__ float strip_nan(float f) {
+    uint32_t b = __float_as_uint(f);
+    if ((b & 0x7F800000u) == 0x7F800000u) return (b & 0x007FFFFFu) == 0u ? 10000.0f : 0.0f;
+    return f;
+}
+__device__ __forceinline__ V3 strip_nan3(V3 v) {
+    return v3(strip_nan(v.x), strip_nan(v.y), strip_nan(v.z));
+}
+
+// general.wesl:148-156
+__device__ __forceinline__ uint32_t incoming_hash(V3 v) {
+    return (v.x > 0.0f ? 1u : 0u) | (v.y > 0.0f ? 2u : 0u) | (v.z > 0.0f ? 4u : 0u);
+}
+
+struct GridKey {
+    uint32_t idx;        // primary hash % table size
+    uint32_t secondary;  // secondary hash (never 0)
+};
+
+// f32 -> i32 toward zero, saturating, NaN -> 0.
+__device__ __forceinline__ int32_t f2i_sat(float f) {
+    if (!(f == f)) return 0;
+    if (f >= 2147483520.0f) return 2147483647;
+    if (f <= -2147483648.0f) return (int32_t)0x80000000;
+    return (int32_t)f;
+}
+
+// general.wesl:182-188 grid_loc.  num_shifts = u32(log2(max(len, 1))) is
+// taken exactly from the float exponent (floor(log2 x) for x >= 1).
+__device__ __forceinline__ void grid_loc(V3 pos, V3 camera_loc, int32_t loc[3]) {
+    V3 d = pos - camera_loc;
+    float len = fmaxf(fabsf(d.x), fmaxf(fabsf(d.y), fabsf(d.z)));
+    float x = fmaxf(len / 1.0f, 1.0f);
+    uint32_t shifts;
+    if (!(x < 3.0e38f)) {
+        shifts = x == x ? 127u : 0u;
+    } else {
+        shifts = (__float_as_uint(x) >> 23) - 127u;
+    }
+    uint32_t mask = 0xFFFFFFFFu << (shifts < 30u ? shifts : 30u);
+    loc[0] = f2i_sat(pos.x / kGridSize) & (int32_t)mask;
+    loc[1] = f2i_sat(pos.y / kGridSize) & (int32_t)mask;
+    loc[2] = f2i_sat(pos.z / kGridSize) & (int32_t)mask;
+}
+
+// general.wesl:161-178 primary_hasher / secondary_hasher
+__device__ __forceinline__ GridKey grid_key(V3 pos, V3 normal, V3 incoming, V3 camera_loc,
+                                            uint32_t table_size) {
+    int32_t loc[3];
+    grid_loc(pos, camera_loc, loc);
+    const uint32_t n = normalised_to_u32(normal) & 0xFF00FF00u;
+    const uint32_t inc = rand_u32(incoming_hash(incoming));
+    const uint32_t h1 = ((((uint32_t)loc[0] * 1u) ^ ((uint32_t)loc[1] * 687603047u)) ^
+                         ((uint32_t)loc[2] * 571899763u)) ^
+                        (n * 894271171u);
+    const uint32_t primary = h1 ^ (inc * 840057787u);
+    const uint32_t s3 = (((uint32_t)loc[0] * 1699644091u) ^ ((uint32_t)loc[1] * 550105817u)) ^
+                        ((uint32_t)loc[2] * 956114303u);
+    const uint32_t secondary = ((rand_u32(s3) ^ (n * 155096983u)) ^ (0u * 284005801u)) | 1u;
+    GridKey k;
+    k.idx = primary % table_size;
+    k.secondary = secondary;
+    return k;
+}
+
+__device__ __forceinline__ uint32_t aload(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void astore(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ Chain world_chain_read(const WorldMarkovChain* w) {
+    Chain c;
+    c.ls = v3(__uint_as_float(aload(&w->chain_light_source[0])),
+              __uint_as_float(aload(&w->chain_light_source[1])),
+              __uint_as_float(aload(&w->chain_light_source[2])));
+    c.mean_cosine = __uint_as_float(aload(&w->chain_mean_cosine));
+    c.weight_sum = __uint_as_float(aload(&w->chain_weight_sum));
+    c.num_samples = aload(&w->chain_num_samples);
+    c.score = __uint_as_float(aload(&w->chain_score));
+    return c;
+}
+
+// general.wesl:192-226 grid_load
+__device__ __forceinline__ Chain grid_load(WorldMarkovChain* cur, const WorldMarkovChain* old,
+                                           const GridKey& k, bool force_old) {
+    Chain s = world_chain_read(cur + k.idx);
+    if (aload(&cur[k.idx].secondary_hash) != k.secondary || force_old) {
+        if (aload(&old[k.idx].secondary_hash) == k.secondary)
+            s = world_chain_read(old + k.idx);
+        else
+            s = chain_zero();
+    } else {
+        cur[k.idx].timeout = kCacheTimeout;
+    }
+    s.ls = strip_nan3(s.ls);
+    s.mean_cosine = strip_nan(s.mean_cosine);
+    s.weight_sum = strip_nan(s.weight_sum);
+    s.score = strip_nan(s.score);
+    return s;
+}
+
+// general.wesl:228-246 grid_store_markov_chain
+__device__ __forceinline__ void grid_store(WorldMarkovChain* cur, const GridKey& k,
+                                           const Chain& s) {
+    WorldMarkovChain* w = cur + k.idx;
+    if (aload(&w->secondary_hash) != k.secondary) {
+        if (atomicCAS(&w->secondary_hash, 0u, k.secondary) != 0u) return;
+    }
+    astore(&w->chain_light_source[0], __float_as_uint(s.ls.x));
+    astore(&w->chain_light_source[1], __float_as_uint(s.ls.y));
+    astore(&w->chain_light_source[2], __float_as_uint(s.ls.z));
+    astore(&w->chain_mean_cosine, __float_as_uint(s.mean_cosine));
+    astore(&w->chain_weight_sum, __float_as_uint(s.weight_sum));
+    astore(&w->chain_num_samples, s.num_samples);
+    astore(&w->chain_score, __float_as_uint(s.score));
+    w->timeout = kCacheTimeout;
+}
+
+// general.wesl:248-271 grid_load_radiance_estimate
+__device__ __forceinline__ V3 grid_radiance(WorldMarkovChain* cur, const WorldMarkovChain* old,
+                                            const GridKey& k) {
+    uint32_t n = aload(&cur[k.idx].num_samples);
+    V3 rad = v3(__uint_as_float(aload(&cur[k.idx].luminance[0])),
+                __uint_as_float(aload(&cur[k.idx].luminance[1])),
+                __uint_as_float(aload(&cur[k.idx].luminance[2])));
+    if (aload(&cur[k.idx].secondary_hash) != k.secondary) {
+        if (aload(&old[k.idx].secondary_hash) == k.secondary) {
+            n = aload(&old[k.idx].num_samples);
+            rad = v3(__uint_as_float(aload(&old[k.idx].luminance[0])),
+                     __uint_as_float(aload(&old[k.idx].luminance[1])),
+                     __uint_as_float(aload(&old[k.idx].luminance[2])));
+        } else {
+            return v3s(0.0f);
+        }
+    } else {
+        cur[k.idx].timeout = kCacheTimeout;
+    }
+    if (n == 0u) return v3s(0.0f);
+    V3 r = rad / (float)n;
+    return v3(fminf(fmaxf(r.x, 0.0f), 1000.0f), fminf(fmaxf(r.y, 0.0f), 1000.0f),
+              fminf(fmaxf(r.z, 0.0f), 1000.0f));
+}
+
+// general.wesl:273-311 grid_add_radiance (try-lock: gives up when held)
+__device__ __forceinline__ void grid_add_radiance(WorldMarkovChain* cur, const GridKey& k,
+                                                  V3 radiance) {
+    WorldMarkovChain* w = cur + k.idx;
+    V3 c = v3(fminf(fmaxf(radiance.x, 0.0f), 100.0f), fminf(fmaxf(radiance.y, 0.0f), 100.0f),
+              fminf(fmaxf(radiance.z, 0.0f), 100.0f));
+    if (aload(&w->secondary_hash) != k.secondary) {
+        if (atomicCAS(&w->secondary_hash, 0u, k.secondary) != 0u) return;
+    }
+    if (atomicExch(&w->lock, 1u) == 1u) return;
+    uint32_t n = aload(&w->num_samples);
+    if (n >= kMaxNumSamples) {
+        astore(&w->lock, 0u);
+        w->timeout = kCacheTimeout;
+        return;
+    }
+    atomicAdd(&w->num_samples, 1u);
+    float r = strip_nan(c.x + __uint_as_float(aload(&w->luminance[0])));
+    astore(&w->luminance[0], __float_as_uint(fmaxf(r, 0.0f)));
+    float g = strip_nan(c.y + __uint_as_float(aload(&w->luminance[1])));
+    astore(&w->luminance[1], __float_as_uint(fmaxf(g, 0.0f)));
+    float b = strip_nan(c.z + __uint_as_float(aload(&w->luminance[2])));
+    astore(&w->luminance[2], __float_as_uint(fmaxf(b, 0.0f)));
+    astore(&w->lock, 0u);
+    w->timeout = kCacheTimeout;
+}
+
+// general.wesl:495-498
+__device__ __forceinline__ V3 get_light_dir(const Chain& s, V3 current) {
+    return safe_div_v3(s.ls, s.weight_sum) - current;
+}
+
+__device__ __forceinline__ V3 mix3(V3 a, V3 b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ float mix1(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// general.wesl:464-492 update_marcov_state
+__device__ __forceinline__ Chain update_markov_state(V3 x1, V3 x2, float p_hat, uint32_t seed,
+                                                     bool not_vmf, Chain state, float confidence,
+                                                     float sum_mc) {
+    if (not_vmf) state = chain_zero();
+    const float score = p_hat;
+    const float rnd = rand_f32(seed);
+    if (rnd * sum_mc < p_hat * (float)kNumResamples) {
+        if (near_zero(length3(get_light_dir(state, x1)))) return state;
+        state.num_samples = min(state.num_samples + 1u, kChainSampleClamp);
+        const float alpha = state.num_samples == 1u
+                                ? 1.0f
+                                : fmaxf(1.0f / (float)state.num_samples, 0.01f) * confidence;
+        state.weight_sum = mix1(state.weight_sum, score, alpha);
+        state.ls = mix3(state.ls, score * x2, alpha);
+        const V3 mean = normalize3(get_light_dir(state, x1));
+        state.mean_cosine = mix1(state.mean_cosine, score * dot3(normalize3(x2 - x1), mean), alpha);
+    }
+    return state;
+}
+
+// general.wesl:1000-1016
+__device__ __forceinline__ float get_kappa(const Chain& s) {
+    const float np_ = 0.2f, rp = 0.99f;
+    const float n2 = (float)(s.num_samples * s.num_samples);
+    const float r = ((n2 * (s.mean_cosine / s.weight_sum)) + (np_ * rp)) / (n2 + np_);
+    const float k = ((3.0f * r) - ((r * r) * r)) / (1.0f - (r * r));
+    return fminf(fmaxf(k, 0.001f), 8.50705867e37f);
+}
+// general.wesl:1025-1047
+__device__ __forceinline__ float exp_minus_one(float x) {
+    float u = expf(x);
+    if (u == 1.0f) return x;
+    float y = u - 1.0f;
+    if (fabsf(x) < 1.0f) return (x * y) / logf(u);
+    return y;
+}
+__device__ __forceinline__ float x_div_exp_minus_one(float x) {
+    float u = expf(x);
+    if (u == 1.0f) return 1.0f;
+    float y = u - 1.0f;
+    if (fabsf(x) < 1.0f) return logf(u) / y;
+    return x / y;
+}
+// general.wesl:1018-1023
+__device__ __forceinline__ float vmf_pdf(V3 mean, float kappa, V3 dir) {
+    const float fraction = x_div_exp_minus_one(-2.0f * kappa) / (4.0f * kPi);
+    const float c = dot3(dir, mean);
+    return fmaxf(fraction * expf(kappa * (c - 1.0f)), 0.0f);
+}
+
+}  // namespace lmb
