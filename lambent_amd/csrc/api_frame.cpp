@@ -10,6 +10,7 @@
 #include "../../include/lambent.h"
 #include "lmb_frame_dev.h"
 #include "lmb_internal.h"
+#include "lmb_guided.h"
 #include "lmb_restir.h"
 
 using namespace lmb;
@@ -93,7 +94,68 @@ struct lmb_data_buffers_s {
     SpillBuffer spill;
     uint64_t gi_count = 0, mc_count = 0, wmc_count = 0;
     FrameDev dev{};
+    DeviceBuffer guided_pool;  // guided-mode state, allocated on first use
+    GuidedDev guided{};
 };
+
+namespace {
+
+// Carves the guided-mode arrays out of one allocation (256-byte aligned).
+lmb_status ensure_guided(lmb_data_buffers b) {
+    const uint64_t n = (uint64_t)b->width * b->height;
+    struct Part {
+        void** dst;
+        uint64_t bytes;
+    };
+    GuidedDev& g = b->guided;
+    Part parts[] = {
+        {(void**)&g.tent, n * sizeof(MarkovChain)},
+        {(void**)&g.tent_snap, n * sizeof(MarkovChain)},
+        {(void**)&g.sum_mc, n * 4},
+        {(void**)&g.cand_seed, n * 4},
+        {(void**)&g.pix_pdf, n * 4},
+        {(void**)&g.pix_res, n * sizeof(Reservoir)},
+        {(void**)&g.pix_sample, n * sizeof(Reservoir)},
+        {(void**)&g.out_rad_sum, n * 16},
+        {(void**)&g.pix_last_normal, n * 16},
+        {(void**)&g.pix_last_albedo, n * 16},
+        {(void**)&g.pix_last_emission, n * 16},
+        {(void**)&g.color, n * 16},
+        {(void**)&g.scol, n * 16},
+        {(void**)&g.orc, n * 16},
+        {(void**)&g.sout, n * 16},
+        {(void**)&g.aem, n * 16},
+        {(void**)&g.aalb, n * 16},
+        {(void**)&g.lastn, n * 16},
+        {(void**)&g.lastd, n * 16},
+        {(void**)&g.s_point, n * 16},
+        {(void**)&g.s_normal, n * 16},
+        {(void**)&g.s_albedo, n * 16},
+        {(void**)&g.s_emission, n * 16},
+        {(void**)&g.s_vis_point, n * 16},
+        {(void**)&g.s_vis_normal, n * 16},
+        {(void**)&g.s_start_dir, n * 16},
+        {(void**)&g.gr0, n * 16},
+        {(void**)&g.gr1, n * 16},
+        {(void**)&g.gr2, n * 16},
+        {(void**)&g.gr3, n * 16},
+    };
+    uint64_t total = 0;
+    for (const Part& p : parts) total += (p.bytes + 255) & ~255ull;
+    if (b->guided_pool.ptr && b->guided_pool.bytes == total) return LMB_OK;
+    lmb_status st = b->guided_pool.alloc(total);
+    if (st) return st;
+    hipError_t e = hipMemset(b->guided_pool.ptr, 0, total);
+    if (e != hipSuccess) return fail_hip(e, "hipMemset");
+    char* base = (char*)b->guided_pool.ptr;
+    for (const Part& p : parts) {
+        *p.dst = base;
+        base += (p.bytes + 255) & ~255ull;
+    }
+    return LMB_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -363,9 +425,6 @@ lmb_status lmb_trace(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
                      lmb_data_buffers bufs, uint32_t seed_offset, void* stream) {
     if (!tracer || !scene || !cam || !bufs) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
     const lmb_tracer_opts& o = tracer->opts;
-    if (o.guiding != LMB_GUIDING_OFF)
-        return fail(LMB_ERR_INVALID_ARGUMENT,
-                    "Markov-chain guiding is not available in this build; use LMB_GUIDING_OFF");
     TraceParams p;
     p.quality = o.quality;
     p.samples = o.samples;
@@ -378,8 +437,15 @@ lmb_status lmb_trace(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
     p.seed_offset = seed_offset;
     CameraDev cd;
     std::memcpy(&cd, cam, sizeof cd);
-    hipError_t e =
-        launch_trace(scene->dev, cd, bufs->dev, p, (hipStream_t)stream, bufs->timer.get());
+    hipError_t e;
+    if (o.guiding == LMB_GUIDING_MARKOV_CHAIN) {
+        lmb_status st = ensure_guided(bufs);
+        if (st) return st;
+        e = launch_guided(scene->dev, cd, bufs->dev, bufs->guided, p, (hipStream_t)stream,
+                          bufs->timer.get());
+    } else {
+        e = launch_trace(scene->dev, cd, bufs->dev, p, (hipStream_t)stream, bufs->timer.get());
+    }
     if (e != hipSuccess) return fail_hip(e, "trace launch");
     return LMB_OK;
 }

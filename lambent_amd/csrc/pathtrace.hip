@@ -24,6 +24,7 @@
 #include "dev_shade.h"
 #include "dev_traverse.h"
 #include "lmb_frame_dev.h"
+#include "lmb_guided.h"
 
 namespace lmb {
 
@@ -247,6 +248,13 @@ __global__ __launch_bounds__(kPtBlock) void k_resolve(FrameDev f, TraceParams p)
         f.out_color[pix] = make_float4(fmaxf(s4.x / ns, 0.f), fmaxf(s4.y / ns, 0.f),
                                        fmaxf(s4.z / ns, 0.f), 1.f);
     }
+}
+
+hipError_t launch_extend(const SceneDev& s, const FrameDev& f, const TraceParams& p,
+                         uint32_t bounce, hipStream_t stream) {
+    const dim3 grid((uint32_t)(frame_threads(f.npix) / kPtBlock)), block(kPtBlock);
+    hipLaunchKernelGGL(k_extend, grid, block, 0, stream, s, f, p, bounce);
+    return hipGetLastError();
 }
 
 template <uint32_t Q>
