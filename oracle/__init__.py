@@ -318,3 +318,41 @@ def eof_world(w: np.ndarray) -> np.ndarray:
     out = np.ascontiguousarray(w.copy())
     _lib.orc_eof_world(out.view(np.uint32).ctypes.data_as(_u32p), len(out))
     return out
+
+
+_sig("orc_trace_guided", [_vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                          ctypes.c_uint32, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
+                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                          _vp, _vp, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint64, _vp,
+                          ctypes.c_uint64, _f32p, _f32p, _f32p, _f32p], ctypes.c_uint64)
+
+
+class GuidedState:
+    """Persistent state of the guided mode for the oracle (one frame's
+    inputs/outputs, WGSL layouts)."""
+
+    def __init__(self, width, height, world_size, screen=True, spatial=True):
+        n = width * height
+        self.world = np.zeros(max(world_size, 1), WORLD_DTYPE)
+        self.world_old = np.zeros(max(world_size, 1), WORLD_DTYPE)
+        self.mc_old = np.zeros((n if screen else 1, 8), np.uint32)
+        self.mc = np.zeros_like(self.mc_old)
+        self.gi = np.zeros(n if spatial else 1, RESERVOIR_DTYPE)
+        self.info = np.zeros((n, 9), np.float32)
+
+
+def trace_guided(scene: "Scene", camera_bytes: bytes, width: int, height: int, state: GuidedState,
+                 quality=1, samples=4, tmin=1e-4, tmax=1000.0, bounces=5, rr_start=None,
+                 seed_offset=0, restir=2, subgroup=64):
+    cam = camera_floats(camera_bytes)
+    c = np.empty((height, width, 4), np.float32)
+    nrm = np.empty_like(c)
+    alb = np.empty_like(c)
+    v = _lib.orc_trace_guided(scene._h, _f(cam), width, height, quality, samples, tmin, tmax,
+                              bounces, 0xFFFFFFFF if rr_start is None else rr_start, seed_offset,
+                              restir, subgroup, state.world.ctypes.data_as(_vp),
+                              state.world_old.ctypes.data_as(_vp), len(state.world),
+                              _u(state.mc_old), _u(state.mc), len(state.mc_old),
+                              state.gi.ctypes.data_as(_vp), len(state.gi), _f(state.info), _f(c),
+                              _f(nrm), _f(alb))
+    return c, nrm, alb, v

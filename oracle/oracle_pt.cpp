@@ -425,9 +425,659 @@ SampleOut rt_sample_plain(const Scene& s, const PlainOpts& o, F3 org, F3 dir, ui
     return out;
 }
 
+// ===================================================================== guided
+// The reference's full rt_main / rt_sample: Markov-chain (vMF) guiding over
+// 21 candidates, the world hash grid + radiance cache, the ReSTIR reservoir.
+// Interleaving: sample-major; within a sample pixels run one after another in
+// row order, each updating the world grid immediately; subgroup shuffles of
+// the tentative chains act on the values at sample start (lockstep lanes).
+
+struct Chain {
+    F3 ls;
+    float mean_cosine, weight_sum;
+    uint32_t num_samples;
+    float score;
+};
+Chain czero() { return Chain{splat(0.0f), 0.0f, 0.0f, 0u, 0.0f}; }
+Chain chain_from_words(const uint32_t* w) {
+    Chain c;
+    float f[7];
+    std::memcpy(f, w, 28);
+    c.ls = mk(f[0], f[1], f[2]);
+    c.mean_cosine = f[3];
+    c.weight_sum = f[4];
+    std::memcpy(&c.num_samples, w + 5, 4);
+    std::memcpy(&c.score, w + 6, 4);
+    return c;
+}
+void chain_to_words(const Chain& c, uint32_t* w) {
+    float f[5] = {c.ls.x, c.ls.y, c.ls.z, c.mean_cosine, c.weight_sum};
+    std::memcpy(w, f, 20);
+    std::memcpy(w + 5, &c.num_samples, 4);
+    std::memcpy(w + 6, &c.score, 4);
+    w[7] = 0u;
+}
+
+struct WorldC {  // bindings.wesl:33-40, 56 bytes
+    uint32_t secondary_hash, num_samples;
+    float lum[3];
+    uint32_t lock;
+    float cls[3];
+    float cmc, cws;
+    uint32_t cn;
+    float cscore;
+    uint32_t timeout;
+};
+static_assert(sizeof(WorldC) == 56, "world chain layout");
+
+float strip_nan(float f) {
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    if ((b & 0x7F800000u) == 0x7F800000u) return (b & 0x007FFFFFu) == 0u ? 10000.0f : 0.0f;
+    return f;
+}
+F3 strip_nan3(F3 v) { return mk(strip_nan(v.x), strip_nan(v.y), strip_nan(v.z)); }
+
+}  // namespace
+extern "C" uint32_t orc_normalised_to_u32(const float* n);
+namespace {
+
+uint32_t n2u(F3 n) {
+    float a[3] = {n.x, n.y, n.z};
+    return orc_normalised_to_u32(a);
+}
+
+int32_t f2i_sat(float f) {
+    if (!(f == f)) return 0;
+    if (f >= 2147483520.0f) return 2147483647;
+    if (f <= -2147483648.0f) return (int32_t)0x80000000;
+    return (int32_t)f;
+}
+
+struct GKey {
+    uint32_t idx, secondary;
+};
+
+// general.wesl:148-188: level-of-detail cell, normal and incoming octant
+// hashed with the prime multiplies; floor(log2(x)) for x >= 1 from frexp.
+GKey grid_key(F3 pos, F3 normal, F3 incoming, F3 cam, uint32_t table) {
+    F3 d = sub(pos, cam);
+    float len = std::fmax(std::fabs(d.x), std::fmax(std::fabs(d.y), std::fabs(d.z)));
+    float x = std::fmax(len / 1.0f, 1.0f);
+    uint32_t shifts;
+    if (std::isnan(x)) {
+        shifts = 0;
+    } else if (std::isinf(x)) {
+        shifts = 127;
+    } else {
+        int e;
+        std::frexp(x, &e);
+        shifts = (uint32_t)(e - 1);
+    }
+    uint32_t mask = 0xFFFFFFFFu << (shifts < 30u ? shifts : 30u);
+    uint32_t lx = (uint32_t)(f2i_sat(pos.x / 0.1f)) & mask;
+    uint32_t ly = (uint32_t)(f2i_sat(pos.y / 0.1f)) & mask;
+    uint32_t lz = (uint32_t)(f2i_sat(pos.z / 0.1f)) & mask;
+    uint32_t n = n2u(normal) & 0xFF00FF00u;
+    uint32_t inc_hash = (incoming.x > 0.0f ? 1u : 0u) | (incoming.y > 0.0f ? 2u : 0u) |
+                        (incoming.z > 0.0f ? 4u : 0u);
+    uint32_t inc = orc_rand_u32(inc_hash);
+    uint32_t primary = ((((lx * 1u) ^ (ly * 687603047u)) ^ (lz * 571899763u)) ^ (n * 894271171u)) ^
+                       (inc * 840057787u);
+    uint32_t s3 = ((lx * 1699644091u) ^ (ly * 550105817u)) ^ (lz * 956114303u);
+    uint32_t secondary = ((orc_rand_u32(s3) ^ (n * 155096983u)) ^ 0u) | 1u;
+    return GKey{primary % table, secondary};
+}
+
+struct Grid {
+    WorldC* cur;
+    const WorldC* old;
+    uint32_t table;
+    F3 cam;
+};
+
+Chain world_chain(const WorldC& w) {
+    return Chain{mk(w.cls[0], w.cls[1], w.cls[2]), w.cmc, w.cws, w.cn, w.cscore};
+}
+
+// general.wesl:192-226
+Chain grid_load(Grid& g, const GKey& k, bool force_old) {
+    Chain s = world_chain(g.cur[k.idx]);
+    if (g.cur[k.idx].secondary_hash != k.secondary || force_old) {
+        if (g.old[k.idx].secondary_hash == k.secondary)
+            s = world_chain(g.old[k.idx]);
+        else
+            s = czero();
+    } else {
+        g.cur[k.idx].timeout = 30u;
+    }
+    s.ls = strip_nan3(s.ls);
+    s.mean_cosine = strip_nan(s.mean_cosine);
+    s.weight_sum = strip_nan(s.weight_sum);
+    s.score = strip_nan(s.score);
+    return s;
+}
+
+// general.wesl:228-246
+void grid_store(Grid& g, const GKey& k, const Chain& s) {
+    WorldC& w = g.cur[k.idx];
+    if (w.secondary_hash != k.secondary) {
+        if (w.secondary_hash != 0u) return;
+        w.secondary_hash = k.secondary;
+    }
+    w.cls[0] = s.ls.x;
+    w.cls[1] = s.ls.y;
+    w.cls[2] = s.ls.z;
+    w.cmc = s.mean_cosine;
+    w.cws = s.weight_sum;
+    w.cn = s.num_samples;
+    w.cscore = s.score;
+    w.timeout = 30u;
+}
+
+// general.wesl:248-271
+F3 grid_radiance(Grid& g, const GKey& k) {
+    uint32_t n = g.cur[k.idx].num_samples;
+    F3 rad = mk(g.cur[k.idx].lum[0], g.cur[k.idx].lum[1], g.cur[k.idx].lum[2]);
+    if (g.cur[k.idx].secondary_hash != k.secondary) {
+        if (g.old[k.idx].secondary_hash == k.secondary) {
+            n = g.old[k.idx].num_samples;
+            rad = mk(g.old[k.idx].lum[0], g.old[k.idx].lum[1], g.old[k.idx].lum[2]);
+        } else {
+            return splat(0.0f);
+        }
+    } else {
+        g.cur[k.idx].timeout = 30u;
+    }
+    if (n == 0u) return splat(0.0f);
+    F3 r = divs(rad, (float)n);
+    return mk(std::fmin(std::fmax(r.x, 0.0f), 1000.0f), std::fmin(std::fmax(r.y, 0.0f), 1000.0f),
+              std::fmin(std::fmax(r.z, 0.0f), 1000.0f));
+}
+
+// general.wesl:273-311 (no lock contention in a sequential run)
+void grid_add(Grid& g, const GKey& k, F3 radiance) {
+    WorldC& w = g.cur[k.idx];
+    F3 c = mk(std::fmin(std::fmax(radiance.x, 0.0f), 100.0f), std::fmin(std::fmax(radiance.y, 0.0f), 100.0f),
+              std::fmin(std::fmax(radiance.z, 0.0f), 100.0f));
+    if (w.secondary_hash != k.secondary) {
+        if (w.secondary_hash != 0u) return;
+        w.secondary_hash = k.secondary;
+    }
+    if (w.lock == 1u) return;
+    if (w.num_samples >= 0x0FFFFFFFu) {
+        w.timeout = 30u;
+        return;
+    }
+    w.num_samples += 1u;
+    w.lum[0] = std::fmax(strip_nan(c.x + w.lum[0]), 0.0f);
+    w.lum[1] = std::fmax(strip_nan(c.y + w.lum[1]), 0.0f);
+    w.lum[2] = std::fmax(strip_nan(c.z + w.lum[2]), 0.0f);
+    w.timeout = 30u;
+}
+
+F3 safe_div3(F3 v, float d) { return near_zero(d) ? splat(0.0f) : divs(v, d); }
+F3 light_dir(const Chain& s, F3 cur) { return sub(safe_div3(s.ls, s.weight_sum), cur); }
+F3 mix3(F3 a, F3 b, float t) { return add(scale(a, 1.0f - t), scale(b, t)); }
+float mix1(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// general.wesl:464-492
+Chain update_state(F3 x1, F3 x2, float p_hat, uint32_t seed, bool not_vmf, Chain state, float conf,
+                   float sum_mc) {
+    if (not_vmf) state = czero();
+    float rnd = orc_rand_f32(seed);
+    if (rnd * sum_mc < p_hat * 21.0f) {
+        if (near_zero(length(light_dir(state, x1)))) return state;
+        state.num_samples = std::min(state.num_samples + 1u, 1024u);
+        float alpha = state.num_samples == 1u ? 1.0f
+                                              : std::fmax(1.0f / (float)state.num_samples, 0.01f) * conf;
+        state.weight_sum = mix1(state.weight_sum, p_hat, alpha);
+        state.ls = mix3(state.ls, scale(x2, p_hat), alpha);
+        F3 mean = normalize(light_dir(state, x1));
+        state.mean_cosine = mix1(state.mean_cosine, p_hat * dot(normalize(sub(x2, x1)), mean), alpha);
+    }
+    return state;
+}
+
+// general.wesl:1000-1047
+float get_kappa(const Chain& s) {
+    float n2 = (float)(s.num_samples * s.num_samples);
+    float r = ((n2 * (s.mean_cosine / s.weight_sum)) + (0.2f * 0.99f)) / (n2 + 0.2f);
+    float k = ((3.0f * r) - ((r * r) * r)) / (1.0f - (r * r));
+    return std::fmin(std::fmax(k, 0.001f), 8.50705867e37f);
+}
+float expm1_ref(float x) {
+    float u = std::exp(x);
+    if (u == 1.0f) return x;
+    float y = u - 1.0f;
+    if (std::fabs(x) < 1.0f) return (x * y) / std::log(u);
+    return y;
+}
+float x_div_expm1(float x) {
+    float u = std::exp(x);
+    if (u == 1.0f) return 1.0f;
+    float y = u - 1.0f;
+    if (std::fabs(x) < 1.0f) return std::log(u) / y;
+    return x / y;
+}
+float vmf_pdf(F3 mean, float kappa, F3 d) {
+    float fraction = x_div_expm1(-2.0f * kappa) / (4.0f * PI);
+    return std::fmax(fraction * std::exp(kappa * (dot(d, mean) - 1.0f)), 0.0f);
+}
+F3 from_onb_no_tangent(F3 src, F3 up) {
+    F3 other = near_zero(up.x) ? mk(1.0f, 0.0f, 0.0f) : mk(0.0f, 1.0f, 0.0f);
+    return from_onb(src, up, normalize(cross(up, other)));
+}
+
+struct GuidedOpts {
+    uint32_t quality, bounces, rr_start, samples, restir, subgroup;
+    float tmin, tmax;
+};
+
+struct SampleRet {
+    F3 color, normal, albedo, point, emission, visible_point, visible_normal, out_radiance;
+    uint32_t ty;
+    float pdf;
+    uint32_t vertices;
+};
+
+// general.wesl:643-975 rt_sample (cands[0..15] = screen-space candidates)
+SampleRet rt_sample_guided(const Scene& s, Grid& grid, const GuidedOpts& o, F3 org, F3 dir,
+                           uint32_t own_seed, const Chain* cands, Chain& tent, float& sum_mc) {
+    SampleRet out;
+    std::memset(&out, 0, sizeof out);
+    out.ty = 0xFFu;
+    out.pdf = 1.0f;
+    uint32_t self_seed = own_seed;
+    F3 color = splat(1.0f), orc = splat(1.0f);
+    float picked = -1.0f;
+    F3 last_normal = splat(0.0f), last_dir = splat(0.0f);
+    float cached_pdf = 0.0f, cached_brdf = 0.0f;
+    bool last_not_vmf = false;
+    F3 accum_emission = splat(0.0f), accum_albedo = splat(1.0f);
+    const F3 start_dir = dir;
+    for (uint32_t i = 0; i < o.bounces; ++i) {
+        out.vertices++;
+        Hit it = ray_hit(s, org, dir, o.tmin, o.tmax);
+        if (it.hit) {
+            F3 new_pos = fma3(dir, splat(it.t), org);
+            if (i != 0) {
+                GKey kn = grid_key(new_pos, it.normal, dir, grid.cam, grid.table);
+                F3 reflected = strip_nan3(divs(scale(mul(it.color, grid_radiance(grid, kn)), cached_brdf), cached_pdf));
+                F3 rout = divs(scale(it.emission, cached_brdf), cached_pdf);
+                GKey kp = grid_key(org, last_normal, last_dir, grid.cam, grid.table);
+                grid_add(grid, kp, add(rout, reflected));
+                float weight = length(add(rout, reflected));
+                if (weight > 0.0f) {
+                    Chain st = update_state(org, new_pos, weight, self_seed, false,
+                                            grid_load(grid, kp, false), 1.0f, sum_mc);
+                    grid_store(grid, kp, st);
+                    tent = update_state(org, new_pos, length(rout), self_seed, last_not_vmf, tent,
+                                        i == 1 ? 1.0f : 0.5f, sum_mc);
+                    self_seed++;
+                }
+            }
+            last_normal = it.normal;
+            last_dir = dir;
+            last_not_vmf = false;
+            org = new_pos;
+            accum_emission = fma3(it.emission, accum_albedo, accum_emission);
+            accum_albedo = mul(accum_albedo, it.color);
+            if (i == 0) {
+                out.point = org;
+                out.emission = accum_emission;
+                out.ty = it.ty;
+                out.normal = it.normal;
+                out.albedo = accum_albedo;
+            } else if (i == 1) {
+                out.visible_point = org;
+                out.visible_normal = it.normal;
+            }
+            out.color = fma3(color, it.emission, out.color);
+            if (i != 0) out.out_radiance = fma3(orc, it.emission, out.out_radiance);
+            F3 assign_normal = get_normal(o.quality, dir, it.normal, it.tangent, self_seed, it.rx, it.ry);
+            F3 transparent_normal = get_normal(o.quality, dir, it.normal, it.tangent, self_seed, it.rtx, it.rty);
+            F3 normal = transparent_normal;
+            bool should_reflect = false;
+            self_seed += 1u;
+            F3 ccol;
+            float ri = pick_ref_idx(o.quality, &picked, it.n_lo, it.n_hi, self_seed, &ccol);
+            self_seed += 1u;
+            if (!(!it.front_face && it.ty == 2u)) ri = 1.0f / ri;
+            color = mul(color, ccol);
+            orc = mul(orc, ccol);
+            if (ri != 1.0f) {
+                float cos_theta = std::fmin(dot(neg(dir), normal), 1.0f);
+                float sin_theta = std::sqrt(1.0f - cos_theta * cos_theta);
+                bool cannot = ri * sin_theta > 1.0f;
+                float pdf = reflectance(cos_theta, ri);
+                if (cannot || pdf > orc_rand_f32(self_seed)) should_reflect = true;
+                self_seed += 1u;
+            }
+            if (!should_reflect || it.ty == 2u) {
+                color = mul(color, it.color);
+                if (i != 0) orc = mul(orc, it.color);
+            }
+            if (!should_reflect) {
+                normal = assign_normal;
+                self_seed += 1u;
+            }
+            // fill_mc_world_space_states (general.wesl:389-400)
+            Chain all[21];
+            for (int k = 0; k < 16; ++k) all[k] = cands[k];
+            for (int k = 16; k < 20; ++k) {
+                bool rand_candidate = orc_rand_f32(self_seed) < 0.7f;
+                float jitter_mul = rand_candidate ? 0.75f : 1.0f;
+                self_seed = orc_rand_u32(self_seed);
+                float jx = (float)self_seed * (1.0f / 4294967296.0f);
+                self_seed = orc_rand_u32(self_seed);
+                float jz = (float)self_seed * (1.0f / 4294967296.0f);
+                F3 jitter = scale(scale(mk(jx - std::floor(jx), 0.0f, jz - std::floor(jz)), 0.1f * 1.5f), jitter_mul);
+                GKey k2 = grid_key(add(org, from_onb_no_tangent(jitter, it.normal)), it.normal, dir,
+                                   grid.cam, grid.table);
+                all[k] = grid_load(grid, k2, rand_candidate);
+                self_seed++;
+            }
+            all[20] = tent;
+            // resample_mcmm (general.wesl:404-425)
+            Chain sel = czero();
+            sum_mc = 0.0f;
+            for (int k = 0; k < 21; ++k) {
+                self_seed = orc_rand_u32(self_seed);
+                sum_mc += all[k].weight_sum;
+                if (orc_rand_f32(self_seed) * sum_mc < all[k].weight_sum) sel = all[k];
+                self_seed = orc_rand_u32(self_seed);
+            }
+            bool reflecting = should_reflect || it.ty == 1u;
+            if (!reflecting) {
+                if (it.ty == 0u) {
+                    F3 ld = light_dir(sel, org);
+                    F3 mean = normalize(ld);
+                    float kappa = get_kappa(sel);
+                    F3 dir_cos = cosine_weighted_hemisphere(self_seed, it.normal, it.tangent);
+                    float cos_pdf = std::fmax(dot(dir_cos, it.normal) / PI, 0.0f);
+                    float theta = orc_rand_f32(self_seed) * TAU;
+                    float vx = std::cos(theta), vy = std::sin(theta);
+                    self_seed += 1u;
+                    float xi = orc_rand_f32(self_seed);
+                    self_seed += 1u;
+                    float wm1 = std::log(1.0f + xi * expm1_ref(-2.0f * kappa)) / kappa;
+                    float W = 1.0f + wm1;
+                    float norm_v = std::sqrt(-std::fma(wm1, wm1, 2.0f * wm1));
+                    F3 Xs = from_onb_no_tangent(mk(strip_nan(vx * norm_v), W, strip_nan(vy * norm_v)), mean);
+                    F3 dir_markov = normalize(Xs);
+                    float c0 = 0.2f, c1 = 0.8f;
+                    if (!(sel.weight_sum > 0.0f) || near_zero(length(ld)) || near_zero(sel.weight_sum) ||
+                        near_zero(length(Xs))) {
+                        c0 = 1.0f;
+                        c1 = 0.0f;
+                    }
+                    float csum = c0 + c1;
+                    float conf0 = c0 / csum, conf1 = c1 / csum;
+                    float rnd = orc_rand_f32(self_seed);
+                    self_seed += 1u;
+                    F3 d;
+                    if (rnd < conf0) {
+                        d = dir_cos;
+                        last_not_vmf = true;
+                    } else {
+                        d = dir_markov;
+                    }
+                    float mix_pdf = 0.0f;
+                    if (conf1 != 0.0f && sum_mc > 0.0f) {
+                        for (int k = 0; k < 21; ++k) {
+                            if (!(all[k].weight_sum > 0.0f)) continue;
+                            mix_pdf += vmf_pdf(normalize(light_dir(all[k], org)), get_kappa(all[k]), d) *
+                                       (all[k].weight_sum / sum_mc);
+                        }
+                    }
+                    float avg_pdf = conf0 * cos_pdf + (conf1 == 0.0f ? 0.0f : conf1 * std::fmax(mix_pdf, 0.0f));
+                    F3 diffuse = make_diffuse(o.quality, it.color, dir, it.normal, it.tangent, it.rx, d);
+                    cached_pdf = avg_pdf;
+                    cached_brdf = std::fmax(dot(d, it.normal) / PI, 0.0f);
+                    float nf = std::fabs(cached_brdf) / std::fmax(avg_pdf, 0.001f);
+                    color = mul(scale(color, nf), diffuse);
+                    orc = mul(scale(orc, nf), diffuse);
+                    dir = d;
+                    if (i == 0) out.pdf = avg_pdf;
+                } else if (it.ty == 2u) {
+                    cached_pdf = 1.0f;
+                    cached_brdf = 1.0f;
+                    dir = refract(dir, normal, ri);
+                }
+            } else {
+                cached_pdf = 1.0f;
+                cached_brdf = 1.0f;
+                dir = reflect(dir, normal);
+                if (dot(dir, it.normal) < 0.0f) dir = reflect(dir, it.normal);
+            }
+            if (i + 1 < o.bounces && i >= o.rr_start) {
+                float q = std::fmin(std::fmax(std::fmax(color.x, color.y), color.z), 1.0f);
+                float r = orc_rand_f32(self_seed);
+                self_seed += 1u;
+                if (r >= q) break;
+                color = divs(color, q);
+                orc = divs(orc, q);
+            }
+        } else {
+            F3 bg = background(s, dir);
+            F3 new_pos = fma3(dir, splat(o.tmax), org);
+            if (i == 1) {
+                out.visible_point = new_pos;
+                out.visible_normal = neg(dir);
+            }
+            if (i != 0) {
+                out.out_radiance = fma3(orc, bg, out.out_radiance);
+                F3 rout = divs(scale(bg, cached_brdf), cached_pdf);
+                GKey kp = grid_key(org, last_normal, last_dir, grid.cam, grid.table);
+                grid_add(grid, kp, rout);
+                float weight = length(rout);
+                if (weight > 0.0f) {
+                    Chain st = update_state(org, new_pos, weight, self_seed, false,
+                                            grid_load(grid, kp, false), 1.0f, sum_mc);
+                    grid_store(grid, kp, st);
+                    tent = update_state(org, new_pos, length(rout), self_seed, false, tent,
+                                        i == 1 ? 1.0f : 0.5f, sum_mc);
+                    self_seed++;
+                }
+            }
+            out.color = fma3(color, bg, out.color);
+            break;
+        }
+    }
+    // general.wesl:944-948
+    GKey k = grid_key(out.point, out.normal, start_dir, grid.cam, grid.table);
+    Chain wc = grid_load(grid, k, false);
+    Chain st = update_state(out.point, out.visible_point, length(out.out_radiance), self_seed, false,
+                            wc, 1.0f, sum_mc);
+    grid_store(grid, k, st);
+    grid_add(grid, k, out.out_radiance);
+    return out;
+}
+
+struct Res {  // bindings.wesl:158-169
+    float vp[3];
+    uint32_t vn;
+    float sp[3];
+    uint32_t sn;
+    float L[3];
+    uint32_t ty;
+    float rough, w;
+    uint32_t mv;
+    float W;
+};
+static_assert(sizeof(Res) == 64, "reservoir layout");
+
+void res_update(Res& R, const Res& smp, float w, uint32_t seed) {
+    R.w = R.w + w;
+    R.mv = ((R.mv + 1u) & 0x7FFFFFFFu) | (R.mv & 0x80000000u);
+    if (orc_rand_f32(seed) * R.w < w) {
+        std::memcpy(R.vp, smp.vp, 12);
+        R.vn = smp.vn;
+        std::memcpy(R.sp, smp.sp, 12);
+        R.sn = smp.sn;
+        std::memcpy(R.L, smp.L, 12);
+        R.ty = smp.ty;
+        R.rough = smp.rough;
+        R.mv |= 0x80000000u;
+    }
+}
+float res_p_hat(const Res& r) {
+    return std::fmax(std::sqrt((r.L[0] * r.L[0] + r.L[1] * r.L[1]) + r.L[2] * r.L[2]), 0.00001f);
+}
+float safe_div1(float n, float d) { return near_zero(d) ? 0.0f : n / d; }
+
 }  // namespace
 
 extern "C" {
+
+// general.wesl:506-610 over a whole image.  world/world_old: 56-byte chains
+// (world is updated in place); mc_old / mc_out: 32-byte chains (mc_count
+// entries each); gi_out: 64-byte reservoirs (gi_count entries); info_out:
+// 9 floats per pixel.  Returns the number of path vertices.
+uint64_t orc_trace_guided(void* h, const float* cam, uint32_t width, uint32_t height,
+                          uint32_t quality, uint32_t samples, float tmin, float tmax,
+                          uint32_t bounces, uint32_t rr_start, uint32_t seed_offset, uint32_t restir,
+                          uint32_t subgroup, void* world, const void* world_old, uint32_t world_size,
+                          const uint32_t* mc_old, uint32_t* mc_out, uint64_t mc_count, void* gi_out,
+                          uint64_t gi_count, float* info_out, float* out_color, float* out_normal,
+                          float* out_albedo) {
+    const Scene& s = *(const Scene*)h;
+    const uint64_t n = (uint64_t)width * height;
+    GuidedOpts o{quality, bounces, rr_start, samples, restir, subgroup, tmin, tmax};
+    Grid grid{(WorldC*)world, (const WorldC*)world_old, world_size, mk(cam[16 + 12], cam[16 + 13], cam[16 + 14])};
+    std::vector<Chain> tent(n), snap(n);
+    std::vector<uint32_t> own(n);
+    std::vector<float> sum_mc(n, 0.0f), pdf(n, 0.0f);
+    std::vector<F3> color_sum(n, splat(0.0f)), out_rad(n, splat(0.0f)), last_n(n), last_a(n), last_e(n);
+    std::vector<Res> R(n), smp(n);
+    std::vector<uint8_t> smp_valid(n, 0);
+    std::memset(R.data(), 0, n * sizeof(Res));
+    std::memset(smp.data(), 0, n * sizeof(Res));
+    auto mc_at = [&](uint64_t idx) {
+        return idx < mc_count ? chain_from_words(mc_old + 8 * idx) : czero();
+    };
+    for (uint64_t p = 0; p < n; ++p) {
+        uint32_t x = (uint32_t)(p % width), y = (uint32_t)(p / width);
+        own[p] = orc_rand_u32(x * y) + seed_offset + y;
+        tent[p] = mc_at(p);
+    }
+    uint64_t verts = 0;
+    for (uint32_t si = 0; si < samples; ++si) {
+        snap = tent;
+        for (uint64_t p = 0; p < n; ++p) {
+            uint32_t x = (uint32_t)(p % width), y = (uint32_t)(p / width);
+            // fill_mc_screen_space_states (general.wesl:367-385)
+            Chain cands[16];
+            uint32_t base = (x / subgroup) * subgroup;
+            uint32_t lane = orc_rand_u32(own[p]) % subgroup;
+            Chain lc = base + lane < width ? snap[(uint64_t)y * width + base + lane] : czero();
+            for (int k = 0; k < 4; ++k) cands[k] = lc;
+            for (int k = 4; k < 16; ++k) {
+                uint32_t xr = orc_rand_u32(own[p]);
+                own[p] = xr;
+                uint32_t yr = orc_rand_u32(own[p]);
+                own[p] = yr;
+                int32_t dx = (int32_t)(xr % 91u) - 45, dy = (int32_t)(yr % 91u) - 45;
+                uint32_t nx = (uint32_t)((int32_t)x + dx), ny = (uint32_t)((int32_t)y + dy);
+                cands[k] = mc_at((uint64_t)(uint32_t)(nx + ny * width));
+            }
+            float o3[3], d3[3];
+            orc_create_ray(cam, x, y, width, height, o3, d3);
+            SampleRet rs = rt_sample_guided(s, grid, o, mk(o3[0], o3[1], o3[2]), mk(d3[0], d3[1], d3[2]),
+                                            own[p], cands, tent[p], sum_mc[p]);
+            verts += rs.vertices;
+            last_n[p] = rs.normal;
+            last_a[p] = rs.albedo;
+            last_e[p] = rs.emission;
+            color_sum[p] = add(color_sum[p], rs.color);
+            out_rad[p] = add(out_rad[p], rs.out_radiance);
+            own[p] = orc_rand_u32(own[p]);
+            Res cur;
+            std::memset(&cur, 0, sizeof cur);
+            cur.vp[0] = rs.visible_point.x;
+            cur.vp[1] = rs.visible_point.y;
+            cur.vp[2] = rs.visible_point.z;
+            cur.vn = n2u(rs.visible_normal);
+            cur.sp[0] = rs.point.x;
+            cur.sp[1] = rs.point.y;
+            cur.sp[2] = rs.point.z;
+            cur.sn = n2u(rs.normal);
+            cur.ty = rs.ty;
+            cur.rough = 0.0f;
+            if (restir == 2u) {
+                F3 L = divs(out_rad[p], (float)samples);
+                cur.L[0] = L.x;
+                cur.L[1] = L.y;
+                cur.L[2] = L.z;
+                smp[p] = cur;
+                smp_valid[p] = 1;
+                pdf[p] += rs.pdf;
+            } else if (restir == 1u) {
+                cur.L[0] = rs.out_radiance.x;
+                cur.L[1] = rs.out_radiance.y;
+                cur.L[2] = rs.out_radiance.z;
+                if (cur.ty != 0xFFu) {
+                    res_update(R[p], cur, res_p_hat(cur), own[p]);
+                    own[p] = orc_rand_u32(own[p]);
+                    R[p].W = safe_div1(R[p].w, (float)(R[p].mv & 0x7FFFFFFFu) * res_p_hat(R[p]));
+                } else {
+                    R[p].ty = 0xFFu;
+                }
+            }
+        }
+        // subgroupShuffle(tentative, (lane + 1) % sub_size)
+        std::vector<Chain> next(n);
+        for (uint64_t p = 0; p < n; ++p) {
+            uint32_t x = (uint32_t)(p % width), y = (uint32_t)(p / width);
+            uint32_t base = (x / subgroup) * subgroup;
+            uint32_t active = std::min(subgroup, width - base);
+            next[p] = tent[(uint64_t)y * width + base + ((x - base + 1u) % active)];
+        }
+        tent.swap(next);
+    }
+    for (uint64_t p = 0; p < n; ++p) {
+        if (restir != 0u) {
+            float* inf = info_out + 9 * p;
+            inf[0] = grid.cam.x;
+            inf[1] = grid.cam.y;
+            inf[2] = grid.cam.z;
+            inf[3] = last_e[p].x;
+            inf[4] = last_e[p].y;
+            inf[5] = last_e[p].z;
+            inf[6] = last_a[p].x;
+            inf[7] = last_a[p].y;
+            inf[8] = last_a[p].z;
+        }
+        if (p < mc_count) chain_to_words(tent[p], mc_out + 8 * p);
+        float* oc = out_color + 4 * p;
+        oc[0] = std::fmax(color_sum[p].x / (float)samples, 0.0f);
+        oc[1] = std::fmax(color_sum[p].y / (float)samples, 0.0f);
+        oc[2] = std::fmax(color_sum[p].z / (float)samples, 0.0f);
+        oc[3] = 1.0f;
+        float* on = out_normal + 4 * p;
+        on[0] = last_n[p].x;
+        on[1] = last_n[p].y;
+        on[2] = last_n[p].z;
+        on[3] = 1.0f;
+        float* oa = out_albedo + 4 * p;
+        oa[0] = last_a[p].x;
+        oa[1] = last_a[p].y;
+        oa[2] = last_a[p].z;
+        oa[3] = 1.0f;
+        if (restir == 2u) {
+            if (smp[p].ty != 0xFFu && smp_valid[p]) {
+                res_update(R[p], smp[p], res_p_hat(smp[p]), own[p]);
+                own[p] = orc_rand_u32(own[p]);
+                R[p].W = safe_div1(R[p].w, (float)(R[p].mv & 0x7FFFFFFFu) * res_p_hat(R[p]));
+            } else if (smp[p].ty == 0xFFu) {
+                R[p].ty = 0xFFu;
+            }
+        }
+        if (p < gi_count) std::memcpy((char*)gi_out + 64 * p, &R[p], 64);
+    }
+    return verts;
+}
 
 // descs: offset, width, height per texture.
 void* orc_scene_create(const float* verts, uint64_t nverts, const uint32_t* idx, uint64_t ntri,

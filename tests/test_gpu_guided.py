@@ -1,0 +1,114 @@
+"""GPU guided mode (Markov-chain guiding + world grid + ReSTIR reservoir per
+pixel, general.wesl:148-610) against the oracle's sequential restatement.
+
+The world grid is updated by concurrent paths with relaxed atomics and a
+try-lock, as in the reference, so runs are not bit-reproducible: parity is
+statistical (image relMSE at the level of the Monte-Carlo noise, image means)
+plus structural checks of the state the frame leaves behind.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from lambent_amd import api, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_mse(img, ref):
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+
+
+def _guided(spec, scene, samples, seed_offset=0, world=100_000, bounces=8, quality=api.Medium,
+            restir=api.RestirTemporalMode.PerPixel, bufs=None):
+    pipe = api.RayTracer(quality).create_pipeline(
+        api.PathTracerOpts(samples=samples, bounces=bounces, guiding=api.Guiding.MARKOV_CHAIN,
+                           restir_temporal=restir))
+    bufs = bufs or api.DataBuffers(spec.width, spec.height, world, api.BufferType.all())
+    pipe.dispatch(scene, spec.camera(), bufs, seed_offset)
+    return bufs
+
+
+def test_guided_matches_oracle_statistically():
+    spec = scenes.cornell_box(64, 64)
+    scene = spec.build()
+    a = _guided(spec, scene, 64, seed_offset=0).output(api.Output.COLOR)[..., :3]
+    b = _guided(spec, scene, 64, seed_offset=9000).output(api.Output.COLOR)[..., :3]
+    st = oracle.GuidedState(64, 64, 100_000)
+    o, _, _, _ = oracle.trace_guided(oracle.Scene(spec), spec.camera().to_bytes(), 64, 64, st,
+                                     samples=64, bounces=8, seed_offset=4242)
+    o = o[..., :3]
+    e_ref, e_gpu = rel_mse(a, o), rel_mse(a, b)
+    assert np.isfinite(a).all()
+    assert e_ref < 1.5 * e_gpu + 1e-3, (e_ref, e_gpu)
+    assert abs(a.mean() - o.mean()) / o.mean() < 0.03
+
+
+def test_guided_frame_state():
+    spec = scenes.cornell_box(64, 48)
+    scene = spec.build()
+    bufs = _guided(spec, scene, 4, world=50_000)
+    cam = spec.camera()
+    info = np.frombuffer(bufs.state_bytes(api.StateBuffer.INFO), np.float32).reshape(-1, 9)
+    np.testing.assert_array_equal(info[:, :3], np.tile(cam.view_inverse[3, :3], (64 * 48, 1)))
+    albedo = bufs.output(api.Output.ALBEDO)[..., :3].reshape(-1, 3)
+    np.testing.assert_array_equal(info[:, 6:9], albedo)
+    gi = np.frombuffer(bufs.state_bytes(api.StateBuffer.GI_RESERVOIRS), oracle.RESERVOIR_DTYPE)
+    hit = gi["ty"] != 0xFF
+    assert hit.mean() > 0.5
+    assert ((gi["packed_confidence_valid"][hit] & 0x7FFFFFFF) == 1).all()
+    assert ((gi["packed_confidence_valid"][hit] >> 31) == 1).all()
+    assert np.isfinite(gi["W"]).all() and (gi["W"][hit] > 0).all()
+    mc = np.frombuffer(bufs.state_bytes(api.StateBuffer.MARKOV_CHAINS), np.float32).reshape(-1, 8)
+    assert (mc[:, 4] > 0).mean() > 0.2  # trained tentative chains (weight_sum)
+    world = np.frombuffer(bufs.state_bytes(api.StateBuffer.WORLD_MARKOV_CHAINS), oracle.WORLD_DTYPE)
+    used = world["secondary_hash"] != 0
+    assert used.sum() > 100
+    assert (world["secondary_hash"][used] & 1).all()  # secondary hashes are odd
+    assert (world["timeout"][used] == 30).all()
+    assert (world["lock"] == 0).all()
+    assert np.isfinite(world["luminance"]).all() and (world["luminance"] >= 0).all()
+    assert (world["luminance"] <= 100.0 * np.maximum(world["num_samples"], 1)[:, None] + 1e-3).all()
+
+
+def test_guided_frame_loop_with_resampling():
+    spec = scenes.cornell_box(64, 64)
+    scene = spec.build()
+    bufs = api.DataBuffers(64, 64, 100_000, api.BufferType.all())
+    pipe = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=4, bounces=8, guiding=api.Guiding.MARKOV_CHAIN))
+    means = []
+    for frame in range(6):
+        pipe.dispatch(scene, spec.camera(), bufs, frame)
+        bufs.temporal_resample(frame)
+        bufs.spatial_resample(5, 61, frame)
+        img = bufs.output(api.Output.COLOR)
+        assert np.isfinite(img).all()
+        means.append(float(img[..., :3].mean()))
+        bufs.advance_frame(api.BufferType.all(), api.AdvanceOptions(20))
+    gi = np.frombuffer(bufs.state_bytes(api.StateBuffer.OLD_GI_RESERVOIRS), oracle.RESERVOIR_DTYPE)
+    assert (gi["packed_confidence_valid"] & 0x7FFFFFFF).max() <= 20
+    plain_pipe = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=256, bounces=8, guiding=api.Guiding.OFF))
+    pb = api.DataBuffers(64, 64, 1)
+    plain_pipe.dispatch(scene, spec.camera(), pb, 0)
+    ref = float(pb.output(api.Output.COLOR)[..., :3].mean())
+    assert abs(np.mean(means[2:]) - ref) / ref < 0.25, (means, ref)
+
+
+@pytest.mark.parametrize("quality", [api.Low, api.High])
+def test_guided_other_qualities_run(quality):
+    spec = scenes.cornell_box(32, 32)
+    scene = spec.build()
+    bufs = _guided(spec, scene, 8, quality=quality, restir=api.RestirTemporalMode.PerSample)
+    img = bufs.output(api.Output.COLOR)
+    assert np.isfinite(img).all() and img[..., :3].mean() > 0.05
+
+
+def test_guided_procedural_scene_runs():
+    spec = scenes.procedural(100_000, seed=5, width=320, height=180)
+    scene = spec.build()
+    bufs = _guided(spec, scene, 1, bounces=5, world=1_000_000)
+    img = bufs.output(api.Output.COLOR)
+    assert np.isfinite(img).all()
+    assert bufs.path_vertices() > 320 * 180
