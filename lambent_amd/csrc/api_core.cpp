@@ -346,8 +346,9 @@ lmb_status lmb_scene_intersect_device(lmb_scene scene, const void* rays_dev, uin
     if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
     lmb_status st = ensure_spill(&scene->spill, query_threads(n));
     if (st) return st;
+    if (!scene->fetch.ptr && (st = scene->fetch.alloc(256))) return st;
     e = launch_intersect(scene->dev, rays_dev, n, hits_dev, scene->spill.ptr,
-                         (hipStream_t)stream);
+                         scene->fetch.as<uint32_t>(), (hipStream_t)stream);
     if (e != hipSuccess) return fail_hip(e, "intersect kernel launch");
     return LMB_OK;
 }
@@ -387,10 +388,11 @@ lmb_status lmb_primary_hits(lmb_scene scene, const lmb_camera* cam, uint32_t wid
     if (st) return st;
     st = ensure_spill(&scene->spill, query_threads(n));
     if (st) return st;
+    if (!scene->fetch.ptr && (st = scene->fetch.alloc(256))) return st;
     CameraDev cd;
     std::memcpy(&cd, cam, sizeof cd);
     e = launch_primary(scene->dev, cd, width, height, t_min, t_max, (uint32_t*)dp.ptr,
-                       (float*)dt.ptr, scene->spill.ptr, nullptr);
+                       (float*)dt.ptr, scene->spill.ptr, scene->fetch.as<uint32_t>(), nullptr);
     if (e != hipSuccess) return fail_hip(e, "primary kernel launch");
     e = hipMemcpy(prim, dp.ptr, n * 4, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return fail_hip(e, "hipMemcpy prim");

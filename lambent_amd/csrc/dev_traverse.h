@@ -144,30 +144,42 @@ struct ShortStack {
     }
 };
 
-// Closest hit over the BVH.  Ties in t resolve to the lower primitive index,
-// so the result does not depend on the traversal order.
-template <int BLOCK>
-__device__ __forceinline__ Hit trace_closest(const Bvh8Node* __restrict__ nodes,
-                                             const LeafTri* __restrict__ tris,
-                                             uint32_t num_nodes, const Ray& r,
-                                             ShortStack<BLOCK>& st) {
+// Closest-hit traversal as a per-lane state machine: init() sets up a ray,
+// each step() visits one node (box tests of its 8 children, then the
+// triangles of its hit leaves) and chooses the next node, or pops the stack.
+// Ties in t resolve to the lower primitive index, so the result does not
+// depend on the traversal order.
+struct Traversal {
+    Ray r;
+    float ix, iy, iz;
+    uint32_t oct;
+    Shear sh;
     Hit h;
-    h.prim = kMissPrim;
-    h.leaf = kMissPrim;
-    h.t = r.tmax;
-    h.u = 0.0f;
-    h.v = 0.0f;
-    if (num_nodes == 0) return h;
-    // Near-zero direction components get a large positive inverse: the slab
-    // test then accepts exactly the boxes whose slab contains the origin.
-    const float ix = fabsf(r.dx) < 1e-30f ? kZeroDirInv : 1.0f / r.dx;
-    const float iy = fabsf(r.dy) < 1e-30f ? kZeroDirInv : 1.0f / r.dy;
-    const float iz = fabsf(r.dz) < 1e-30f ? kZeroDirInv : 1.0f / r.dz;
-    const uint32_t oct = (ix < 0.0f ? 1u : 0u) | (iy < 0.0f ? 2u : 0u) | (iz < 0.0f ? 4u : 0u);
-    const Shear sh = make_shear(r.dx, r.dy, r.dz);
-    st.sp = 0;
-    uint32_t node = 0;
-    while (true) {
+    uint32_t node;
+    bool active;
+
+    __device__ __forceinline__ void init(const Ray& ray, uint32_t num_nodes) {
+        r = ray;
+        h.prim = kMissPrim;
+        h.leaf = kMissPrim;
+        h.t = r.tmax;
+        h.u = 0.0f;
+        h.v = 0.0f;
+        // Near-zero direction components get a large positive inverse: the
+        // slab test then accepts exactly the boxes whose slab contains the
+        // origin.
+        ix = fabsf(r.dx) < 1e-30f ? kZeroDirInv : 1.0f / r.dx;
+        iy = fabsf(r.dy) < 1e-30f ? kZeroDirInv : 1.0f / r.dy;
+        iz = fabsf(r.dz) < 1e-30f ? kZeroDirInv : 1.0f / r.dz;
+        oct = (ix < 0.0f ? 1u : 0u) | (iy < 0.0f ? 2u : 0u) | (iz < 0.0f ? 4u : 0u);
+        sh = make_shear(r.dx, r.dy, r.dz);
+        node = 0;
+        active = num_nodes != 0;
+    }
+
+    template <int BLOCK>
+    __device__ __forceinline__ void step(const Bvh8Node* __restrict__ nodes,
+                                         const LeafTri* __restrict__ tris, ShortStack<BLOCK>& st) {
         const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
         const uint4 w0 = np[0];
         const uint4 w1 = np[1];
@@ -241,7 +253,10 @@ __device__ __forceinline__ Hit trace_closest(const Bvh8Node* __restrict__ nodes,
             ghits = node_hits;
             gimask = imask;
         } else {
-            if (st.sp == 0) break;
+            if (st.sp == 0) {
+                active = false;
+                return;
+            }
             const uint2 e = st.pop();
             gbase = e.x;
             ghits = e.y & 0xFFu;
@@ -253,7 +268,65 @@ __device__ __forceinline__ Hit trace_closest(const Bvh8Node* __restrict__ nodes,
         const uint32_t slot = key ^ oct;
         node = gbase + __builtin_popcount(gimask & ((1u << slot) - 1u));
     }
-    return h;
+};
+
+// Whole traversal of one ray by one lane.
+template <int BLOCK>
+__device__ __forceinline__ Hit trace_closest(const Bvh8Node* __restrict__ nodes,
+                                             const LeafTri* __restrict__ tris,
+                                             uint32_t num_nodes, const Ray& r,
+                                             ShortStack<BLOCK>& st) {
+    Traversal tv;
+    tv.init(r, num_nodes);
+    st.sp = 0;
+    while (tv.active) tv.step<BLOCK>(nodes, tris, st);
+    return tv.h;
+}
+
+// Persistent traversal over a ray queue with wave-level refill: whenever at
+// least REFILL lanes of the wave are idle, the wave takes that many rays from
+// the queue with one atomic and each idle lane starts one, so lanes do not
+// wait for the longest ray of the wave.  `fetch` counts handed-out rays.
+// Source must provide: Ray load(uint64_t slot), void store(uint64_t slot, Hit).
+template <int BLOCK, int REFILL, class Source>
+__device__ __forceinline__ void traverse_queue(const Bvh8Node* __restrict__ nodes,
+                                               const LeafTri* __restrict__ tris,
+                                               uint32_t num_nodes, uint64_t count,
+                                               uint32_t* fetch, ShortStack<BLOCK>& st,
+                                               Source& src) {
+    const uint32_t lane = __lane_id();
+    Traversal tv;
+    tv.active = false;
+    uint64_t slot = 0;
+    bool exhausted = false;  // wave-uniform
+    while (true) {
+        if (!exhausted) {
+            const uint64_t idle = __ballot(!tv.active);
+            const uint32_t n_idle = (uint32_t)__popcll(idle);
+            if (n_idle >= (uint32_t)REFILL) {
+                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(fetch, n_idle);
+                base = __shfl(base, (int)leader);
+                if ((uint64_t)base + n_idle >= count) exhausted = true;
+                if (!tv.active) {
+                    const uint64_t mine =
+                        (uint64_t)base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                    if (mine < count) {
+                        slot = mine;
+                        tv.init(src.load(slot), num_nodes);
+                        st.sp = 0;
+                        if (!tv.active) src.store(slot, tv.h);
+                    }
+                }
+            }
+        }
+        if (tv.active) {
+            tv.step<BLOCK>(nodes, tris, st);
+            if (!tv.active) src.store(slot, tv.h);
+        }
+        if (exhausted && !__any(tv.active)) break;
+    }
 }
 
 }  // namespace lmb

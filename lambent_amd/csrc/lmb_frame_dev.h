@@ -16,7 +16,9 @@ constexpr uint32_t kMaxBounces = 64;
 // [kCounterVerticesTotal] (64-bit) path vertices since the last stats read.
 constexpr uint32_t kCounterVertices = kMaxBounces + 2;
 constexpr uint32_t kCounterVerticesTotal = kMaxBounces + 4;
-constexpr uint32_t kNumCounters = kMaxBounces + 6;
+// [kCounterFetch + b]: rays of bounce b handed out to traversal lanes.
+constexpr uint32_t kCounterFetch = kMaxBounces + 6;
+constexpr uint32_t kNumCounters = kCounterFetch + kMaxBounces + 1;
 
 struct FrameDev {
     uint32_t width, height;

@@ -60,6 +60,7 @@ struct lmb_scene_s {
     int device = 0;
     lmb::SceneDev dev{};
     lmb::DeviceBuffer nodes, tris, prim_material, materials, texels, tex;
+    lmb::DeviceBuffer fetch;  // ray counter of the query kernels
     lmb::SpillBuffer spill;
     lmb_bvh_stats stats{};
     uint64_t num_prims = 0;

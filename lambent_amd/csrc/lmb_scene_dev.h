@@ -33,11 +33,12 @@ uint64_t query_threads(uint64_t n);
 
 // rays: n x 32 bytes (origin, tmin, direction, tmax); hits: n x 16 bytes
 // (prim, t, u, v).  spill: query_threads(n) * (kStackMax - kStackLds) uint2.
+// fetch: a device u32 the kernel uses as its ray counter (zeroed here).
 hipError_t launch_intersect(const SceneDev& s, const void* rays, uint64_t n, void* hits,
-                            uint2* spill, hipStream_t stream);
+                            uint2* spill, uint32_t* fetch, hipStream_t stream);
 
 hipError_t launch_primary(const SceneDev& s, const CameraDev& cam, uint32_t width,
                           uint32_t height, float tmin, float tmax, uint32_t* prim,
-                          float* t, uint2* spill, hipStream_t stream);
+                          float* t, uint2* spill, uint32_t* fetch, hipStream_t stream);
 
 }  // namespace lmb

@@ -50,6 +50,7 @@ __global__ __launch_bounds__(kPtBlock) void k_generate(CameraDev cam, FrameDev f
     if (gtid == 0) {
         f.counters[0] = (uint32_t)f.npix;
         for (uint32_t i = 1; i <= kMaxBounces; ++i) f.counters[i] = 0;
+        for (uint32_t i = 0; i <= kMaxBounces; ++i) f.counters[kCounterFetch + i] = 0;
         if (sample == 0) *vertex_counter(f) = 0;
     }
     for (uint64_t pix = gtid; pix < f.npix; pix += stride) {
@@ -72,37 +73,49 @@ __global__ __launch_bounds__(kPtBlock) void k_generate(CameraDev cam, FrameDev f
     }
 }
 
-__global__ __launch_bounds__(kPtBlock) void k_extend(SceneDev s, FrameDev f, TraceParams p,
-                                                     uint32_t bounce) {
-    __shared__ uint2 lds_stack[kStackLds * kPtBlock];
-    const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
-    const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
-    const uint32_t count = f.counters[bounce];
-    if (gtid == 0) {
-        atomicAdd((unsigned long long*)vertex_counter(f), (unsigned long long)count);
-        atomicAdd((unsigned long long*)(f.counters + kCounterVerticesTotal),
-                  (unsigned long long)count);
-    }
-    const float4* __restrict__ q = f.queue[bounce & 1];
-    ShortStack<kPtBlock> st;
-    st.lds = lds_stack + threadIdx.x;
-    st.spill = f.spill + gtid * (kStackMax - kStackLds);
-    for (uint64_t slot = gtid; slot < count; slot += stride) {
+// Ray source of the wavefront queue for traverse_queue().
+struct QueueSource {
+    const float4* __restrict__ q;
+    uint4* __restrict__ hits;
+    float tmin, tmax;
+    __device__ __forceinline__ Ray load(uint64_t slot) const {
         const float4 o = q[2 * slot];
         const float4 d = q[2 * slot + 1];
         Ray r;
         r.ox = o.x;
         r.oy = o.y;
         r.oz = o.z;
-        r.tmin = p.tmin;
+        r.tmin = tmin;
         r.dx = d.x;
         r.dy = d.y;
         r.dz = d.z;
-        r.tmax = p.tmax;
-        const Hit h = trace_closest<kPtBlock>(s.nodes, s.tris, s.num_nodes, r, st);
-        f.hits[slot] = make_uint4(h.leaf, __float_as_uint(h.t), __float_as_uint(h.u),
-                                  __float_as_uint(h.v));
+        r.tmax = tmax;
+        return r;
     }
+    __device__ __forceinline__ void store(uint64_t slot, const Hit& h) const {
+        hits[slot] = make_uint4(h.leaf, __float_as_uint(h.t), __float_as_uint(h.u),
+                                __float_as_uint(h.v));
+    }
+};
+
+constexpr int kRefill = 16;
+
+__global__ __launch_bounds__(kPtBlock) void k_extend(SceneDev s, FrameDev f, TraceParams p,
+                                                     uint32_t bounce) {
+    __shared__ uint2 lds_stack[kStackLds * kPtBlock];
+    const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
+    const uint32_t count = f.counters[bounce];
+    if (gtid == 0) {
+        atomicAdd((unsigned long long*)vertex_counter(f), (unsigned long long)count);
+        atomicAdd((unsigned long long*)(f.counters + kCounterVerticesTotal),
+                  (unsigned long long)count);
+    }
+    ShortStack<kPtBlock> st;
+    st.lds = lds_stack + threadIdx.x;
+    st.spill = f.spill + gtid * (kStackMax - kStackLds);
+    QueueSource src{f.queue[bounce & 1], f.hits, p.tmin, p.tmax};
+    traverse_queue<kPtBlock, kRefill>(s.nodes, s.tris, s.num_nodes, count,
+                                      f.counters + kCounterFetch + bounce, st, src);
 }
 
 template <uint32_t Q>

@@ -1,4 +1,5 @@
-// query.hip - ray-query kernels: arbitrary rays and camera primary rays.
+// query.hip - ray-query kernels: arbitrary rays and camera primary rays, both
+// through the persistent queue traversal (traverse_queue).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -10,6 +11,7 @@
 namespace lmb {
 
 constexpr int kQueryBlock = 256;
+constexpr int kQueryRefill = 16;
 
 int device_cu_count() {
     static int cached_dev = -1;
@@ -35,38 +37,24 @@ uint64_t query_threads(uint64_t n) {
     return blocks * kQueryBlock;
 }
 
-__global__ __launch_bounds__(kQueryBlock) void k_intersect(SceneDev s, const Ray* __restrict__ rays,
-                                                           uint64_t n, uint4* __restrict__ hits,
-                                                           uint2* __restrict__ spill) {
-    __shared__ uint2 lds_stack[kStackLds * kQueryBlock];
-    const uint64_t gtid = (uint64_t)blockIdx.x * kQueryBlock + threadIdx.x;
-    const uint64_t stride = (uint64_t)gridDim.x * kQueryBlock;
-    ShortStack<kQueryBlock> st;
-    st.lds = lds_stack + threadIdx.x;
-    st.spill = spill + gtid * (kStackMax - kStackLds);
-    for (uint64_t i = gtid; i < n; i += stride) {
-        Ray r = rays[i];
-        Hit h = trace_closest<kQueryBlock>(s.nodes, s.tris, s.num_nodes, r, st);
+struct RaySource {
+    const Ray* __restrict__ rays;
+    uint4* __restrict__ hits;
+    __device__ __forceinline__ Ray load(uint64_t i) const { return rays[i]; }
+    __device__ __forceinline__ void store(uint64_t i, const Hit& h) const {
         hits[i] = make_uint4(h.prim, __float_as_uint(h.t), __float_as_uint(h.u),
                              __float_as_uint(h.v));
     }
-}
+};
 
-__global__ __launch_bounds__(kQueryBlock) void k_primary(SceneDev s, CameraDev cam,
-                                                         uint32_t width, uint32_t height,
-                                                         float tmin, float tmax,
-                                                         uint32_t* __restrict__ prim_out,
-                                                         float* __restrict__ t_out,
-                                                         uint2* __restrict__ spill) {
-    __shared__ uint2 lds_stack[kStackLds * kQueryBlock];
-    const uint64_t gtid = (uint64_t)blockIdx.x * kQueryBlock + threadIdx.x;
-    const uint64_t stride = (uint64_t)gridDim.x * kQueryBlock;
-    const uint64_t n = (uint64_t)width * height;
-    ShortStack<kQueryBlock> st;
-    st.lds = lds_stack + threadIdx.x;
-    st.spill = spill + gtid * (kStackMax - kStackLds);
-    for (uint64_t i = gtid; i < n; i += stride) {
-        uint32_t x = (uint32_t)(i % width), y = (uint32_t)(i / width);
+struct CameraSource {
+    CameraDev cam;
+    uint32_t width, height;
+    float tmin, tmax;
+    uint32_t* __restrict__ prim_out;
+    float* __restrict__ t_out;
+    __device__ __forceinline__ Ray load(uint64_t i) const {
+        const uint32_t x = (uint32_t)(i % width), y = (uint32_t)(i / width);
         float o[3], d[3];
         create_ray(cam, x, y, width, height, o, d);
         Ray r;
@@ -78,29 +66,65 @@ __global__ __launch_bounds__(kQueryBlock) void k_primary(SceneDev s, CameraDev c
         r.dy = d[1];
         r.dz = d[2];
         r.tmax = tmax;
-        Hit h = trace_closest<kQueryBlock>(s.nodes, s.tris, s.num_nodes, r, st);
+        return r;
+    }
+    __device__ __forceinline__ void store(uint64_t i, const Hit& h) const {
         prim_out[i] = h.prim;
         t_out[i] = h.t;
     }
+};
+
+__global__ __launch_bounds__(kQueryBlock) void k_intersect(SceneDev s, const Ray* __restrict__ rays,
+                                                           uint64_t n, uint4* __restrict__ hits,
+                                                           uint2* __restrict__ spill,
+                                                           uint32_t* __restrict__ fetch) {
+    __shared__ uint2 lds_stack[kStackLds * kQueryBlock];
+    const uint64_t gtid = (uint64_t)blockIdx.x * kQueryBlock + threadIdx.x;
+    ShortStack<kQueryBlock> st;
+    st.lds = lds_stack + threadIdx.x;
+    st.spill = spill + gtid * (kStackMax - kStackLds);
+    RaySource src{rays, hits};
+    traverse_queue<kQueryBlock, kQueryRefill>(s.nodes, s.tris, s.num_nodes, n, fetch, st, src);
+}
+
+__global__ __launch_bounds__(kQueryBlock) void k_primary(SceneDev s, CameraDev cam,
+                                                         uint32_t width, uint32_t height,
+                                                         float tmin, float tmax,
+                                                         uint32_t* __restrict__ prim_out,
+                                                         float* __restrict__ t_out,
+                                                         uint2* __restrict__ spill,
+                                                         uint32_t* __restrict__ fetch) {
+    __shared__ uint2 lds_stack[kStackLds * kQueryBlock];
+    const uint64_t gtid = (uint64_t)blockIdx.x * kQueryBlock + threadIdx.x;
+    ShortStack<kQueryBlock> st;
+    st.lds = lds_stack + threadIdx.x;
+    st.spill = spill + gtid * (kStackMax - kStackLds);
+    CameraSource src{cam, width, height, tmin, tmax, prim_out, t_out};
+    traverse_queue<kQueryBlock, kQueryRefill>(s.nodes, s.tris, s.num_nodes,
+                                              (uint64_t)width * height, fetch, st, src);
 }
 
 hipError_t launch_intersect(const SceneDev& s, const void* rays, uint64_t n, void* hits,
-                            uint2* spill, hipStream_t stream) {
+                            uint2* spill, uint32_t* fetch, hipStream_t stream) {
     if (n == 0) return hipSuccess;
+    hipError_t e = hipMemsetAsync(fetch, 0, sizeof(uint32_t), stream);
+    if (e != hipSuccess) return e;
     uint64_t threads = query_threads(n);
     hipLaunchKernelGGL(k_intersect, dim3((uint32_t)(threads / kQueryBlock)), dim3(kQueryBlock),
-                       0, stream, s, (const Ray*)rays, n, (uint4*)hits, spill);
+                       0, stream, s, (const Ray*)rays, n, (uint4*)hits, spill, fetch);
     return hipGetLastError();
 }
 
 hipError_t launch_primary(const SceneDev& s, const CameraDev& cam, uint32_t width,
                           uint32_t height, float tmin, float tmax, uint32_t* prim, float* t,
-                          uint2* spill, hipStream_t stream) {
+                          uint2* spill, uint32_t* fetch, hipStream_t stream) {
     uint64_t n = (uint64_t)width * height;
     if (n == 0) return hipSuccess;
+    hipError_t e = hipMemsetAsync(fetch, 0, sizeof(uint32_t), stream);
+    if (e != hipSuccess) return e;
     uint64_t threads = query_threads(n);
     hipLaunchKernelGGL(k_primary, dim3((uint32_t)(threads / kQueryBlock)), dim3(kQueryBlock), 0,
-                       stream, s, cam, width, height, tmin, tmax, prim, t, spill);
+                       stream, s, cam, width, height, tmin, tmax, prim, t, spill, fetch);
     return hipGetLastError();
 }
 

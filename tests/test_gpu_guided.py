@@ -58,7 +58,11 @@ def test_guided_frame_state():
     assert hit.mean() > 0.5
     assert ((gi["packed_confidence_valid"][hit] & 0x7FFFFFFF) == 1).all()
     assert ((gi["packed_confidence_valid"][hit] >> 31) == 1).all()
-    assert np.isfinite(gi["W"]).all() and (gi["W"][hit] > 0).all()
+    # one sample per reservoir: W = w / (1 * p_hat) = 1, except that safe_div
+    # (shared.wesl:294-304) returns 0 when p_hat = max(|L|, 1e-5) is < 1e-4
+    p_hat = np.maximum(np.linalg.norm(gi["out_radiance"].astype(np.float64), axis=1), 1e-5)
+    expect_w = np.where(p_hat < 1e-4, 0.0, 1.0)
+    np.testing.assert_allclose(gi["W"][hit], expect_w[hit], atol=1e-6)
     mc = np.frombuffer(bufs.state_bytes(api.StateBuffer.MARKOV_CHAINS), np.float32).reshape(-1, 8)
     assert (mc[:, 4] > 0).mean() > 0.2  # trained tentative chains (weight_sum)
     world = np.frombuffer(bufs.state_bytes(api.StateBuffer.WORLD_MARKOV_CHAINS), oracle.WORLD_DTYPE)
