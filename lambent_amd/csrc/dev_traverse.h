@@ -202,26 +202,35 @@ struct Traversal {
         const uint32_t fy0 = (oct & 2u) ? w2.z : w4.x, fy1 = (oct & 2u) ? w2.w : w4.y;
         const uint32_t nz0 = (oct & 4u) ? w4.z : w3.x, nz1 = (oct & 4u) ? w4.w : w3.y;
         const uint32_t fz0 = (oct & 4u) ? w3.x : w4.z, fz1 = (oct & 4u) ? w3.y : w4.w;
-        const float best_far = h.t * kRobustFar;
+        // Plane distance t = q * (2^e / d) + (p - o) / d: one fma per plane.
+        // 2^e / d is exact (power-of-two scaling); (p - o) / d carries two
+        // roundings, so each t is within 2.0001 eps |O| + 2 eps |t| of the
+        // distance to the exact dequantized plane.  `slack` widens every box
+        // by that bound, keeping the test conservative.
+        const float Sx = fminf(fmaxf(sx * ix, -1e38f), 1e38f);
+        const float Sy = fminf(fmaxf(sy * iy, -1e38f), 1e38f);
+        const float Sz = fminf(fmaxf(sz * iz, -1e38f), 1e38f);
+        const float Ox = (px - r.ox) * ix;
+        const float Oy = (py - r.oy) * iy;
+        const float Oz = (pz - r.oz) * iz;
+        const float node_slack =
+            fmaxf(fmaxf(fabsf(Ox), fabsf(Oy)), fabsf(Oz)) * (4.0002f * 5.9604645e-8f);
+        const float best = h.t;
         uint32_t node_hits = 0, tri_hits = 0;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             const uint32_t meta = byte_of(w1.z, w1.w, s);
-            const float qnx = (float)byte_of(nx0, nx1, s);
-            const float qfx = (float)byte_of(fx0, fx1, s);
-            const float qny = (float)byte_of(ny0, ny1, s);
-            const float qfy = (float)byte_of(fy0, fy1, s);
-            const float qnz = (float)byte_of(nz0, nz1, s);
-            const float qfz = (float)byte_of(fz0, fz1, s);
-            const float tnx = (fmaf(qnx, sx, px) - r.ox) * ix;
-            const float tfx = (fmaf(qfx, sx, px) - r.ox) * ix;
-            const float tny = (fmaf(qny, sy, py) - r.oy) * iy;
-            const float tfy = (fmaf(qfy, sy, py) - r.oy) * iy;
-            const float tnz = (fmaf(qnz, sz, pz) - r.oz) * iz;
-            const float tfz = (fmaf(qfz, sz, pz) - r.oz) * iz;
-            const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tmin));
-            const float tf = fminf(fminf(tfx, tfy), tfz) * kRobustFar;
-            const bool hit = meta != kMetaEmpty && tn <= tf && tn <= best_far;
+            const float tnx = fmaf((float)byte_of(nx0, nx1, s), Sx, Ox);
+            const float tfx = fmaf((float)byte_of(fx0, fx1, s), Sx, Ox);
+            const float tny = fmaf((float)byte_of(ny0, ny1, s), Sy, Oy);
+            const float tfy = fmaf((float)byte_of(fy0, fy1, s), Sy, Oy);
+            const float tnz = fmaf((float)byte_of(nz0, nz1, s), Sz, Oz);
+            const float tfz = fmaf((float)byte_of(fz0, fz1, s), Sz, Oz);
+            const float tn = fmaxf(fmaxf(tnx, tny), tnz);
+            const float tf = fminf(fminf(tfx, tfy), tfz);
+            const float slack = fmaf(fabsf(tf), 4.0f * 5.9604645e-8f, node_slack);
+            const bool hit = meta != kMetaEmpty && tn <= fminf(tf, best) + slack &&
+                             r.tmin <= tf + slack;
             if (hit) {
                 if (meta == kMetaInternal) {
                     node_hits |= 1u << (s ^ oct);
