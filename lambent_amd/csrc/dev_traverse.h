@@ -203,32 +203,40 @@ struct Traversal {
         const uint32_t nz0 = (oct & 4u) ? w4.z : w3.x, nz1 = (oct & 4u) ? w4.w : w3.y;
         const uint32_t fz0 = (oct & 4u) ? w3.x : w4.z, fz1 = (oct & 4u) ? w3.y : w4.w;
         // Plane distance t = q * (2^e / d) + (p - o) / d: one fma per plane.
-        // 2^e / d is exact (power-of-two scaling); (p - o) / d carries two
-        // roundings, so each t is within 2.0001 eps |O| + 2 eps |t| of the
-        // distance to the exact dequantized plane.  `slack` widens every box
-        // by that bound, keeping the test conservative.
+        // With u = 2^-24, S = 2^e * fl(1/d) carries the one rounding of 1/d and
+        // O = fl(fl(p - o) * fl(1/d)) three, so against the distance to the
+        // exact dequantized plane the fma result errs by at most
+        // u |q S| + 3.0001 u |O| + u |t| <= 4.0002 u |O| + 2 u |t|.
+        // The |O| part is folded into the per-axis offsets (near planes use
+        // O - 6u|O|, far planes O + 6u|O|, covering the rounding of that fma
+        // too), so it never couples axes: a near-parallel axis with a huge
+        // |O| widens only its own planes.  The 2u |t| part is the per-child
+        // `slack` = 3u (|tn| + |tf|).  Every box the exact slab test hits is
+        // therefore hit here.
         const float Sx = fminf(fmaxf(sx * ix, -1e38f), 1e38f);
         const float Sy = fminf(fmaxf(sy * iy, -1e38f), 1e38f);
         const float Sz = fminf(fmaxf(sz * iz, -1e38f), 1e38f);
         const float Ox = (px - r.ox) * ix;
         const float Oy = (py - r.oy) * iy;
         const float Oz = (pz - r.oz) * iz;
-        const float node_slack =
-            fmaxf(fmaxf(fabsf(Ox), fabsf(Oy)), fabsf(Oz)) * (4.0002f * 5.9604645e-8f);
+        constexpr float kOffsetSlack = 6.0f * 5.9604645e-8f;
+        const float Onx = fmaf(-fabsf(Ox), kOffsetSlack, Ox), Ofx = fmaf(fabsf(Ox), kOffsetSlack, Ox);
+        const float Ony = fmaf(-fabsf(Oy), kOffsetSlack, Oy), Ofy = fmaf(fabsf(Oy), kOffsetSlack, Oy);
+        const float Onz = fmaf(-fabsf(Oz), kOffsetSlack, Oz), Ofz = fmaf(fabsf(Oz), kOffsetSlack, Oz);
         const float best = h.t;
         uint32_t node_hits = 0, tri_hits = 0;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             const uint32_t meta = byte_of(w1.z, w1.w, s);
-            const float tnx = fmaf((float)byte_of(nx0, nx1, s), Sx, Ox);
-            const float tfx = fmaf((float)byte_of(fx0, fx1, s), Sx, Ox);
-            const float tny = fmaf((float)byte_of(ny0, ny1, s), Sy, Oy);
-            const float tfy = fmaf((float)byte_of(fy0, fy1, s), Sy, Oy);
-            const float tnz = fmaf((float)byte_of(nz0, nz1, s), Sz, Oz);
-            const float tfz = fmaf((float)byte_of(fz0, fz1, s), Sz, Oz);
+            const float tnx = fmaf((float)byte_of(nx0, nx1, s), Sx, Onx);
+            const float tfx = fmaf((float)byte_of(fx0, fx1, s), Sx, Ofx);
+            const float tny = fmaf((float)byte_of(ny0, ny1, s), Sy, Ony);
+            const float tfy = fmaf((float)byte_of(fy0, fy1, s), Sy, Ofy);
+            const float tnz = fmaf((float)byte_of(nz0, nz1, s), Sz, Onz);
+            const float tfz = fmaf((float)byte_of(fz0, fz1, s), Sz, Ofz);
             const float tn = fmaxf(fmaxf(tnx, tny), tnz);
             const float tf = fminf(fminf(tfx, tfy), tfz);
-            const float slack = fmaf(fabsf(tf), 4.0f * 5.9604645e-8f, node_slack);
+            const float slack = (fabsf(tn) + fabsf(tf)) * (3.0f * 5.9604645e-8f);
             const bool hit = meta != kMetaEmpty && tn <= fminf(tf, best) + slack &&
                              r.tmin <= tf + slack;
             if (hit) {
