@@ -279,6 +279,10 @@ typedef struct lmb_kernel_stats {
     uint64_t extend_launches;
     uint64_t shade_launches;
     uint64_t path_vertices;    /* closest-hit queries issued meanwhile      */
+    double cache_infer_ms;     /* neural-cache inference launches (lmb_trace_nrc) */
+    double cache_train_ms;     /* neural-cache training steps (lmb_trace_nrc)     */
+    uint64_t cache_infer_launches;
+    uint64_t cache_train_steps;
 } lmb_kernel_stats;
 lmb_status lmb_data_buffers_set_profiling(lmb_data_buffers bufs, int32_t enable);
 lmb_status lmb_data_buffers_kernel_stats(lmb_data_buffers bufs,
@@ -311,6 +315,84 @@ lmb_status lmb_spatial_resample(lmb_data_buffers bufs, uint32_t is_samples,
  * AdvanceOptions{confidence_cap} (data_buffer/mod.rs:442-472). */
 lmb_status lmb_advance_frame(lmb_data_buffers bufs, uint32_t buffers_to_advance,
                              uint32_t confidence_cap, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Neural radiance cache                                               */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference, whose only cache is the world-grid
+ * radiance estimate that trains its guiding chains (general.wesl:398-444)
+ * and never ends a path.  This cache ends every path at its second vertex
+ * with a network estimate of the radiance scattered there, and trains the
+ * network online from a sparse set of paths traced to full depth.
+ *
+ * Network: 13 inputs in [0, 1] (position in the scene box, direction to the
+ * previous vertex and normal mapped from [-1, 1], roughness, albedo), a
+ * frequency encoding sin / cos (2^k pi x), k = 0..11, plus a constant 1,
+ * then 5 hidden layers of 64 with ReLU and 3 linear outputs; bf16 weights on
+ * the matrix cores with fp32 accumulation; relative L2 loss
+ * (y - t)^2 / (y^2 + 0.01) (y held constant in the denominator); Adam.
+ *
+ * Weights (LMB_NRC_NUM_WEIGHTS fp32), each layer [in][out] row-major:
+ *   [0, 20480)      layer 0: 320 x 64; rows 0..311 encoding feature
+ *                   s * 24 + 2 k + c (input s, frequency k, c 0 sin / 1 cos),
+ *                   row 312 the constant feature, rows 313..319 unused (0)
+ *   [20480, 36864)  layers 1..4: 64 x 64 each
+ *   [36864, 37056)  layer 5: 64 x 3 */
+#define LMB_NRC_NUM_WEIGHTS 37056u
+#define LMB_NRC_INPUTS 13u
+
+typedef struct lmb_nrc_s *lmb_nrc;
+
+typedef struct lmb_nrc_opts {
+    float learning_rate;   /* Adam step size                        (1e-3)  */
+    float beta1, beta2;    /* Adam moment decays                    (0.9, 0.99) */
+    float epsilon;         /* Adam denominator offset               (1e-8)  */
+    uint32_t train_paths;  /* training paths per lmb_trace_nrc      (65536) */
+    uint32_t train_steps;  /* optimiser steps per lmb_trace_nrc     (4)     */
+    uint32_t seed;         /* weight initialisation (He-uniform)            */
+    uint32_t reserved;
+} lmb_nrc_opts;
+
+void lmb_nrc_opts_default(lmb_nrc_opts *opts);
+lmb_status lmb_nrc_create(const lmb_nrc_opts *opts, lmb_nrc *out);
+lmb_status lmb_nrc_destroy(lmb_nrc nrc);
+/* fp32 master weights (LMB_NRC_NUM_WEIGHTS), host memory.  Setting them
+ * resets the optimiser state. */
+lmb_status lmb_nrc_get_weights(lmb_nrc nrc, float *weights);
+lmb_status lmb_nrc_set_weights(lmb_nrc nrc, const float *weights);
+/* Network outputs for n input rows (n x 13 floats) -> n x 3 floats. */
+lmb_status lmb_nrc_infer(lmb_nrc nrc, const float *inputs, uint64_t n, float *outputs);
+/* Gradient of the mean relative L2 loss over n samples (targets n x 3) with
+ * respect to the master weights (bf16 forward / backward, fp32 sums); the
+ * weights are not changed.  loss (optional) receives the mean loss. */
+lmb_status lmb_nrc_gradient(lmb_nrc nrc, const float *inputs, const float *targets,
+                            uint64_t n, float *grad, float *loss);
+/* One Adam step on n samples. */
+lmb_status lmb_nrc_train(lmb_nrc nrc, const float *inputs, const float *targets,
+                         uint64_t n, float *loss);
+/* One Adam step with a caller-supplied gradient (LMB_NRC_NUM_WEIGHTS). */
+lmb_status lmb_nrc_apply_gradient(lmb_nrc nrc, const float *grad);
+
+typedef struct lmb_nrc_stats {
+    uint64_t steps;          /* optimiser steps taken                     */
+    uint64_t records;        /* training records of the last lmb_trace_nrc */
+    uint64_t queries;        /* cache queries of its last sample          */
+    float loss;              /* mean loss of its training steps           */
+    uint32_t reserved;
+} lmb_nrc_stats;
+/* Blocks until the cache's work is done. */
+lmb_status lmb_nrc_stats_get(lmb_nrc nrc, lmb_nrc_stats *stats);
+
+/* One frame of the cached tracer: for each of the tracer's samples, every
+ * path is traced to its second vertex, where it ends with
+ * throughput * cache(vertex); about train_paths of them (chosen at random
+ * there) continue to full depth instead and become training records (their
+ * full-path estimate also goes to the image).  Then train_steps Adam steps
+ * on the records.  Uses the tracer's quality, samples, bounces, rr_start,
+ * t_min / t_max; guiding must be off. */
+lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera *cam,
+                         lmb_data_buffers bufs, uint32_t seed_offset, lmb_nrc nrc,
+                         void *stream);
 
 /* Waits for all work on `stream` (NULL = device-wide). */
 lmb_status lmb_sync(void *stream);

@@ -120,7 +120,22 @@ class _BvhStatsC(ctypes.Structure):
 class _KernelStatsC(ctypes.Structure):
     _fields_ = [("extend_ms", ctypes.c_double), ("shade_ms", ctypes.c_double),
                 ("extend_launches", ctypes.c_uint64), ("shade_launches", ctypes.c_uint64),
-                ("path_vertices", ctypes.c_uint64)]
+                ("path_vertices", ctypes.c_uint64), ("cache_infer_ms", ctypes.c_double),
+                ("cache_train_ms", ctypes.c_double), ("cache_infer_launches", ctypes.c_uint64),
+                ("cache_train_steps", ctypes.c_uint64)]
+
+
+class _NrcOptsC(ctypes.Structure):
+    _fields_ = [("learning_rate", ctypes.c_float), ("beta1", ctypes.c_float),
+                ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float),
+                ("train_paths", ctypes.c_uint32), ("train_steps", ctypes.c_uint32),
+                ("seed", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class _NrcStatsC(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_uint64), ("records", ctypes.c_uint64),
+                ("queries", ctypes.c_uint64), ("loss", ctypes.c_float),
+                ("reserved", ctypes.c_uint32)]
 
 
 class _TracerOptsC(ctypes.Structure):
@@ -186,6 +201,18 @@ _OPTIONAL = {
     "lmb_spatial_resample": ([_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp],
                              ctypes.c_int32),
     "lmb_advance_frame": ([_vp, ctypes.c_uint32, ctypes.c_uint32, _vp], ctypes.c_int32),
+    "lmb_nrc_opts_default": ([ctypes.POINTER(_NrcOptsC)], None),
+    "lmb_nrc_create": ([ctypes.POINTER(_NrcOptsC), ctypes.POINTER(_vp)], ctypes.c_int32),
+    "lmb_nrc_destroy": ([_vp], ctypes.c_int32),
+    "lmb_nrc_get_weights": ([_vp, _f32p], ctypes.c_int32),
+    "lmb_nrc_set_weights": ([_vp, _f32p], ctypes.c_int32),
+    "lmb_nrc_infer": ([_vp, _f32p, ctypes.c_uint64, _f32p], ctypes.c_int32),
+    "lmb_nrc_gradient": ([_vp, _f32p, _f32p, ctypes.c_uint64, _f32p, _f32p], ctypes.c_int32),
+    "lmb_nrc_train": ([_vp, _f32p, _f32p, ctypes.c_uint64, _f32p], ctypes.c_int32),
+    "lmb_nrc_apply_gradient": ([_vp, _f32p], ctypes.c_int32),
+    "lmb_nrc_stats_get": ([_vp, ctypes.POINTER(_NrcStatsC)], ctypes.c_int32),
+    "lmb_trace_nrc": ([_vp, _vp, ctypes.POINTER(_CameraC), _vp, ctypes.c_uint32, _vp, _vp],
+                      ctypes.c_int32),
 }
 for _name, (_args, _res) in _OPTIONAL.items():
     if hasattr(_lib, _name):
@@ -642,6 +669,14 @@ class Pipeline:
         _check(_lib.lmb_trace(self._h, scene.handle, ctypes.byref(camera._c), buffers.handle,
                               int(seed_offset) & 0xFFFFFFFF, stream))
 
+    def dispatch_cached(self, scene: Scene, camera: Camera, buffers: "DataBuffers",
+                        cache: "NeuralCache", seed_offset: int = 0, stream=None) -> None:
+        """One frame of the tracer with the neural radiance cache (paths end
+        at their second vertex in the cache; about cache.opts.train_paths of
+        them train it).  No counterpart in the reference."""
+        _check(_lib.lmb_trace_nrc(self._h, scene.handle, ctypes.byref(camera._c), buffers.handle,
+                                  int(seed_offset) & 0xFFFFFFFF, cache.handle, stream))
+
     def close(self):
         if self._h:
             _lib.lmb_tracer_destroy(self._h)
@@ -774,6 +809,96 @@ class DataBuffers:
     def close(self):
         if self._h:
             _lib.lmb_data_buffers_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- neural cache
+NRC_NUM_WEIGHTS = 37056
+NRC_INPUTS = 13
+
+
+@dataclass
+class NeuralCacheOpts:
+    """lmb_nrc_opts: Adam parameters, training paths and optimiser steps per
+    cached frame, weight-initialisation seed."""
+    learning_rate: float = 1e-3
+    beta1: float = 0.9
+    beta2: float = 0.99
+    epsilon: float = 1e-8
+    train_paths: int = 65536
+    train_steps: int = 4
+    seed: int = 1
+
+
+class NeuralCache:
+    """The neural radiance cache (MLP on the matrix cores, trained online)."""
+
+    def __init__(self, opts: Optional[NeuralCacheOpts] = None):
+        o = opts or NeuralCacheOpts()
+        c = _NrcOptsC()
+        c.learning_rate, c.beta1, c.beta2, c.epsilon = (o.learning_rate, o.beta1, o.beta2,
+                                                        o.epsilon)
+        c.train_paths, c.train_steps, c.seed = o.train_paths, o.train_steps, o.seed
+        h = _vp()
+        _check(_lib.lmb_nrc_create(ctypes.byref(c), ctypes.byref(h)))
+        self._h = h
+        self.opts = o
+
+    @property
+    def handle(self):
+        return self._h
+
+    def weights(self) -> np.ndarray:
+        w = np.empty(NRC_NUM_WEIGHTS, np.float32)
+        _check(_lib.lmb_nrc_get_weights(self._h, _fptr(w)))
+        return w
+
+    def set_weights(self, w: np.ndarray) -> None:
+        w = np.ascontiguousarray(w, np.float32)
+        if w.size != NRC_NUM_WEIGHTS:
+            raise ValueError(f"expected {NRC_NUM_WEIGHTS} weights")
+        _check(_lib.lmb_nrc_set_weights(self._h, _fptr(w)))
+
+    def infer(self, inputs: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, NRC_INPUTS)
+        out = np.empty((len(x), 3), np.float32)
+        _check(_lib.lmb_nrc_infer(self._h, _fptr(x), len(x), _fptr(out)))
+        return out
+
+    def gradient(self, inputs: np.ndarray, targets: np.ndarray):
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, NRC_INPUTS)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        g = np.empty(NRC_NUM_WEIGHTS, np.float32)
+        loss = ctypes.c_float()
+        _check(_lib.lmb_nrc_gradient(self._h, _fptr(x), _fptr(t), len(x), _fptr(g),
+                                     ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, inputs: np.ndarray, targets: np.ndarray) -> float:
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, NRC_INPUTS)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        loss = ctypes.c_float()
+        _check(_lib.lmb_nrc_train(self._h, _fptr(x), _fptr(t), len(x), ctypes.byref(loss)))
+        return loss.value
+
+    def apply_gradient(self, grad: np.ndarray) -> None:
+        g = np.ascontiguousarray(grad, np.float32)
+        _check(_lib.lmb_nrc_apply_gradient(self._h, _fptr(g)))
+
+    def stats(self) -> dict:
+        st = _NrcStatsC()
+        _check(_lib.lmb_nrc_stats_get(self._h, ctypes.byref(st)))
+        return {n: getattr(st, n) for n, _ in _NrcStatsC._fields_ if n != "reserved"}
+
+    def close(self):
+        if self._h:
+            _lib.lmb_nrc_destroy(self._h)
             self._h = _vp()
 
     def __del__(self):

@@ -2,6 +2,7 @@
 // the frame passes.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -418,6 +419,10 @@ lmb_status lmb_data_buffers_kernel_stats(lmb_data_buffers b, lmb_kernel_stats* s
     stats->extend_launches = n[kKernelExtend];
     stats->shade_launches = n[kKernelShade];
     stats->path_vertices = verts;
+    stats->cache_infer_ms = ms[kKernelCacheInfer];
+    stats->cache_train_ms = ms[kKernelCacheTrain];
+    stats->cache_infer_launches = n[kKernelCacheInfer];
+    stats->cache_train_steps = n[kKernelCacheTrain];
     return LMB_OK;
 }
 
@@ -448,6 +453,48 @@ lmb_status lmb_trace(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
     }
     if (e != hipSuccess) return fail_hip(e, "trace launch");
     return LMB_OK;
+}
+
+lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
+                         lmb_data_buffers bufs, uint32_t seed_offset, lmb_nrc nrc, void* stream) {
+    if (!tracer || !scene || !cam || !bufs || !nrc)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    const lmb_tracer_opts& o = tracer->opts;
+    if (o.guiding != LMB_GUIDING_OFF)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "the cached tracer needs guiding off");
+    TraceParams p;
+    p.quality = o.quality;
+    p.samples = o.samples;
+    p.tmin = o.t_min;
+    p.tmax = o.t_max;
+    p.restir = o.restir;
+    p.bounces = o.bounces;
+    p.rr_start = o.rr_start;
+    p.guiding = o.guiding;
+    p.seed_offset = seed_offset;
+    CameraDev cd;
+    std::memcpy(&cd, cam, sizeof cd);
+    const uint64_t npix = (uint64_t)bufs->width * bufs->height;
+    lmb_status st = nrc_ensure_queries(nrc, npix);
+    if (st) return st;
+    const uint32_t steps = nrc->opts.train_steps;
+    st = nrc_ensure_batch(nrc, (nrc->dev.rec_cap + steps - 1) / steps);
+    if (st) return st;
+    NrcTraceParams np{};
+    for (int k = 0; k < 3; ++k) {
+        const float ext = scene->bounds_hi[k] - scene->bounds_lo[k];
+        np.lo[k] = scene->bounds_lo[k];
+        np.inv_ext[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
+    }
+    np.train_expect = (float)nrc->opts.train_paths / (float)std::max<uint32_t>(1, o.samples);
+    const hipStream_t hs = (hipStream_t)stream;
+    hipError_t e = launch_fill_zero(nrc->dev.ctl + kNrcCtlRecords, 4, hs);
+    if (e == hipSuccess)
+        e = launch_trace_nrc(scene->dev, cd, bufs->dev, p, nrc->dev, np, hs, bufs->timer.get());
+    if (e != hipSuccess) return fail_hip(e, "cached trace launch");
+    if (nrc->opts.train_paths == 0) return LMB_OK;
+    return nrc_run_steps(nrc, nrc->dev.rec_in, nrc->dev.rec_target, nrc->dev.ctl + kNrcCtlRecords,
+                         nrc->dev.rec_cap, steps, true, nullptr, hs, bufs->timer.get());
 }
 
 static lmb_status need_spatial_buffers(lmb_data_buffers b) {

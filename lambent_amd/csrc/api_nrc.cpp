@@ -1,0 +1,396 @@
+// api_nrc.cpp - C ABI of the neural radiance cache (creation, weights, the
+// host-array inference / gradient / training calls) and the optimiser-step
+// driver shared with lmb_trace_nrc.  The cache has no counterpart in the
+// reference; its entry points are declared in include/lambent.h.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lmb_internal.h"
+#include "lmb_restir.h"
+
+using namespace lmb;
+
+namespace lmb {
+
+static uint64_t splitmix64(uint64_t& x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// He-uniform initialisation: U(-a, a), a = sqrt(6 / fan_in); the constant
+// feature's row (layer-0 bias) and the unused rows start at zero.
+static void init_weights(uint32_t seed, std::vector<float>& w) {
+    w.assign(kNrcNumWeights, 0.0f);
+    uint64_t st = 0x1234ABCDull ^ ((uint64_t)seed << 32);
+    auto uni = [&](float a) {
+        const double u = (double)(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0);
+        return (float)((2.0 * u - 1.0) * a);
+    };
+    const float a0 = std::sqrt(6.0f / (float)(kNrcEncoded + 1));
+    for (int in = 0; in < kNrcEncoded; ++in)
+        for (int out = 0; out < kNrcWidth; ++out) w[kNrcW0 + in * kNrcWidth + out] = uni(a0);
+    const float ah = std::sqrt(6.0f / (float)kNrcWidth);
+    for (int i = 0; i < kNrcHidden * kNrcWidth * kNrcWidth; ++i) w[kNrcWH + i] = uni(ah);
+    for (int i = 0; i < kNrcWidth * kNrcOut; ++i) w[kNrcW5 + i] = uni(ah);
+}
+
+lmb_status nrc_ensure_batch(lmb_nrc n, uint32_t batch) {
+    batch = (batch + kNrcChunk - 1) / kNrcChunk * kNrcChunk;
+    if (batch == 0) batch = kNrcChunk;
+    if (batch > n->batch_cap) {
+        const size_t S = batch;
+        lmb_status st = n->x0.alloc(S * kNrcIn * 2);
+        if (!st) st = n->h.alloc(S * 5 * kNrcWidth * 2);
+        if (!st) st = n->dz.alloc(S * 5 * kNrcWidth * 2);
+        if (!st) st = n->g.alloc(S * kNrcOut * 2);
+        if (!st) st = n->partial.alloc(S / kNrcChunk * kNrcNumWeights * sizeof(float));
+        if (st) {
+            n->batch_cap = 0;
+            return st;
+        }
+        n->batch_cap = batch;
+        n->dev.x0 = n->x0.as<uint16_t>();
+        n->dev.h = n->h.as<uint16_t>();
+        n->dev.dz = n->dz.as<uint16_t>();
+        n->dev.g = n->g.as<uint16_t>();
+        n->dev.partial = n->partial.as<float>();
+    }
+    n->dev.batch = batch;
+    return LMB_OK;
+}
+
+lmb_status nrc_ensure_queries(lmb_nrc n, uint64_t queries) {
+    if (queries > 0xFFFFFFFFull) return fail(LMB_ERR_INVALID_ARGUMENT, "too many cache queries");
+    if (queries <= n->dev.query_cap && n->dev.queries) return LMB_OK;
+    lmb_status st = n->queries.alloc(queries * sizeof(NrcQuery));
+    if (!st) st = n->query_weight.alloc(queries * sizeof(float4));
+    if (st) {
+        n->dev.query_cap = 0;
+        n->dev.queries = nullptr;
+        return st;
+    }
+    n->dev.queries = n->queries.as<NrcQuery>();
+    n->dev.query_weight = n->query_weight.as<float4>();
+    n->dev.query_cap = (uint32_t)queries;
+    return LMB_OK;
+}
+
+static AdamParams next_adam(lmb_nrc n) {
+    n->steps += 1;
+    n->pow1 *= (double)n->opts.beta1;
+    n->pow2 *= (double)n->opts.beta2;
+    AdamParams a;
+    a.lr = n->opts.learning_rate;
+    a.beta1 = n->opts.beta1;
+    a.beta2 = n->opts.beta2;
+    a.eps = n->opts.epsilon;
+    a.bias1 = (float)(1.0 - n->pow1);
+    a.bias2 = (float)(1.0 - n->pow2);
+    return a;
+}
+
+lmb_status nrc_run_steps(lmb_nrc n, const NrcQuery* in, const float4* target,
+                         const uint32_t* count, uint32_t count_limit, uint32_t steps, bool apply,
+                         float* grad_out, hipStream_t stream, KernelTimer* timer) {
+    hipError_t e = launch_fill_zero(n->dev.loss, 64 * sizeof(float), stream);
+    if (e != hipSuccess) return fail_hip(e, "loss clear");
+    for (uint32_t s = 0; s < steps; ++s) {
+        const AdamParams a = apply ? next_adam(n) : AdamParams{};
+        if (timer) timer->begin(kKernelCacheTrain, stream);
+        e = nrc_launch_step(n->dev, in, target, count, count_limit, steps, s, a, apply, grad_out,
+                            nullptr, stream);
+        if (timer) timer->end(kKernelCacheTrain, stream);
+        if (e != hipSuccess) return fail_hip(e, "cache training step");
+    }
+    n->last_steps = steps;
+    return LMB_OK;
+}
+
+}  // namespace lmb
+
+namespace {
+
+lmb_status check(lmb_nrc n) {
+    if (!n) return fail(LMB_ERR_INVALID_ARGUMENT, "cache is NULL");
+    hipError_t e = hipSetDevice(n->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    return LMB_OK;
+}
+
+// Host rows (n x 13) -> device NrcQuery records in n->io_in.
+lmb_status stage_inputs(lmb_nrc n, const float* inputs, uint64_t count) {
+    std::vector<NrcQuery> q(count);
+    for (uint64_t i = 0; i < count; ++i) {
+        std::memset(&q[i], 0, sizeof(NrcQuery));
+        std::memcpy(q[i].in, inputs + i * kNrcInputs, sizeof(float) * kNrcInputs);
+        q[i].pid = (uint32_t)i;
+    }
+    if (n->io_in.bytes < count * sizeof(NrcQuery)) {
+        lmb_status st = n->io_in.alloc(count * sizeof(NrcQuery));
+        if (st) return st;
+    }
+    hipError_t e = hipMemcpy(n->io_in.ptr, q.data(), count * sizeof(NrcQuery), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy inputs");
+    return LMB_OK;
+}
+
+lmb_status stage_targets(lmb_nrc n, const float* targets, uint64_t count) {
+    std::vector<float> t(count * 4, 0.0f);
+    for (uint64_t i = 0; i < count; ++i)
+        for (int k = 0; k < 3; ++k) t[i * 4 + k] = targets[i * 3 + k];
+    if (n->io_target.bytes < count * 16) {
+        lmb_status st = n->io_target.alloc(count * 16);
+        if (st) return st;
+    }
+    hipError_t e = hipMemcpy(n->io_target.ptr, t.data(), count * 16, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy targets");
+    return LMB_OK;
+}
+
+// Device u32 holding `count`, the record count of a host-array call.
+uint32_t* host_count(lmb_nrc n, uint32_t count) {
+    uint32_t* p = n->dev.ctl + kNrcCtlHostCount;
+    if (hipMemcpy(p, &count, 4, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return p;
+}
+
+float read_loss(lmb_nrc n, uint64_t count, uint32_t steps) {
+    float sums[64] = {0};
+    if (hipMemcpy(sums, n->dev.loss, sizeof(float) * std::min<uint32_t>(steps, 64),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+        return NAN;
+    double tot = 0.0;
+    for (uint32_t s = 0; s < std::min<uint32_t>(steps, 64); ++s) tot += sums[s];
+    return count ? (float)(tot / (3.0 * (double)count)) : 0.0f;
+}
+
+lmb_status train_or_grad(lmb_nrc n, const float* inputs, const float* targets, uint64_t count,
+                         bool apply, float* grad, float* loss) {
+    lmb_status st = check(n);
+    if (st) return st;
+    if (count == 0 || !inputs || !targets) return fail(LMB_ERR_INVALID_ARGUMENT, "no samples");
+    if (count > (1u << 24)) return fail(LMB_ERR_INVALID_ARGUMENT, "at most 2^24 samples per call");
+    st = nrc_ensure_batch(n, (uint32_t)count);
+    if (!st) st = stage_inputs(n, inputs, count);
+    if (!st) st = stage_targets(n, targets, count);
+    if (st) return st;
+    uint32_t* cnt = host_count(n, (uint32_t)count);
+    if (!cnt) return fail(LMB_ERR_DEVICE, "count upload failed");
+    float* gdev = nullptr;
+    if (grad) {
+        if (n->io_grad.bytes < kNrcNumWeights * sizeof(float)) {
+            st = n->io_grad.alloc(kNrcNumWeights * sizeof(float));
+            if (st) return st;
+        }
+        gdev = n->io_grad.as<float>();
+    }
+    st = nrc_run_steps(n, n->io_in.as<NrcQuery>(), n->io_target.as<float4>(), cnt, (uint32_t)count,
+                       1, apply, gdev, nullptr, nullptr);
+    if (st) return st;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail_hip(e, "cache step");
+    if (grad) {
+        e = hipMemcpy(grad, gdev, kNrcNumWeights * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return fail_hip(e, "hipMemcpy gradient");
+    }
+    if (loss) *loss = read_loss(n, count, 1);
+    return LMB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void lmb_nrc_opts_default(lmb_nrc_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->learning_rate = 1e-3f;
+    o->beta1 = 0.9f;
+    o->beta2 = 0.99f;
+    o->epsilon = 1e-8f;
+    o->train_paths = 65536;
+    o->train_steps = 4;
+    o->seed = 1;
+}
+
+lmb_status lmb_nrc_create(const lmb_nrc_opts* opts, lmb_nrc* out) {
+    if (!out) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    lmb_nrc_opts o;
+    lmb_nrc_opts_default(&o);
+    if (opts) o = *opts;
+    if (!(o.learning_rate > 0.0f) || !(o.beta1 >= 0.0f && o.beta1 < 1.0f) ||
+        !(o.beta2 >= 0.0f && o.beta2 < 1.0f) || !(o.epsilon >= 0.0f))
+        return fail(LMB_ERR_INVALID_ARGUMENT, "invalid Adam parameters");
+    if (o.train_steps == 0 || o.train_steps > 64)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "train_steps must be in 1..64");
+    if (o.train_paths > (1u << 24))
+        return fail(LMB_ERR_INVALID_ARGUMENT, "train_paths must be <= 2^24");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return fail(LMB_ERR_DEVICE, "no HIP device available");
+    }
+    lmb_nrc n = new lmb_nrc_s();
+    n->opts = o;
+    (void)hipGetDevice(&n->device);
+    const size_t wbytes = kNrcNumWeights * sizeof(float);
+    lmb_status st = n->master.alloc(wbytes);
+    if (!st) st = n->adam_m.alloc(wbytes);
+    if (!st) st = n->adam_v.alloc(wbytes);
+    if (!st) st = n->fwd.alloc(kNrcFwdFrags * kFragElems * 2);
+    if (!st) st = n->bwd.alloc(kNrcBwdFrags * kFragElems * 2);
+    if (!st) st = n->ctl.alloc(kNrcCtlWords * 4);
+    if (!st) st = n->loss.alloc(64 * sizeof(float));
+    // records: train_paths plus a margin for the random selection
+    const uint64_t rec_cap = std::max<uint64_t>(kNrcChunk, (uint64_t)o.train_paths * 5 / 4);
+    if (!st) st = n->rec_in.alloc(rec_cap * sizeof(NrcQuery));
+    if (!st) st = n->rec_aux.alloc(rec_cap * 2 * sizeof(float4));
+    if (!st) st = n->rec_target.alloc(rec_cap * sizeof(float4));
+    if (st) {
+        delete n;
+        return st;
+    }
+    NrcDev& d = n->dev;
+    d.master = n->master.as<float>();
+    d.adam_m = n->adam_m.as<float>();
+    d.adam_v = n->adam_v.as<float>();
+    d.fwd = n->fwd.as<uint16_t>();
+    d.bwd = n->bwd.as<uint16_t>();
+    d.ctl = n->ctl.as<uint32_t>();
+    d.loss = n->loss.as<float>();
+    d.rec_in = n->rec_in.as<NrcQuery>();
+    d.rec_aux = n->rec_aux.as<float4>();
+    d.rec_target = n->rec_target.as<float4>();
+    d.rec_cap = (uint32_t)rec_cap;
+    hipError_t e = hipMemset(n->ctl.ptr, 0, kNrcCtlWords * 4);
+    if (e != hipSuccess) {
+        delete n;
+        return fail_hip(e, "hipMemset");
+    }
+    std::vector<float> w;
+    init_weights(o.seed, w);
+    st = lmb_nrc_set_weights(n, w.data());
+    if (st) {
+        delete n;
+        return st;
+    }
+    *out = n;
+    return LMB_OK;
+}
+
+lmb_status lmb_nrc_destroy(lmb_nrc n) {
+    if (!n) return LMB_OK;
+    (void)hipSetDevice(n->device);
+    (void)hipDeviceSynchronize();
+    delete n;
+    return LMB_OK;
+}
+
+lmb_status lmb_nrc_get_weights(lmb_nrc n, float* w) {
+    lmb_status st = check(n);
+    if (st) return st;
+    if (!w) return fail(LMB_ERR_INVALID_ARGUMENT, "weights is NULL");
+    hipError_t e = hipMemcpy(w, n->dev.master, kNrcNumWeights * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy weights");
+    return LMB_OK;
+}
+
+lmb_status lmb_nrc_set_weights(lmb_nrc n, const float* w) {
+    lmb_status st = check(n);
+    if (st) return st;
+    if (!w) return fail(LMB_ERR_INVALID_ARGUMENT, "weights is NULL");
+    for (uint64_t i = 0; i < kNrcNumWeights; ++i)
+        if (!std::isfinite(w[i]))
+            return fail(LMB_ERR_INVALID_ARGUMENT, "weight " + std::to_string(i) + " is not finite");
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess)
+        e = hipMemcpy(n->dev.master, w, kNrcNumWeights * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(n->dev.adam_m, 0, kNrcNumWeights * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(n->dev.adam_v, 0, kNrcNumWeights * sizeof(float));
+    if (e == hipSuccess) e = nrc_launch_pack(n->dev, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail_hip(e, "cache weight upload");
+    n->steps = 0;
+    n->pow1 = n->pow2 = 1.0;
+    return LMB_OK;
+}
+
+lmb_status lmb_nrc_infer(lmb_nrc n, const float* inputs, uint64_t count, float* outputs) {
+    lmb_status st = check(n);
+    if (st) return st;
+    if (count == 0) return LMB_OK;
+    if (!inputs || !outputs) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (count > 0x7FFFFFFFull) return fail(LMB_ERR_INVALID_ARGUMENT, "too many rows");
+    st = stage_inputs(n, inputs, count);
+    if (st) return st;
+    if (n->io_out.bytes < count * 16) {
+        st = n->io_out.alloc(count * 16);
+        if (st) return st;
+    }
+    hipError_t e = nrc_launch_infer(n->dev, n->io_in.as<NrcQuery>(), (uint32_t)count,
+                                    n->io_out.as<float4>(), nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail_hip(e, "cache inference");
+    std::vector<float> o(count * 4);
+    e = hipMemcpy(o.data(), n->io_out.ptr, count * 16, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy outputs");
+    for (uint64_t i = 0; i < count; ++i)
+        for (int k = 0; k < 3; ++k) outputs[i * 3 + k] = o[i * 4 + k];
+    return LMB_OK;
+}
+
+lmb_status lmb_nrc_gradient(lmb_nrc n, const float* inputs, const float* targets, uint64_t count,
+                            float* grad, float* loss) {
+    if (!grad) return fail(LMB_ERR_INVALID_ARGUMENT, "grad is NULL");
+    return train_or_grad(n, inputs, targets, count, false, grad, loss);
+}
+
+lmb_status lmb_nrc_train(lmb_nrc n, const float* inputs, const float* targets, uint64_t count,
+                         float* loss) {
+    return train_or_grad(n, inputs, targets, count, true, nullptr, loss);
+}
+
+lmb_status lmb_nrc_apply_gradient(lmb_nrc n, const float* grad) {
+    lmb_status st = check(n);
+    if (st) return st;
+    if (!grad) return fail(LMB_ERR_INVALID_ARGUMENT, "grad is NULL");
+    if (n->io_grad.bytes < kNrcNumWeights * sizeof(float)) {
+        st = n->io_grad.alloc(kNrcNumWeights * sizeof(float));
+        if (st) return st;
+    }
+    hipError_t e = hipMemcpy(n->io_grad.ptr, grad, kNrcNumWeights * sizeof(float),
+                             hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy gradient");
+    const AdamParams a = next_adam(n);
+    e = nrc_launch_step(n->dev, nullptr, nullptr, nullptr, 0, 1, 0, a, true, nullptr,
+                        n->io_grad.as<float>(), nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail_hip(e, "cache Adam step");
+    return LMB_OK;
+}
+
+lmb_status lmb_nrc_stats_get(lmb_nrc n, lmb_nrc_stats* s) {
+    lmb_status st = check(n);
+    if (st) return st;
+    if (!s) return fail(LMB_ERR_INVALID_ARGUMENT, "stats is NULL");
+    hipError_t e = hipDeviceSynchronize();
+    uint32_t ctl[kNrcCtlWords] = {0};
+    if (e == hipSuccess) e = hipMemcpy(ctl, n->dev.ctl, sizeof ctl, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy cache counters");
+    std::memset(s, 0, sizeof *s);
+    s->steps = n->steps;
+    s->records = std::min<uint32_t>(ctl[kNrcCtlRecords], n->dev.rec_cap);
+    s->queries = ctl[kNrcCtlQueries];
+    s->loss = read_loss(n, s->records, n->last_steps);
+    return LMB_OK;
+}
+
+}  // extern "C"

@@ -346,4 +346,60 @@ __device__ __forceinline__ float pick_ref_idx(float* picked, float low, float hi
     }
 }
 
+// general.wesl:714-901 with every guiding candidate invalid ("plain"): the
+// Fresnel choice, the refractive-index pick, and diffuse / metal / dielectric
+// scattering.  Advances `seed` exactly as the reference does, multiplies
+// `color` by the sampled weight and returns the continuation direction.  The
+// caller accumulates the vertex emission first (general.wesl:714).
+template <uint32_t Q>
+__device__ __forceinline__ V3 surface_scatter(const Isect& it, V3 dir, uint32_t& seed, V3& color,
+                                              float& picked) {
+    const V3 assign_normal = get_normal<Q>(dir, it.normal, it.tangent, seed, it.rough_x, it.rough_y);
+    const V3 transparent_normal =
+        get_normal<Q>(dir, it.normal, it.tangent, seed, it.rough_tx, it.rough_ty);
+    V3 normal = transparent_normal;
+    bool should_reflect = false;
+    seed += 1u;
+    V3 ccol;
+    float ri = pick_ref_idx<Q>(&picked, it.ior_low, it.ior_high, seed, &ccol);
+    seed += 1u;
+    if (!(!it.front_face && it.ty == kMatTransparent)) ri = 1.0f / ri;
+    color = color * ccol;
+    if (ri != 1.0f) {
+        const float cos_theta = fminf(dot3(-dir, normal), 1.0f);
+        const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+        const bool cannot_refract = ri * sin_theta > 1.0f;
+        const float pdf = reflectance(cos_theta, ri);
+        const float rnd = rand_f32(seed);
+        if (cannot_refract || pdf > rnd) should_reflect = true;
+        seed += 1u;
+    }
+    if (!should_reflect || it.ty == kMatTransparent) color = color * it.color;
+    if (!should_reflect) {
+        normal = assign_normal;
+        seed += 1u;
+    }
+    const bool reflecting = should_reflect || it.ty == kMatMetallic;
+    V3 ndir = dir;
+    if (!reflecting) {
+        if (it.ty == kMatDiffuse) {
+            const V3 dir_cos = cosine_weighted_hemisphere(seed, it.normal, it.tangent);
+            const float cos_pdf = fmaxf(dot3(dir_cos, it.normal) / kPi, 0.0f);
+            seed += 3u;  // theta, xi and the lobe choice
+            const float avg_pdf = 1.0f * cos_pdf;
+            const V3 diffuse = make_diffuse<Q>(it.color, dir, it.normal, it.tangent, it.rough_x, dir_cos);
+            const float cached_brdf = fmaxf(dot3(dir_cos, it.normal) / kPi, 0.0f);
+            const float nf = fabsf(cached_brdf) / fmaxf(avg_pdf, 0.001f);
+            color = (color * nf) * diffuse;
+            ndir = dir_cos;
+        } else if (it.ty == kMatTransparent) {
+            ndir = refract3(dir, normal, ri);
+        }
+    } else {
+        ndir = reflect3(dir, normal);
+        if (dot3(ndir, it.normal) < 0.0f) ndir = reflect3(ndir, it.normal);
+    }
+    return ndir;
+}
+
 }  // namespace lmb

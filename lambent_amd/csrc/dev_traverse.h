@@ -312,6 +312,11 @@ __device__ __forceinline__ void traverse_queue(const Bvh8Node* __restrict__ node
                                                uint32_t* fetch, ShortStack<BLOCK>& st,
                                                Source& src) {
     const uint32_t lane = __lane_id();
+    // A short queue is spread over every wave of the grid (a few rays each)
+    // instead of filling the first waves: the traversal of a short queue is
+    // latency-bound, and a wave lasts as long as its longest ray.
+    const uint64_t waves = (uint64_t)gridDim.x * BLOCK / 64u;
+    const uint32_t quota = (uint32_t)min<uint64_t>(64u, max<uint64_t>(1u, (count + waves - 1) / waves));
     Traversal tv;
     tv.active = false;
     uint64_t slot = 0;
@@ -320,15 +325,16 @@ __device__ __forceinline__ void traverse_queue(const Bvh8Node* __restrict__ node
         if (!exhausted) {
             const uint64_t idle = __ballot(!tv.active);
             const uint32_t n_idle = (uint32_t)__popcll(idle);
-            if (n_idle >= (uint32_t)REFILL) {
+            if (n_idle >= min((uint32_t)REFILL, quota)) {
+                const uint32_t take = min(n_idle, quota);
                 const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
                 uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(fetch, n_idle);
+                if (lane == leader) base = atomicAdd(fetch, take);
                 base = __shfl(base, (int)leader);
-                if ((uint64_t)base + n_idle >= count) exhausted = true;
-                if (!tv.active) {
-                    const uint64_t mine =
-                        (uint64_t)base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if ((uint64_t)base + take >= count) exhausted = true;
+                const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (!tv.active && rank < take) {
+                    const uint64_t mine = (uint64_t)base + rank;
                     if (mine < count) {
                         slot = mine;
                         tv.init(src.load(slot), num_nodes);

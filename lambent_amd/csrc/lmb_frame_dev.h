@@ -68,12 +68,28 @@ struct TraceParams {
 uint64_t frame_threads(uint64_t npix);
 
 // Optional per-launch timing hooks (HIP events recorded on the launch stream).
-enum KernelKind { kKernelExtend = 0, kKernelShade = 1, kKernelOther = 2, kNumKernelKinds = 3 };
+enum KernelKind {
+    kKernelExtend = 0,
+    kKernelShade = 1,
+    kKernelOther = 2,
+    kKernelCacheInfer = 3,
+    kKernelCacheTrain = 4,
+    kNumKernelKinds = 5
+};
 struct KernelTimer {
     virtual void begin(int kind, hipStream_t stream) = 0;
     virtual void end(int kind, hipStream_t stream) = 0;
     virtual ~KernelTimer() = default;
 };
+
+// Single wavefront stages of the plain tracer, shared by the other modes:
+// per-pixel seeds and primary rays of `sample` into queue 0, closest hits of
+// the bounce-`bounce` queue, and colour = max(pix_sum / samples, 0).
+hipError_t launch_generate(const CameraDev& cam, const FrameDev& f, const TraceParams& p,
+                           uint32_t sample, hipStream_t stream);
+hipError_t launch_extend(const SceneDev& s, const FrameDev& f, const TraceParams& p,
+                         uint32_t bounce, hipStream_t stream);
+hipError_t launch_resolve(const FrameDev& f, const TraceParams& p, hipStream_t stream);
 
 // One full trace (all samples, all bounces) on `stream`.
 hipError_t launch_trace(const SceneDev& s, const CameraDev& cam, const FrameDev& f,

@@ -51,9 +51,6 @@ struct GuidedDev {
 // Number of float4 / 4-byte arrays above, for allocation.
 constexpr int kGuidedFloat4PerPixel = 24;
 
-hipError_t launch_extend(const SceneDev& s, const FrameDev& f, const TraceParams& p,
-                         uint32_t bounce, hipStream_t stream);
-
 hipError_t launch_guided(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
                          const GuidedDev& g, const TraceParams& p, hipStream_t stream,
                          KernelTimer* timer);

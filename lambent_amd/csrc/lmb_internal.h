@@ -8,7 +8,9 @@
 
 #include "../../include/lambent.h"
 #include "bvh_build.h"
+#include "lmb_frame_dev.h"
 #include "lmb_layout.h"
+#include "lmb_nrc.h"
 #include "lmb_scene_dev.h"
 
 namespace lmb {
@@ -64,10 +66,38 @@ struct lmb_scene_s {
     lmb::SpillBuffer spill;
     lmb_bvh_stats stats{};
     uint64_t num_prims = 0;
+    float bounds_lo[3] = {0.f, 0.f, 0.f};  // box of the vertices
+    float bounds_hi[3] = {0.f, 0.f, 0.f};
+};
+
+struct lmb_nrc_s {
+    int device = 0;
+    lmb_nrc_opts opts{};
+    lmb::NrcDev dev{};
+    lmb::DeviceBuffer master, adam_m, adam_v, fwd, bwd;
+    lmb::DeviceBuffer x0, h, dz, g, partial;  // training scratch for batch_cap samples
+    uint32_t batch_cap = 0;
+    lmb::DeviceBuffer ctl, loss;
+    lmb::DeviceBuffer queries, query_weight;  // tracer queries (query_cap)
+    lmb::DeviceBuffer rec_in, rec_aux, rec_target;
+    lmb::DeviceBuffer io_in, io_target, io_out, io_grad;  // staging of the host-array calls
+    uint64_t steps = 0;                  // Adam steps taken (t)
+    double pow1 = 1.0, pow2 = 1.0;       // beta1^t, beta2^t
+    uint32_t last_steps = 0;             // optimiser steps of the last call
 };
 
 namespace lmb {
 lmb_status scene_upload(lmb_scene s, const lmb_scene_desc* desc, uint64_t ntri,
                         const BvhBuildResult& bvh);
 void scene_free(lmb_scene s);
+
+// Neural cache helpers (api_nrc.cpp).
+lmb_status nrc_ensure_batch(lmb_nrc n, uint32_t batch);
+lmb_status nrc_ensure_queries(lmb_nrc n, uint64_t queries);
+// `steps` optimiser steps over the min(*count, count_limit) records of
+// in / target (*count is a device word), record i * steps + s in step s.
+// Zeroes the per-step loss sums first.
+lmb_status nrc_run_steps(lmb_nrc n, const NrcQuery* in, const float4* target,
+                         const uint32_t* count, uint32_t count_limit, uint32_t steps, bool apply,
+                         float* grad_out, hipStream_t stream, KernelTimer* timer);
 }  // namespace lmb

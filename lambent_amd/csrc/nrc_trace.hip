@@ -1,0 +1,259 @@
+// nrc_trace.hip - the path tracer with the neural radiance cache.
+//
+// Per sample: generate -> extend / shade for every bounce -> inference.
+// Shading is the plain tracer's (surface_scatter, Russian roulette) except at
+// the second vertex (bounce 1):
+//   * about train_expect of the paths reaching it (a hash of path id, frame
+//     and sample picks them) become training records: the record keeps the
+//     network inputs there, the throughput T and the radiance S gathered so
+//     far; the path restarts its throughput at 1 and its sum at 0 and is
+//     traced on to full depth; when it ends, its sum (the radiance scattered
+//     at the vertex towards the previous one) is the record's target and the
+//     pixel receives S + T * sum;
+//   * every other path ends there with a query: the pixel receives S now and
+//     T * max(cache(inputs), 0) from the inference kernel.
+// The emission of the second vertex belongs to S (the cache predicts
+// scattered radiance only).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "dev_common.h"
+#include "dev_shade.h"
+#include "lmb_frame_dev.h"
+#include "lmb_nrc.h"
+
+namespace lmb {
+
+constexpr int kNrcTraceBlock = 256;
+constexpr uint32_t kCacheBounce = 1;  // the second path vertex
+
+// Wave-level compaction: every lane with `pred` gets a distinct index from
+// `*counter`; one atomic per wave.  Called by all lanes of the wave.
+__device__ __forceinline__ uint32_t wave_reserve(bool pred, uint32_t* counter, uint32_t lane) {
+    const uint64_t mask = __ballot(pred);
+    if (!mask) return 0;
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ float train_draw(uint32_t pid, uint32_t seed_offset, uint32_t sample) {
+    const uint32_t salt = rand_u32(seed_offset * 0x9E3779B1u + sample * 0x85EBCA77u + 0x165667B1u);
+    return (float)(rand_u32(pid ^ salt) >> 8) * (1.0f / 16777216.0f);
+}
+
+template <uint32_t Q>
+__global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameDev f, TraceParams p,
+                                                              NrcDev d, NrcTraceParams np,
+                                                              uint32_t bounce) {
+    const uint64_t gtid = (uint64_t)blockIdx.x * kNrcTraceBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kNrcTraceBlock;
+    const uint32_t count = f.counters[bounce];
+    const uint64_t count_r = ((uint64_t)count + 63) & ~63ull;
+    const float4* __restrict__ qin = f.queue[bounce & 1];
+    float4* __restrict__ qout = f.queue[(bounce + 1) & 1];
+    const uint32_t lane = __lane_id();
+    if (bounce == 0 && gtid == 0) {
+        d.ctl[kNrcCtlQueries] = 0;
+        const uint32_t hits = d.ctl[kNrcCtlHits];
+        if (hits) d.ctl[kNrcCtlHitsPrev] = hits;
+        d.ctl[kNrcCtlHits] = 0;
+    }
+    float prob = 0.0f;
+    if (bounce == kCacheBounce) {
+        const uint32_t prev = d.ctl[kNrcCtlHitsPrev];
+        const uint32_t ref = prev ? prev : max(count, 1u);
+        prob = fminf(np.train_expect / (float)ref, 1.0f);
+    }
+    for (uint64_t slot = gtid; slot < count_r; slot += stride) {
+        // ---- phase A: hit record and the cache-vertex inputs
+        const bool active = slot < count;
+        float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4;
+        uint4 h = make_uint4(kMissPrim, 0u, 0u, 0u);
+        uint32_t pid = 0, seed = 0, rslot = 0;
+        V3 color = v3s(0.f), scol = v3s(0.f);
+        float picked = -1.0f;
+        if (active) {
+            o4 = qin[2 * slot];
+            d4 = qin[2 * slot + 1];
+            pid = __float_as_uint(o4.w);
+            h = f.hits[slot];
+            seed = f.ps_seed[pid];
+            const float4 th = f.ps_through[pid];
+            const float4 ac = f.ps_accum[pid];
+            color = v3(th.x, th.y, th.z);
+            picked = th.w;
+            scol = v3(ac.x, ac.y, ac.z);
+            rslot = __float_as_uint(ac.w);
+        }
+        const bool hit = active && h.x != kMissPrim;
+        const V3 dir = v3(d4.x, d4.y, d4.z);
+        const V3 org = v3(o4.x, o4.y, o4.z);
+        Isect it{};
+        V3 new_pos = v3s(0.f);
+        bool at_cache = false, cand = false;
+        float in[kNrcInputs];
+        if (hit) {
+            const float t = __uint_as_float(h.y);
+            it = make_isect(s, h.x, t, __uint_as_float(h.z), __uint_as_float(h.w), dir);
+            new_pos = fma3(dir, v3s(t), org);
+            if (bounce == 0) {
+                f.out_normal[pid] = make_float4(it.normal.x, it.normal.y, it.normal.z, 1.f);
+                f.out_albedo[pid] = make_float4(it.color.x, it.color.y, it.color.z, 1.f);
+            }
+            scol = fma3(color, it.emission, scol);  // general.wesl:714
+            if (bounce == kCacheBounce) {
+                at_cache = true;
+                in[0] = (new_pos.x - np.lo[0]) * np.inv_ext[0];
+                in[1] = (new_pos.y - np.lo[1]) * np.inv_ext[1];
+                in[2] = (new_pos.z - np.lo[2]) * np.inv_ext[2];
+                in[3] = 0.5f - 0.5f * dir.x;  // towards the previous vertex
+                in[4] = 0.5f - 0.5f * dir.y;
+                in[5] = 0.5f - 0.5f * dir.z;
+                in[6] = 0.5f + 0.5f * it.normal.x;
+                in[7] = 0.5f + 0.5f * it.normal.y;
+                in[8] = 0.5f + 0.5f * it.normal.z;
+                in[9] = saturate(it.rough_x);
+                in[10] = saturate(it.color.x);
+                in[11] = saturate(it.color.y);
+                in[12] = saturate(it.color.z);
+                cand = train_draw(pid, p.seed_offset, np.sample) < prob;
+            }
+        }
+        // ---- phase B (whole wave): training-record slots
+        const uint32_t r = wave_reserve(cand, d.ctl + kNrcCtlRecords, lane);
+        if (cand && r >= d.rec_cap) cand = false;
+        const bool is_query = at_cache && !cand;
+        // ---- phase C: scattering, continuation or the end of the path
+        bool cont = false;
+        float4 out_o = make_float4(0.f, 0.f, 0.f, 0.f), out_d = out_o;
+        bool finish = false;
+        if (active) {
+            if (hit) {
+                if (cand) {
+                    NrcQuery* rq = d.rec_in + r;
+#pragma unroll
+                    for (int k = 0; k < kNrcInputs; ++k) rq->in[k] = in[k];
+                    rq->pid = pid;
+                    d.rec_aux[2 * (uint64_t)r] = make_float4(color.x, color.y, color.z, scol.x);
+                    d.rec_aux[2 * (uint64_t)r + 1] = make_float4(scol.y, scol.z, 0.f, 0.f);
+                    rslot = r + 1u;
+                    color = v3s(1.0f);
+                    scol = v3s(0.0f);
+                }
+                if (is_query) {
+                    const float4 ps = f.pix_sum[pid];
+                    f.pix_sum[pid] = make_float4(ps.x + scol.x, ps.y + scol.y, ps.z + scol.z, 0.f);
+                } else {
+                    const V3 ndir = surface_scatter<Q>(it, dir, seed, color, picked);
+                    bool alive = bounce + 1 < p.bounces;
+                    if (alive && bounce >= p.rr_start) {
+                        const float q = fminf(fmaxf(fmaxf(color.x, color.y), color.z), 1.0f);
+                        const float rnd = rand_f32(seed);
+                        seed += 1u;
+                        if (rnd >= q)
+                            alive = false;
+                        else
+                            color = color / q;
+                    }
+                    if (alive) {
+                        cont = true;
+                        out_o = make_float4(new_pos.x, new_pos.y, new_pos.z, __uint_as_float(pid));
+                        out_d = make_float4(ndir.x, ndir.y, ndir.z, 0.f);
+                        f.ps_seed[pid] = seed;
+                        f.ps_through[pid] = make_float4(color.x, color.y, color.z, picked);
+                        f.ps_accum[pid] = make_float4(scol.x, scol.y, scol.z, __uint_as_float(rslot));
+                    } else {
+                        finish = true;
+                    }
+                }
+            } else {
+                if (bounce == 0) {
+                    f.out_normal[pid] = make_float4(0.f, 0.f, 0.f, 1.f);
+                    f.out_albedo[pid] = make_float4(0.f, 0.f, 0.f, 1.f);
+                }
+                scol = fma3(color, background(s, dir), scol);  // general.wesl:915-938
+                finish = true;
+            }
+            if (finish) {
+                V3 pix = scol;
+                if (rslot) {
+                    const uint64_t rr = rslot - 1u;
+                    const float4 a0 = d.rec_aux[2 * rr], a1 = d.rec_aux[2 * rr + 1];
+                    d.rec_target[rr] = make_float4(scol.x, scol.y, scol.z, 0.f);
+                    pix = v3(fmaf(a0.x, scol.x, a0.w), fmaf(a0.y, scol.y, a1.x),
+                             fmaf(a0.z, scol.z, a1.y));
+                }
+                const float4 ps = f.pix_sum[pid];
+                f.pix_sum[pid] = make_float4(ps.x + pix.x, ps.y + pix.y, ps.z + pix.z, 0.f);
+            }
+        }
+        // ---- phase D (whole wave): queries, hit count, continuation queue
+        const uint32_t qi = wave_reserve(is_query, d.ctl + kNrcCtlQueries, lane);
+        if (is_query) {
+            NrcQuery* qq = d.queries + qi;
+#pragma unroll
+            for (int k = 0; k < kNrcInputs; ++k) qq->in[k] = in[k];
+            qq->pid = pid;
+            d.query_weight[qi] = make_float4(color.x, color.y, color.z, 0.f);
+        }
+        const uint64_t at_mask = __ballot(at_cache);
+        if (at_mask && lane == (uint32_t)__ffsll((unsigned long long)at_mask) - 1u)
+            atomicAdd(d.ctl + kNrcCtlHits, (uint32_t)__popcll(at_mask));
+        const uint32_t ci = wave_reserve(cont, f.counters + bounce + 1, lane);
+        if (cont) {
+            qout[2 * (uint64_t)ci] = out_o;
+            qout[2 * (uint64_t)ci + 1] = out_d;
+        }
+    }
+}
+
+template <uint32_t Q>
+static hipError_t launch_nrc_q(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
+                               const TraceParams& p, const NrcDev& d, NrcTraceParams np,
+                               hipStream_t stream, KernelTimer* tm) {
+    const dim3 grid((uint32_t)(frame_threads(f.npix) / kNrcTraceBlock)), block(kNrcTraceBlock);
+    hipError_t e;
+    for (uint32_t sample = 0; sample < p.samples; ++sample) {
+        np.sample = sample;
+        e = launch_generate(cam, f, p, sample, stream);
+        if (e != hipSuccess) return e;
+        for (uint32_t b = 0; b < p.bounces; ++b) {
+            if (tm) tm->begin(kKernelExtend, stream);
+            e = launch_extend(s, f, p, b, stream);
+            if (tm) tm->end(kKernelExtend, stream);
+            if (e != hipSuccess) return e;
+            if (tm) tm->begin(kKernelShade, stream);
+            hipLaunchKernelGGL(k_shade_nrc<Q>, grid, block, 0, stream, s, f, p, d, np, b);
+            if (tm) tm->end(kKernelShade, stream);
+            e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        if (p.bounces > kCacheBounce) {
+            if (tm) tm->begin(kKernelCacheInfer, stream);
+            e = nrc_launch_infer_accumulate(d, f.pix_sum, stream);
+            if (tm) tm->end(kKernelCacheInfer, stream);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return launch_resolve(f, p, stream);
+}
+
+hipError_t launch_trace_nrc(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
+                            const TraceParams& p, const NrcDev& d, NrcTraceParams np,
+                            hipStream_t stream, KernelTimer* timer) {
+    if (f.npix == 0) return hipSuccess;
+    switch (p.quality) {
+        case kQualityLow:
+            return launch_nrc_q<kQualityLow>(s, cam, f, p, d, np, stream, timer);
+        case kQualityHigh:
+            return launch_nrc_q<kQualityHigh>(s, cam, f, p, d, np, stream, timer);
+        default:
+            return launch_nrc_q<kQualityMedium>(s, cam, f, p, d, np, stream, timer);
+    }
+}
+
+}  // namespace lmb

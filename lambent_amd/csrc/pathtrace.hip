@@ -153,56 +153,9 @@ __global__ __launch_bounds__(kPtBlock) void k_shade_plain(SceneDev s, FrameDev f
                     f.out_normal[pid] = make_float4(it.normal.x, it.normal.y, it.normal.z, 1.f);
                     f.out_albedo[pid] = make_float4(it.color.x, it.color.y, it.color.z, 1.f);
                 }
-                // general.wesl:714-762
+                // general.wesl:714-901
                 scol = fma3(color, it.emission, scol);
-                const V3 assign_normal =
-                    get_normal<Q>(dir, it.normal, it.tangent, seed, it.rough_x, it.rough_y);
-                const V3 transparent_normal =
-                    get_normal<Q>(dir, it.normal, it.tangent, seed, it.rough_tx, it.rough_ty);
-                V3 normal = transparent_normal;
-                bool should_reflect = false;
-                seed += 1u;
-                V3 ccol;
-                float ri = pick_ref_idx<Q>(&picked, it.ior_low, it.ior_high, seed, &ccol);
-                seed += 1u;
-                if (!(!it.front_face && it.ty == kMatTransparent)) ri = 1.0f / ri;
-                color = color * ccol;
-                if (ri != 1.0f) {
-                    const float cos_theta = fminf(dot3(-dir, normal), 1.0f);
-                    const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
-                    const bool cannot_refract = ri * sin_theta > 1.0f;
-                    const float pdf = reflectance(cos_theta, ri);
-                    const float rnd = rand_f32(seed);
-                    if (cannot_refract || pdf > rnd) should_reflect = true;
-                    seed += 1u;
-                }
-                if (!should_reflect || it.ty == kMatTransparent) color = color * it.color;
-                if (!should_reflect) {
-                    normal = assign_normal;
-                    seed += 1u;
-                }
-                // general.wesl:771-901 with every guiding candidate invalid
-                const bool reflecting = should_reflect || it.ty == kMatMetallic;
-                V3 ndir = dir;
-                if (!reflecting) {
-                    if (it.ty == kMatDiffuse) {
-                        const V3 dir_cos = cosine_weighted_hemisphere(seed, it.normal, it.tangent);
-                        const float cos_pdf = fmaxf(dot3(dir_cos, it.normal) / kPi, 0.0f);
-                        seed += 3u;  // theta, xi and the lobe choice
-                        const float avg_pdf = 1.0f * cos_pdf;
-                        const V3 diffuse =
-                            make_diffuse<Q>(it.color, dir, it.normal, it.tangent, it.rough_x, dir_cos);
-                        const float cached_brdf = fmaxf(dot3(dir_cos, it.normal) / kPi, 0.0f);
-                        const float nf = fabsf(cached_brdf) / fmaxf(avg_pdf, 0.001f);
-                        color = (color * nf) * diffuse;
-                        ndir = dir_cos;
-                    } else if (it.ty == kMatTransparent) {
-                        ndir = refract3(dir, normal, ri);
-                    }
-                } else {
-                    ndir = reflect3(dir, normal);
-                    if (dot3(ndir, it.normal) < 0.0f) ndir = reflect3(ndir, it.normal);
-                }
+                const V3 ndir = surface_scatter<Q>(it, dir, seed, color, picked);
                 bool alive = bounce + 1 < p.bounces;
                 if (alive && bounce >= p.rr_start) {
                     const float q = fminf(fmaxf(fmaxf(color.x, color.y), color.z), 1.0f);
@@ -267,6 +220,19 @@ hipError_t launch_extend(const SceneDev& s, const FrameDev& f, const TraceParams
                          uint32_t bounce, hipStream_t stream) {
     const dim3 grid((uint32_t)(frame_threads(f.npix) / kPtBlock)), block(kPtBlock);
     hipLaunchKernelGGL(k_extend, grid, block, 0, stream, s, f, p, bounce);
+    return hipGetLastError();
+}
+
+hipError_t launch_generate(const CameraDev& cam, const FrameDev& f, const TraceParams& p,
+                           uint32_t sample, hipStream_t stream) {
+    const dim3 grid((uint32_t)(frame_threads(f.npix) / kPtBlock)), block(kPtBlock);
+    hipLaunchKernelGGL(k_generate, grid, block, 0, stream, cam, f, p, sample);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(const FrameDev& f, const TraceParams& p, hipStream_t stream) {
+    const dim3 grid((uint32_t)(frame_threads(f.npix) / kPtBlock)), block(kPtBlock);
+    hipLaunchKernelGGL(k_resolve, grid, block, 0, stream, f, p);
     return hipGetLastError();
 }
 

@@ -89,6 +89,15 @@ lmb_status scene_upload(lmb_scene s, const lmb_scene_desc* desc, uint64_t ntri,
     if (e != hipSuccess) return fail_hip(e, "hipGetDevice");
     s->device = dev;
     s->num_prims = ntri;
+    if (desc->num_vertices > 0) {
+        for (int k = 0; k < 3; ++k) s->bounds_lo[k] = s->bounds_hi[k] = desc->vertices[k];
+        for (uint64_t v = 0; v < desc->num_vertices; ++v)
+            for (int k = 0; k < 3; ++k) {
+                const float c = desc->vertices[4 * v + k];  // vec4 per vertex
+                s->bounds_lo[k] = c < s->bounds_lo[k] ? c : s->bounds_lo[k];
+                s->bounds_hi[k] = c > s->bounds_hi[k] ? c : s->bounds_hi[k];
+            }
+    }
 
     lmb_status st = s->nodes.upload(bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh8Node));
     if (st) return st;

@@ -356,3 +356,54 @@ def trace_guided(scene: "Scene", camera_bytes: bytes, width: int, height: int, s
                               state.gi.ctypes.data_as(_vp), len(state.gi), _f(state.info), _f(c),
                               _f(nrm), _f(alb))
     return c, nrm, alb, v
+
+
+# ---------------------------------------------------------------- neural cache
+_f64p = ctypes.POINTER(ctypes.c_double)
+_sig("orc_nrc_num_weights", [], ctypes.c_int)
+_sig("orc_nrc_encode", [_f32p, ctypes.c_int, _f64p])
+_sig("orc_nrc_forward", [_f64p, _f32p, ctypes.c_uint64, ctypes.c_int, _f64p])
+_sig("orc_nrc_grad", [_f64p, _f32p, _f32p, ctypes.c_uint64, ctypes.c_int, _f64p, _f64p])
+_sig("orc_nrc_adam", [_f32p, _f32p, _f32p, _f32p, ctypes.c_uint64, ctypes.c_float,
+                      ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                      ctypes.c_float])
+
+NRC_NUM_WEIGHTS = _lib.orc_nrc_num_weights()
+NRC_EXACT, NRC_BF16 = 0, 1
+
+
+def _d(a):
+    return a.ctypes.data_as(_f64p)
+
+
+def nrc_encode(x, mode=NRC_EXACT) -> np.ndarray:
+    """320 encoded features of one 13-input row."""
+    xv = np.ascontiguousarray(x, np.float32)
+    out = np.empty(320, np.float64)
+    _lib.orc_nrc_encode(_f(xv), mode, _d(out))
+    return out
+
+
+def nrc_forward(weights, inputs, mode=NRC_EXACT) -> np.ndarray:
+    w = np.ascontiguousarray(weights, np.float64)
+    x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 13)
+    out = np.empty((len(x), 3), np.float64)
+    _lib.orc_nrc_forward(_d(w), _f(x), len(x), mode, _d(out))
+    return out
+
+
+def nrc_grad(weights, inputs, targets, mode=NRC_EXACT):
+    """(gradient of the mean relative L2 loss, loss)."""
+    w = np.ascontiguousarray(weights, np.float64)
+    x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 13)
+    t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+    g = np.empty(NRC_NUM_WEIGHTS, np.float64)
+    loss = np.zeros(1, np.float64)
+    _lib.orc_nrc_grad(_d(w), _f(x), _f(t), len(x), mode, _d(g), _d(loss))
+    return g, float(loss[0])
+
+
+def nrc_adam(w, m, v, grad, lr, beta1, beta2, eps, bias1, bias2):
+    """In-place Adam step on float32 arrays (the GPU's float arithmetic)."""
+    g = np.ascontiguousarray(grad, np.float32)
+    _lib.orc_nrc_adam(_f(w), _f(m), _f(v), _f(g), len(w), lr, beta1, beta2, eps, bias1, bias2)

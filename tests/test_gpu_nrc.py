@@ -1,0 +1,176 @@
+"""Neural radiance cache on the GPU against the CPU oracle.
+
+Inference and the loss gradient are compared with the oracle's bf16 mode,
+which rounds at the same points as the kernels (weights, encoded features,
+hidden activations, gradients); the kernels accumulate in fp32 in matrix-core
+order, so agreement is within stated tolerances, not bitwise.  The Adam step
+is float arithmetic in the same order on both sides and must agree exactly.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from lambent_amd import api, scenes
+
+pytestmark = pytest.mark.gpu
+
+NW = api.NRC_NUM_WEIGHTS
+WH, W5 = 320 * 64, 320 * 64 + 4 * 64 * 64
+LAYERS = [(0, WH)] + [(WH + l * 4096, WH + (l + 1) * 4096) for l in range(4)] + [(W5, NW)]
+
+
+def he_weights(seed):
+    rng = np.random.default_rng(seed)
+    w = np.zeros(NW, np.float64)
+    a0 = np.sqrt(6.0 / 313)
+    w0 = rng.uniform(-a0, a0, (320, 64))
+    w0[313:] = 0.0
+    w[:WH] = w0.ravel()
+    w[WH:] = rng.uniform(-np.sqrt(6.0 / 64), np.sqrt(6.0 / 64), NW - WH)
+    return w.astype(np.float32)
+
+
+def test_weights_roundtrip_and_default_init():
+    cache = api.NeuralCache()
+    w0 = cache.weights()
+    assert np.isfinite(w0).all() and np.abs(w0).max() > 0
+    assert (w0[313 * 64:320 * 64] == 0).all()
+    w = he_weights(1)
+    cache.set_weights(w)
+    np.testing.assert_array_equal(cache.weights(), w)
+
+
+def test_infer_matches_oracle():
+    cache = api.NeuralCache()
+    w = he_weights(3)
+    cache.set_weights(w)
+    x = np.random.default_rng(4).random((2000, 13)).astype(np.float32)
+    y = cache.infer(x)
+    ref = oracle.nrc_forward(w.astype(np.float64), x, oracle.NRC_BF16)
+    scale = max(1.0, float(np.abs(ref).max()))
+    err = np.abs(y - ref)
+    assert err.max() < 2e-2 * scale, err.max()
+    assert err.mean() < 2e-3 * scale, err.mean()
+    exact = oracle.nrc_forward(w.astype(np.float64), x, oracle.NRC_EXACT)
+    assert np.abs(y - exact).max() < 5e-2 * scale
+
+
+@pytest.mark.parametrize("n", [1, 31, 33, 64, 65, 257])
+def test_infer_ragged_sizes_are_independent_of_batch(n):
+    cache = api.NeuralCache()
+    cache.set_weights(he_weights(5))
+    x = np.random.default_rng(n).random((300, 13)).astype(np.float32)
+    full = cache.infer(x)
+    part = cache.infer(x[:n])
+    np.testing.assert_array_equal(part, full[:n])  # per-sample arithmetic, no batch coupling
+
+
+def test_gradient_matches_oracle():
+    cache = api.NeuralCache()
+    w = he_weights(6)
+    cache.set_weights(w)
+    rng = np.random.default_rng(7)
+    n = 700  # not a multiple of the 256-sample chunk
+    x = rng.random((n, 13)).astype(np.float32)
+    t = (rng.random((n, 3)) * 2).astype(np.float32)
+    g, loss = cache.gradient(x, t)
+    ref, ref_loss = oracle.nrc_grad(w.astype(np.float64), x, t, oracle.NRC_BF16)
+    assert abs(loss - ref_loss) < 1e-3 * ref_loss + 1e-6, (loss, ref_loss)
+    for a, b in LAYERS:
+        ga, ra = g[a:b].astype(np.float64), ref[a:b]
+        rel = np.linalg.norm(ga - ra) / np.linalg.norm(ra)
+        assert rel < 3e-2, (a, rel)
+    assert (g[313 * 64:320 * 64] == 0).all()
+    np.testing.assert_array_equal(cache.weights(), w)  # gradient only, no step
+    exact, _ = oracle.nrc_grad(w.astype(np.float64), x, t, oracle.NRC_EXACT)
+    cos = g.astype(np.float64) @ exact / (np.linalg.norm(g) * np.linalg.norm(exact))
+    assert cos > 0.99, cos
+
+
+def test_adam_step_matches_oracle_exactly():
+    cache = api.NeuralCache()
+    w = he_weights(8)
+    cache.set_weights(w)
+    rng = np.random.default_rng(9)
+    m = np.zeros(NW, np.float32)
+    v = np.zeros(NW, np.float32)
+    ref = w.copy()
+    o = cache.opts
+    p1 = p2 = 1.0
+    for step in range(3):
+        g = (rng.standard_normal(NW) * 10.0 ** rng.integers(-6, 0)).astype(np.float32)
+        g[::17] = 0.0
+        cache.apply_gradient(g)
+        p1 *= float(np.float32(o.beta1))
+        p2 *= float(np.float32(o.beta2))
+        oracle.nrc_adam(ref, m, v, g, np.float32(o.learning_rate), np.float32(o.beta1),
+                        np.float32(o.beta2), np.float32(o.epsilon), np.float32(1 - p1),
+                        np.float32(1 - p2))
+        np.testing.assert_array_equal(cache.weights(), ref)
+    assert cache.stats()["steps"] == 3
+
+
+def test_training_fits_a_smooth_field():
+    cache = api.NeuralCache(api.NeuralCacheOpts(learning_rate=3e-3, seed=2))
+    rng = np.random.default_rng(10)
+
+    def field(x):
+        r = 0.6 + 0.4 * np.sin(2 * np.pi * x[:, 0]) * x[:, 10]
+        g = 0.5 + 0.3 * np.cos(np.pi * x[:, 1]) * x[:, 11]
+        b = 0.2 + 0.5 * x[:, 8] * x[:, 12]
+        return np.stack([r, g, b], 1).astype(np.float32)
+
+    xv = rng.random((4096, 13)).astype(np.float32)
+    tv = field(xv)
+    first = None
+    for it in range(300):
+        x = rng.random((4096, 13)).astype(np.float32)
+        loss = cache.train(x, field(x))
+        first = first if first is not None else loss
+    assert np.isfinite(loss)
+    assert loss < first / 10, (first, loss)
+    y = cache.infer(xv)
+    rel = np.mean((y - tv) ** 2 / (tv ** 2 + 0.01))
+    assert rel < 0.05, rel  # 300 steps of online Adam: about 0.02 measured
+
+
+def _rel_mse(img, ref):
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+
+
+def test_cached_tracer_learns_the_indirect_light():
+    spec = scenes.cornell_box(64, 64)
+    scene = spec.build()
+    cam = spec.camera()
+    opts = api.PathTracerOpts(samples=1, bounces=8, rr_start=3, guiding=api.Guiding.OFF)
+    pipe = api.RayTracer(api.Medium).create_pipeline(opts)
+    ref_pipe = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=2048, bounces=8, rr_start=3, guiding=api.Guiding.OFF))
+    rb = api.DataBuffers(64, 64, 1)
+    ref_pipe.dispatch(scene, cam, rb, 777)
+    ref = rb.output(api.Output.COLOR)[..., :3].astype(np.float64)
+
+    def run(cache, frames, skip):
+        bufs = api.DataBuffers(64, 64, 1)
+        acc = np.zeros((64, 64, 3))
+        for f in range(frames):
+            pipe.dispatch_cached(scene, cam, bufs, cache, f)
+            if f >= skip:
+                acc += bufs.output(api.Output.COLOR)[..., :3]
+        return acc / (frames - skip)
+
+    cache = api.NeuralCache(api.NeuralCacheOpts(train_paths=1024, train_steps=4, seed=3))
+    img = run(cache, 300, 100)
+    st = cache.stats()
+    assert st["steps"] == 300 * 4
+    assert 0.6 * 1024 < st["records"] < 1.4 * 1024, st
+    assert st["queries"] > 64 * 64 * 0.3, st
+    assert np.isfinite(img).all()
+    # the same frames with an untrained zero cache: paths truncated at vertex 2
+    zero = api.NeuralCache(api.NeuralCacheOpts(train_paths=0, seed=3))
+    zero.set_weights(np.zeros(NW, np.float32))
+    trunc = run(zero, 200, 0)
+    e_nrc, e_trunc = _rel_mse(img, ref), _rel_mse(trunc, ref)
+    assert e_nrc < 0.5 * e_trunc, (e_nrc, e_trunc)
+    assert abs(img.mean() - ref.mean()) / ref.mean() < 0.1, (img.mean(), ref.mean())
+    assert trunc.mean() < img.mean()
