@@ -13,6 +13,12 @@ BVH level, L = ceil(log8(N/4)) = 6) over its HIP-event launch durations.
 cpu_baseline: the CPU oracle (C++/OpenMP restatement of the reference
 shaders, "port") on a bounded band of rows of the same frame.
 
+time_to_relmse: the neural-cache tracer against the plain tracer (both 8
+bounces) on the same scene: GPU time for each to reach the relMSE that the
+plain tracer has at 16, 64 and 256 spp, judged against a 4096-spp plain
+reference accumulated in fp64; plus the matrix-core rate of the cache's
+inference and training kernels against the bf16 dense peak.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--triangles N]
 """
 from __future__ import annotations
@@ -57,6 +63,125 @@ def cpu_baseline(spec, budget_s: float = 12.0) -> dict:
                       f"restatement, not lavapipe"}
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+# Algorithmic FLOPs of the cache network per query (312 encoded inputs):
+# 2 * (312*64 + 4*64*64 + 64*3); a training sample costs about 3x that.
+NRC_FLOP_FWD = 2 * (312 * 64 + 4 * 64 * 64 + 64 * 3)
+NRC_FLOP_TRAIN = 3 * NRC_FLOP_FWD
+
+
+def rel_mse(img, ref) -> float:
+    import numpy as np
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+
+
+def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int) -> dict:
+    """Time-to-relMSE of the cached tracer against the plain tracer, both with
+    8-bounce paths (RR from bounce 3), judged against a plain reference of
+    ref_spp samples accumulated in fp64.  Times are GPU wall times of the
+    dispatches (host read-back of the images for scoring is not counted)."""
+    import numpy as np
+    W, H = spec.width, spec.height
+    popts = dict(bounces=8, rr_start=3, guiding=api.Guiding.OFF,
+                 restir_temporal=api.RestirTemporalMode.Disabled)
+
+    def plain(spp, seed):
+        pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=spp, **popts))
+        b = api.DataBuffers(W, H, 1, api.BufferType(0))
+        api.sync()
+        t0 = time.perf_counter()
+        pipe.dispatch(scene, cam, b, seed)
+        api.sync()
+        dt = time.perf_counter() - t0
+        return b.output(api.Output.COLOR)[..., :3].astype(np.float64), dt
+
+    t0 = time.perf_counter()
+    chunk = 64
+    ref = np.zeros((H, W, 3))
+    for i in range(ref_spp // chunk):
+        ref += plain(chunk, 10_000_000 + i)[0]
+    ref /= ref_spp // chunk
+    log(f"cache leg: reference {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
+    plain_pts = []
+    for spp in (16, 64, 256):
+        img, dt = plain(spp, 20_000 + spp)
+        plain_pts.append({"spp": spp, "seconds": dt, "relmse": rel_mse(img, ref)})
+        log(f"cache leg: plain {spp} spp {dt * 1e3:.1f} ms relMSE {plain_pts[-1]['relmse']:.5f}")
+
+    cache = api.NeuralCache(api.NeuralCacheOpts(learning_rate=3e-3))
+    pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=1, **popts))
+    b = api.DataBuffers(W, H, 1, api.BufferType(0))
+    b.set_profiling(True)
+    b.kernel_stats()
+    t = 0.0
+    for f in range(warm):
+        api.sync()
+        t0 = time.perf_counter()
+        pipe.dispatch_cached(scene, cam, b, cache, f)
+        api.sync()
+        t += time.perf_counter() - t0
+    acc = np.zeros((H, W, 3))
+    checks = {1, 2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 320, 384, 448, frames}
+    pts = []
+    for n in range(1, frames + 1):
+        api.sync()
+        t0 = time.perf_counter()
+        pipe.dispatch_cached(scene, cam, b, cache, warm + n)
+        api.sync()
+        t += time.perf_counter() - t0
+        acc += b.output(api.Output.COLOR)[..., :3]
+        if n in checks:
+            pts.append({"frames": n, "seconds": t, "relmse": rel_mse(acc / n, ref)})
+    ks = b.kernel_stats()
+    st = cache.stats()
+    total = warm + frames
+    log(f"cache leg: {total} frames in {t:.2f} s, final relMSE {pts[-1]['relmse']:.5f}")
+    targets = []
+    for p in plain_pts:
+        hit = next((q for q in pts if q["relmse"] <= p["relmse"]), None)
+        targets.append({"plain_spp": p["spp"], "relmse": p["relmse"], "plain_seconds": p["seconds"],
+                        "cache_seconds": hit["seconds"] if hit else None,
+                        "speedup": p["seconds"] / hit["seconds"] if hit else None})
+    infer_flops = st["queries"] * NRC_FLOP_FWD * ks["cache_infer_launches"]
+    train_flops = st["records"] * NRC_FLOP_TRAIN * total
+    infer_tf = infer_flops / (ks["cache_infer_ms"] * 1e-3) / 1e12 if ks["cache_infer_ms"] else 0.0
+    train_tf = train_flops / (ks["cache_train_ms"] * 1e-3) / 1e12 if ks["cache_train_ms"] else 0.0
+    return {
+        "workload": f"procedural {W}x{H}, 8 bounces, RR from 3; cache: paths end at the second "
+                    f"vertex, 65536 training paths and 4 Adam steps (lr 3e-3) per frame, "
+                    f"{warm} warm-up frames (timed, not accumulated)",
+        "reference": f"plain tracer {ref_spp} spp, fp64 accumulation of {chunk}-spp dispatches",
+        "plain": plain_pts,
+        "cache_points": pts,
+        "cache_ms_per_frame": t / total * 1e3,
+        "targets": targets,
+        "headline": targets[-1],
+        "mfma": {
+            "infer_tflops": infer_tf, "infer_frac": infer_tf / MFMA_BF16_PEAK_TFLOPS,
+            "infer_ms_per_frame": ks["cache_infer_ms"] / total,
+            "queries_per_frame": st["queries"],
+            "train_tflops": train_tf, "train_frac": train_tf / MFMA_BF16_PEAK_TFLOPS,
+            "train_ms_per_frame": ks["cache_train_ms"] / total,
+            "records_per_frame": st["records"],
+            "peak_tflops": MFMA_BF16_PEAK_TFLOPS, "flop_per_query": NRC_FLOP_FWD,
+        },
+    }
+
+
+def calibrate_copy(gib: int = 1, launches: int = 3) -> int:
+    """A plain device-to-device copy of known size (torch's copy kernel) for
+    scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    import torch
+    n = gib << 30
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(launches):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copied {gib} GiB {launches} times")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,7 +191,15 @@ def main() -> int:
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cache-leg", action="store_true")
+    ap.add_argument("--ref-spp", type=int, default=4096)
+    ap.add_argument("--cache-frames", type=int, default=512)
+    ap.add_argument("--cache-warmup", type=int, default=64)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (rocprofv3 counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(image rows would shard across GPUs; see DESIGN.md). Run with --gpus 1.",
@@ -161,6 +294,9 @@ def main() -> int:
             out["cpu_baseline"] = cpu_baseline(spec)
         except Exception as exc:  # the GPU figure stands on its own
             log(f"cpu baseline failed: {exc!r}")
+    if not args.no_cache_leg:
+        out["time_to_relmse"] = cache_leg(api, scene, spec, cam, args.ref_spp, args.cache_frames,
+                                          args.cache_warmup)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -350,7 +350,9 @@ typedef struct lmb_nrc_opts {
     uint32_t train_paths;  /* training paths per lmb_trace_nrc      (65536) */
     uint32_t train_steps;  /* optimiser steps per lmb_trace_nrc     (4)     */
     uint32_t seed;         /* weight initialisation (He-uniform)            */
-    uint32_t reserved;
+    float ema_decay;       /* inference uses an exponential moving average
+                              of the weights with this decay per step; 0 = the
+                              trained weights themselves               (0)  */
 } lmb_nrc_opts;
 
 void lmb_nrc_opts_default(lmb_nrc_opts *opts);
@@ -372,6 +374,10 @@ lmb_status lmb_nrc_train(lmb_nrc nrc, const float *inputs, const float *targets,
                          uint64_t n, float *loss);
 /* One Adam step with a caller-supplied gradient (LMB_NRC_NUM_WEIGHTS). */
 lmb_status lmb_nrc_apply_gradient(lmb_nrc nrc, const float *grad);
+/* Replaces the options of a live cache (learning-rate schedules, stopping
+ * the training with train_paths = 0); the weights and the optimiser state
+ * are kept.  The seed is ignored. */
+lmb_status lmb_nrc_set_opts(lmb_nrc nrc, const lmb_nrc_opts *opts);
 
 typedef struct lmb_nrc_stats {
     uint64_t steps;          /* optimiser steps taken                     */

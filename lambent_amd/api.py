@@ -129,7 +129,7 @@ class _NrcOptsC(ctypes.Structure):
     _fields_ = [("learning_rate", ctypes.c_float), ("beta1", ctypes.c_float),
                 ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float),
                 ("train_paths", ctypes.c_uint32), ("train_steps", ctypes.c_uint32),
-                ("seed", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("seed", ctypes.c_uint32), ("ema_decay", ctypes.c_float)]
 
 
 class _NrcStatsC(ctypes.Structure):
@@ -210,6 +210,7 @@ _OPTIONAL = {
     "lmb_nrc_gradient": ([_vp, _f32p, _f32p, ctypes.c_uint64, _f32p, _f32p], ctypes.c_int32),
     "lmb_nrc_train": ([_vp, _f32p, _f32p, ctypes.c_uint64, _f32p], ctypes.c_int32),
     "lmb_nrc_apply_gradient": ([_vp, _f32p], ctypes.c_int32),
+    "lmb_nrc_set_opts": ([_vp, ctypes.POINTER(_NrcOptsC)], ctypes.c_int32),
     "lmb_nrc_stats_get": ([_vp, ctypes.POINTER(_NrcStatsC)], ctypes.c_int32),
     "lmb_trace_nrc": ([_vp, _vp, ctypes.POINTER(_CameraC), _vp, ctypes.c_uint32, _vp, _vp],
                       ctypes.c_int32),
@@ -834,6 +835,15 @@ class NeuralCacheOpts:
     train_paths: int = 65536
     train_steps: int = 4
     seed: int = 1
+    ema_decay: float = 0.0
+
+
+def _nrc_opts_c(o: NeuralCacheOpts) -> _NrcOptsC:
+    c = _NrcOptsC()
+    c.learning_rate, c.beta1, c.beta2, c.epsilon = (o.learning_rate, o.beta1, o.beta2, o.epsilon)
+    c.train_paths, c.train_steps, c.seed = o.train_paths, o.train_steps, o.seed
+    c.ema_decay = o.ema_decay
+    return c
 
 
 class NeuralCache:
@@ -841,14 +851,18 @@ class NeuralCache:
 
     def __init__(self, opts: Optional[NeuralCacheOpts] = None):
         o = opts or NeuralCacheOpts()
-        c = _NrcOptsC()
-        c.learning_rate, c.beta1, c.beta2, c.epsilon = (o.learning_rate, o.beta1, o.beta2,
-                                                        o.epsilon)
-        c.train_paths, c.train_steps, c.seed = o.train_paths, o.train_steps, o.seed
+        c = _nrc_opts_c(o)
         h = _vp()
         _check(_lib.lmb_nrc_create(ctypes.byref(c), ctypes.byref(h)))
         self._h = h
         self.opts = o
+
+    def set_opts(self, opts: NeuralCacheOpts) -> None:
+        """Learning-rate schedules, freezing (train_paths=0): weights and the
+        optimiser state are kept."""
+        c = _nrc_opts_c(opts)
+        _check(_lib.lmb_nrc_set_opts(self._h, ctypes.byref(c)))
+        self.opts = opts
 
     @property
     def handle(self):

@@ -218,6 +218,64 @@ void lmb_nrc_opts_default(lmb_nrc_opts* o) {
     o->train_paths = 65536;
     o->train_steps = 4;
     o->seed = 1;
+    o->ema_decay = 0.0f;
+}
+
+static lmb_status validate(const lmb_nrc_opts& o) {
+    if (!(o.learning_rate > 0.0f) || !(o.beta1 >= 0.0f && o.beta1 < 1.0f) ||
+        !(o.beta2 >= 0.0f && o.beta2 < 1.0f) || !(o.epsilon >= 0.0f))
+        return fail(LMB_ERR_INVALID_ARGUMENT, "invalid Adam parameters");
+    if (o.train_steps == 0 || o.train_steps > 64)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "train_steps must be in 1..64");
+    if (o.train_paths > (1u << 24))
+        return fail(LMB_ERR_INVALID_ARGUMENT, "train_paths must be <= 2^24");
+    if (!(o.ema_decay >= 0.0f && o.ema_decay < 1.0f))
+        return fail(LMB_ERR_INVALID_ARGUMENT, "ema_decay must be in [0, 1)");
+    return LMB_OK;
+}
+
+// Record buffers for train_paths plus a margin for the random selection.
+static lmb_status ensure_records(lmb_nrc n, uint32_t train_paths) {
+    const uint64_t cap = std::max<uint64_t>(kNrcChunk, (uint64_t)train_paths * 5 / 4);
+    if (cap <= n->dev.rec_cap && n->dev.rec_in) return LMB_OK;
+    lmb_status st = n->rec_in.alloc(cap * sizeof(NrcQuery));
+    if (!st) st = n->rec_aux.alloc(cap * 2 * sizeof(float4));
+    if (!st) st = n->rec_target.alloc(cap * sizeof(float4));
+    if (st) {
+        n->dev.rec_cap = 0;
+        n->dev.rec_in = nullptr;
+        return st;
+    }
+    n->dev.rec_in = n->rec_in.as<NrcQuery>();
+    n->dev.rec_aux = n->rec_aux.as<float4>();
+    n->dev.rec_target = n->rec_target.as<float4>();
+    n->dev.rec_cap = (uint32_t)cap;
+    return LMB_OK;
+}
+
+// Inference weights: the EMA copy and its fragments when ema_decay > 0.
+static lmb_status ensure_ema(lmb_nrc n) {
+    NrcDev& d = n->dev;
+    d.ema_decay = n->opts.ema_decay;
+    if (n->opts.ema_decay > 0.0f) {
+        if (!n->ema.ptr) {
+            lmb_status st = n->ema.alloc(kNrcNumWeights * sizeof(float));
+            if (!st) st = n->fwd_inf.alloc(kNrcFwdFrags * kFragElems * 2);
+            if (st) return st;
+            hipError_t e = hipMemcpy(n->ema.ptr, d.master, kNrcNumWeights * sizeof(float),
+                                     hipMemcpyDeviceToDevice);
+            if (e != hipSuccess) return fail_hip(e, "hipMemcpy weights");
+        }
+        d.ema = n->ema.as<float>();
+        d.fwd_inf = n->fwd_inf.as<uint16_t>();
+    } else {
+        d.ema = nullptr;
+        d.fwd_inf = d.fwd;
+    }
+    hipError_t e = nrc_launch_pack(d, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail_hip(e, "cache weight pack");
+    return LMB_OK;
 }
 
 lmb_status lmb_nrc_create(const lmb_nrc_opts* opts, lmb_nrc* out) {
@@ -226,13 +284,8 @@ lmb_status lmb_nrc_create(const lmb_nrc_opts* opts, lmb_nrc* out) {
     lmb_nrc_opts o;
     lmb_nrc_opts_default(&o);
     if (opts) o = *opts;
-    if (!(o.learning_rate > 0.0f) || !(o.beta1 >= 0.0f && o.beta1 < 1.0f) ||
-        !(o.beta2 >= 0.0f && o.beta2 < 1.0f) || !(o.epsilon >= 0.0f))
-        return fail(LMB_ERR_INVALID_ARGUMENT, "invalid Adam parameters");
-    if (o.train_steps == 0 || o.train_steps > 64)
-        return fail(LMB_ERR_INVALID_ARGUMENT, "train_steps must be in 1..64");
-    if (o.train_paths > (1u << 24))
-        return fail(LMB_ERR_INVALID_ARGUMENT, "train_paths must be <= 2^24");
+    lmb_status vs = validate(o);
+    if (vs) return vs;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
         (void)hipGetLastError();
@@ -249,11 +302,7 @@ lmb_status lmb_nrc_create(const lmb_nrc_opts* opts, lmb_nrc* out) {
     if (!st) st = n->bwd.alloc(kNrcBwdFrags * kFragElems * 2);
     if (!st) st = n->ctl.alloc(kNrcCtlWords * 4);
     if (!st) st = n->loss.alloc(64 * sizeof(float));
-    // records: train_paths plus a margin for the random selection
-    const uint64_t rec_cap = std::max<uint64_t>(kNrcChunk, (uint64_t)o.train_paths * 5 / 4);
-    if (!st) st = n->rec_in.alloc(rec_cap * sizeof(NrcQuery));
-    if (!st) st = n->rec_aux.alloc(rec_cap * 2 * sizeof(float4));
-    if (!st) st = n->rec_target.alloc(rec_cap * sizeof(float4));
+    if (!st) st = ensure_records(n, o.train_paths);
     if (st) {
         delete n;
         return st;
@@ -266,10 +315,7 @@ lmb_status lmb_nrc_create(const lmb_nrc_opts* opts, lmb_nrc* out) {
     d.bwd = n->bwd.as<uint16_t>();
     d.ctl = n->ctl.as<uint32_t>();
     d.loss = n->loss.as<float>();
-    d.rec_in = n->rec_in.as<NrcQuery>();
-    d.rec_aux = n->rec_aux.as<float4>();
-    d.rec_target = n->rec_target.as<float4>();
-    d.rec_cap = (uint32_t)rec_cap;
+    d.fwd_inf = d.fwd;
     hipError_t e = hipMemset(n->ctl.ptr, 0, kNrcCtlWords * 4);
     if (e != hipSuccess) {
         delete n;
@@ -278,12 +324,34 @@ lmb_status lmb_nrc_create(const lmb_nrc_opts* opts, lmb_nrc* out) {
     std::vector<float> w;
     init_weights(o.seed, w);
     st = lmb_nrc_set_weights(n, w.data());
+    if (!st) st = ensure_ema(n);
     if (st) {
         delete n;
         return st;
     }
     *out = n;
     return LMB_OK;
+}
+
+lmb_status lmb_nrc_set_opts(lmb_nrc n, const lmb_nrc_opts* opts) {
+    lmb_status st = check(n);
+    if (st) return st;
+    if (!opts) return fail(LMB_ERR_INVALID_ARGUMENT, "opts is NULL");
+    st = validate(*opts);
+    if (st) return st;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail_hip(e, "hipDeviceSynchronize");
+    st = ensure_records(n, opts->train_paths);
+    if (st) return st;
+    const uint32_t seed = n->opts.seed;
+    const bool beta_changed = opts->beta1 != n->opts.beta1 || opts->beta2 != n->opts.beta2;
+    n->opts = *opts;
+    n->opts.seed = seed;
+    if (beta_changed) {  // the bias corrections restart with the new decays
+        n->pow1 = std::pow((double)n->opts.beta1, (double)n->steps);
+        n->pow2 = std::pow((double)n->opts.beta2, (double)n->steps);
+    }
+    return ensure_ema(n);
 }
 
 lmb_status lmb_nrc_destroy(lmb_nrc n) {
@@ -315,6 +383,8 @@ lmb_status lmb_nrc_set_weights(lmb_nrc n, const float* w) {
         e = hipMemcpy(n->dev.master, w, kNrcNumWeights * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(n->dev.adam_m, 0, kNrcNumWeights * sizeof(float));
     if (e == hipSuccess) e = hipMemset(n->dev.adam_v, 0, kNrcNumWeights * sizeof(float));
+    if (e == hipSuccess && n->dev.ema)
+        e = hipMemcpy(n->dev.ema, w, kNrcNumWeights * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = nrc_launch_pack(n->dev, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) return fail_hip(e, "cache weight upload");

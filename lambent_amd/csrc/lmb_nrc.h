@@ -121,6 +121,11 @@ struct NrcDev {
     float* adam_v;
     uint16_t* fwd;  // kNrcFwdFrags fragments (bf16 bits)
     uint16_t* bwd;  // kNrcBwdFrags fragments
+    // Inference weights: an exponential moving average of the masters and its
+    // forward fragments, or (ema == nullptr) the masters and `fwd` themselves.
+    float* ema;
+    uint16_t* fwd_inf;
+    float ema_decay;
     // Training scratch, feature-major bf16 with row stride `batch`:
     // x0 [320][batch], h [5][64][batch], dz [5][64][batch], g [3][batch].
     uint16_t* x0;

@@ -172,6 +172,7 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_pack(NrcDev d) {
     if (frag < kNrcFwdFrags) {
         const int w = nrc_fwd_index(frag, lane, e);
         d.fwd[i] = w >= 0 ? bits((__bf16)d.master[w]) : (uint16_t)0;
+        if (d.ema) d.fwd_inf[i] = w >= 0 ? bits((__bf16)d.ema[w]) : (uint16_t)0;
     } else {
         const int w = nrc_bwd_index(frag - kNrcFwdFrags, lane, e);
         d.bwd[i - kNrcFwdFrags * kFragElems] = w >= 0 ? bits((__bf16)d.master[w]) : (uint16_t)0;
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_infer(NrcDev d, const NrcQuer
                                                          uint32_t count_fixed,
                                                          float4* __restrict__ dst) {
     __shared__ bf16x8 lds[kNrcFwdFrags * 64];
-    const bf16x8* src = reinterpret_cast<const bf16x8*>(d.fwd);
+    const bf16x8* src = reinterpret_cast<const bf16x8*>(d.fwd_inf);
     for (int i = threadIdx.x; i < kNrcFwdFrags * 64; i += kNrcBlock) lds[i] = src[i];
     __syncthreads();
     const uint32_t count = count_ptr ? *count_ptr : count_fixed;
@@ -428,7 +429,9 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_adam(NrcDev d, AdamParams p,
     d.adam_v[w] = v;
     const float mhat = m / p.bias1;
     const float vhat = v / p.bias2;
-    d.master[w] = d.master[w] - p.lr * mhat / (sqrtf(vhat) + p.eps);
+    const float wn = d.master[w] - p.lr * mhat / (sqrtf(vhat) + p.eps);
+    d.master[w] = wn;
+    if (d.ema) d.ema[w] = d.ema_decay * d.ema[w] + (1.0f - d.ema_decay) * wn;
 }
 
 hipError_t nrc_launch_pack(const NrcDev& d, hipStream_t stream) {

@@ -174,3 +174,47 @@ def test_cached_tracer_learns_the_indirect_light():
     assert e_nrc < 0.5 * e_trunc, (e_nrc, e_trunc)
     assert abs(img.mean() - ref.mean()) / ref.mean() < 0.1, (img.mean(), ref.mean())
     assert trunc.mean() < img.mean()
+
+
+def test_ema_inference_weights():
+    d = 0.75
+    cache = api.NeuralCache(api.NeuralCacheOpts(ema_decay=d, learning_rate=0.05))
+    w = he_weights(12)
+    cache.set_weights(w)
+    x = np.random.default_rng(13).random((512, 13)).astype(np.float32)
+    ref0 = oracle.nrc_forward(w.astype(np.float64), x, oracle.NRC_BF16)
+    assert np.abs(cache.infer(x) - ref0).max() < 2e-2 * max(1.0, np.abs(ref0).max())
+    g = (np.random.default_rng(14).standard_normal(NW) * 1e-3).astype(np.float32)
+    cache.apply_gradient(g)
+    w1 = cache.weights()
+    assert np.abs(w1 - w).max() > 0.04  # an lr-0.05 Adam step moved the masters
+    ema = (np.float32(d) * w + np.float32(1 - d) * w1).astype(np.float32)
+    y = cache.infer(x)
+    ref_ema = oracle.nrc_forward(ema.astype(np.float64), x, oracle.NRC_BF16)
+    ref_raw = oracle.nrc_forward(w1.astype(np.float64), x, oracle.NRC_BF16)
+    scale = max(1.0, np.abs(ref_ema).max())
+    assert np.abs(y - ref_ema).max() < 2e-2 * scale
+    assert np.abs(y - ref_ema).mean() < 0.2 * np.abs(y - ref_raw).mean()
+
+
+def test_set_opts_freezes_training_in_the_tracer():
+    spec = scenes.cornell_box(32, 32)
+    scene = spec.build()
+    pipe = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=1, bounces=8, rr_start=3, guiding=api.Guiding.OFF))
+    bufs = api.DataBuffers(32, 32, 1)
+    o = api.NeuralCacheOpts(train_paths=256, train_steps=2)
+    cache = api.NeuralCache(o)
+    for f in range(3):
+        pipe.dispatch_cached(scene, spec.camera(), bufs, cache, f)
+    assert cache.stats()["steps"] == 6
+    w = cache.weights()
+    cache.set_opts(api.NeuralCacheOpts(**{**o.__dict__, "train_paths": 0}))
+    for f in range(3, 6):
+        pipe.dispatch_cached(scene, spec.camera(), bufs, cache, f)
+    st = cache.stats()
+    assert st["steps"] == 6 and st["records"] == 0, st
+    np.testing.assert_array_equal(cache.weights(), w)
+    assert np.isfinite(bufs.output(api.Output.COLOR)).all()
+    with pytest.raises(api.LambentError):
+        cache.set_opts(api.NeuralCacheOpts(train_steps=0))
