@@ -13,6 +13,9 @@ BVH level, L = ceil(log8(N/4)) = 6) over its HIP-event launch durations.
 cpu_baseline: the CPU oracle (C++/OpenMP restatement of the reference
 shaders, "port") on a bounded band of rows of the same frame.
 
+spectral_restir: the spectral + ReSTIR GI frame loop (guided mode, four
+wavelengths, Cauchy dispersion, temporal + spatial reuse) over a 60-frame
+camera orbit, in ms per frame.
 time_to_relmse: the neural-cache tracer against the plain tracer (both 8
 bounces) on the same scene: GPU time for each to reach the relMSE that the
 plain tracer has at 16, 64 and 256 spp, judged against a 4096-spp plain
@@ -168,6 +171,64 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int) -> di
     }
 
 
+def spectral_leg(api, scenes, triangles: int, width: int, height: int, frames: int) -> dict:
+    """The spectral + ReSTIR GI frame loop: the procedural scene with 20% of
+    its dielectrics dispersive (Cauchy A = 1.5, B = 0.004 um^2), four
+    wavelengths per path, Markov-chain guiding with a 1.5M-entry world grid, a
+    camera orbit of radius 25 (6 degrees per frame), 1 spp per frame, per-pixel
+    reservoirs, temporal reuse, 5 spatial neighbours in a 61-px window and a
+    confidence cap of 20.  One frame = trace + temporal + spatial +
+    advance_frame; the timed region holds `frames` frames back to back."""
+    spec = scenes.procedural(triangles, seed=2, width=width, height=height, dispersive=0.2)
+    scene = spec.build()
+    opts = api.PathTracerOpts(samples=1, bounces=5, rr_start=3, guiding=api.Guiding.MARKOV_CHAIN,
+                              restir_temporal=api.RestirTemporalMode.PerPixel)
+    pipe = api.RayTracer(api.Spectral).create_pipeline(opts)
+    bufs = api.DataBuffers(width, height, 1_500_000, api.BufferType.all())
+    proj = api.perspective(45.0, width / height, 0.1, 200.0)
+    cams = []
+    for f in range(frames + 2):
+        a = math.radians(6.0 * f)
+        eye = (-25.0 * math.sin(a), 0.0, -25.0 * math.cos(a))
+        d = (-eye[0] / 25.0, 0.0, -eye[2] / 25.0)
+        cams.append(api.Camera.from_proj_view(proj, api.look_to_rh(eye, d, (0.0, 1.0, 0.0))))
+    cap = api.AdvanceOptions(20)
+
+    def frame(f):
+        pipe.dispatch(scene, cams[f], bufs, f)
+        bufs.temporal_resample(f)
+        bufs.spatial_resample(5, 61, f)
+        bufs.advance_frame(api.BufferType.all(), cap)
+
+    for f in range(2):  # warm-up: first-use allocations of the guided state
+        frame(f)
+    api.sync()
+    bufs.set_profiling(True)
+    bufs.kernel_stats()
+    t0 = time.perf_counter()
+    for f in range(2, frames + 2):
+        frame(f)
+    api.sync()
+    dt = time.perf_counter() - t0
+    ks = bufs.kernel_stats()
+    img = bufs.output(api.Output.COLOR)
+    log(f"spectral leg: {frames} frames in {dt:.3f} s, {ks['path_vertices']} path vertices")
+    return {
+        "workload": f"procedural {triangles} triangles, 20% of dielectrics dispersive (Cauchy "
+                    f"A=1.5, B=0.004 um^2), {width}x{height}, 4 wavelengths/path (380-780 nm), "
+                    f"1 spp, 5 bounces, RR from 3, Markov-chain guiding (world grid 1.5M), "
+                    f"per-pixel reservoir, temporal + 5 spatial neighbours (61 px), M cap 20, "
+                    f"{frames}-frame orbit of radius 25",
+        "frames": frames,
+        "seconds": dt,
+        "ms_per_frame": dt / frames * 1e3,
+        "path_vertices_per_s": ks["path_vertices"] / dt,
+        "extend_ms_per_frame": ks["extend_ms"] / frames,
+        "shade_ms_per_frame": ks["shade_ms"] / frames,
+        "image_finite": bool(math.isfinite(float(img.sum()))),
+    }
+
+
 def calibrate_copy(gib: int = 1, launches: int = 3) -> int:
     """A plain device-to-device copy of known size (torch's copy kernel) for
     scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
@@ -192,6 +253,8 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cache-leg", action="store_true")
+    ap.add_argument("--no-spectral-leg", action="store_true")
+    ap.add_argument("--spectral-frames", type=int, default=60)
     ap.add_argument("--ref-spp", type=int, default=4096)
     ap.add_argument("--cache-frames", type=int, default=512)
     ap.add_argument("--cache-warmup", type=int, default=64)
@@ -297,6 +360,10 @@ def main() -> int:
     if not args.no_cache_leg:
         out["time_to_relmse"] = cache_leg(api, scene, spec, cam, args.ref_spp, args.cache_frames,
                                           args.cache_warmup)
+    if not args.no_spectral_leg:
+        del scene
+        out["spectral_restir"] = spectral_leg(api, scenes, args.triangles, args.width,
+                                              args.height, args.spectral_frames)
     print(json.dumps(out), flush=True)
     return 0
 

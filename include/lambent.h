@@ -194,6 +194,11 @@ lmb_status lmb_scene_intersect_device(lmb_scene scene, const void *rays_dev,
 #define LMB_QUALITY_LOW 0u    /* path_tracing/mod.rs:126 Low    */
 #define LMB_QUALITY_MEDIUM 1u /* path_tracing/mod.rs:98  Medium */
 #define LMB_QUALITY_HIGH 2u   /* path_tracing/mod.rs:70  High   */
+/* No reference counterpart: High's spectral path with four wavelengths per
+ * path over 380-780 nm (hero-wavelength sampling) and Cauchy dispersion
+ * n = A + B / lambda^2 through the material's IOR-range endpoints
+ * n(760 nm), n(340 nm); see lmb_material_new's ior argument. */
+#define LMB_QUALITY_SPECTRAL 3u
 
 /* path_tracing/mod.rs:25-34 RestirTemporalMode */
 #define LMB_RESTIR_DISABLED 0u

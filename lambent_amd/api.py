@@ -610,13 +610,15 @@ def sync(stream=None) -> None:
 
 # ----------------------------------------------------------------- tracer
 class Quality(enum.IntEnum):
-    """path_tracing/mod.rs:66-147: High (spectral), Medium, Low."""
+    """path_tracing/mod.rs:66-147: High (spectral), Medium, Low; SPECTRAL (no
+    reference counterpart): four wavelengths per path with Cauchy dispersion."""
     LOW = 0
     MEDIUM = 1
     HIGH = 2
+    SPECTRAL = 3
 
 
-Low, Medium, High = Quality.LOW, Quality.MEDIUM, Quality.HIGH
+Low, Medium, High, Spectral = Quality.LOW, Quality.MEDIUM, Quality.HIGH, Quality.SPECTRAL
 
 
 class RestirTemporalMode(enum.IntEnum):

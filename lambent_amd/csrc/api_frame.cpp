@@ -262,7 +262,7 @@ void lmb_tracer_opts_default(lmb_tracer_opts* o) {
 
 lmb_status lmb_tracer_create(const lmb_tracer_opts* opts, lmb_tracer* out) {
     if (!opts || !out) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (opts->quality > LMB_QUALITY_HIGH) return fail(LMB_ERR_INVALID_ARGUMENT, "bad quality");
+    if (opts->quality > LMB_QUALITY_SPECTRAL) return fail(LMB_ERR_INVALID_ARGUMENT, "bad quality");
     if (opts->samples == 0)  // NonZeroU32 in the reference
         return fail(LMB_ERR_INVALID_ARGUMENT, "samples must be non-zero");
     if (opts->bounces == 0 || opts->bounces > kMaxBounces)

@@ -19,6 +19,8 @@ constexpr float kHalfPi = kPi / 2.0f;
 constexpr float kTau = 2.0f * kPi;
 
 constexpr uint32_t kQualityLow = 0, kQualityMedium = 1, kQualityHigh = 2;
+// Four wavelengths per path with Cauchy dispersion (no reference counterpart).
+constexpr uint32_t kQualitySpectral = 3;
 
 struct V3 {
     float x, y, z;
@@ -312,6 +314,56 @@ __device__ __forceinline__ float cie_g(float x, float u, float t1, float t2) {
     float d = x - u;
     return expf(-((d * d) * (tau * tau)) * 0.5f);
 }
+// high.wesl:38-60: saturated sRGB of the multi-lobe CIE fit at lambda (nm).
+__device__ __forceinline__ V3 cie_rgb(float lambda) {
+    float x = fmaf(1.056f, cie_g(lambda, 599.8f, 0.0264f, 0.0323f),
+                   fmaf(0.362f, cie_g(lambda, 442.0f, 0.0624f, 0.0374f),
+                        -(0.065f * cie_g(lambda, 501.1f, 0.049f, 0.0382f))));
+    float y = fmaf(0.821f, cie_g(lambda, 568.8f, 0.0213f, 0.0248f),
+                   0.286f * cie_g(lambda, 530.9f, 0.0613f, 0.0322f));
+    float z = fmaf(1.217f, cie_g(lambda, 437.0f, 0.0845f, 0.0278f),
+                   0.681f * cie_g(lambda, 459.0f, 0.0385f, 0.0725f));
+    float r = (3.2406f * x + -1.5372f * y) + -0.4986f * z;
+    float g = (-0.9689f * x + 1.8758f * y) + 0.0415f * z;
+    float b = (0.0557f * x + -0.204f * y) + 1.057f * z;
+    return v3(saturate(r), saturate(g), saturate(b));
+}
+
+// Spectral quality: four wavelengths per path, lambda_j = 380 + 400
+// fract(u + j/4) nm with u the hero draw (hero-wavelength sampling, Wilkie et
+// al. 2014).  The weights are normalised so that their mean over 380-780 nm
+// is 1 per channel (1 / mean of cie_rgb: 3.7238771, 4.2015504, 5.2524004, by
+// numerical integration; the reference's High constants are the same
+// normalisation over 340-760 nm).  Until the path meets a dispersive
+// interface (IOR range low != high) the colour weight is the mean of the four
+// wavelengths'; there only the hero can follow its refraction, so the path
+// continues with the hero alone (weight = 4 x 1/4 of it) and its IOR from the
+// Cauchy law n = A + B / lambda^2 (lambda in um) through n(760 nm) = low and
+// n(340 nm) = high, the reference's IOR-range endpoints.  *picked >= 2 marks
+// a collapsed path.
+constexpr float kSpecK0 = 3.7238771f, kSpecK1 = 4.2015504f, kSpecK2 = 5.2524004f;
+__device__ __forceinline__ V3 spec_weight(float lambda) {
+    const V3 c = cie_rgb(lambda);
+    return v3(c.x * kSpecK0, c.y * kSpecK1, c.z * kSpecK2);
+}
+__device__ __forceinline__ V3 spec_mean_weight(float u) {
+    V3 acc = v3s(0.0f);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float v = u + 0.25f * (float)j;
+        v = v - floorf(v);
+        acc = acc + spec_weight(fmaf(400.0f, v, 380.0f));
+    }
+    return acc * v3s(0.25f);
+}
+__device__ __forceinline__ float cauchy_ior(float low, float high, float lambda) {
+    const float inv_hi = 1.0f / (0.34f * 0.34f), inv_lo = 1.0f / (0.76f * 0.76f);
+    const float B = (high - low) / (inv_hi - inv_lo);
+    const float A = low - B * inv_lo;
+    const float um = lambda * 0.001f;
+    return A + B / (um * um);
+}
+
 // pick_ref_idx: *picked < 0 means not yet picked (the index is in [0, 1)).
 template <uint32_t Q>
 __device__ __forceinline__ float pick_ref_idx(float* picked, float low, float high,
@@ -325,24 +377,38 @@ __device__ __forceinline__ float pick_ref_idx(float* picked, float low, float hi
             *picked = rand_f32(seed);
             fresh = true;
         }
-        float idx = *picked;
-        if constexpr (Q == kQualityHigh) {
-            if (fresh) {
-                float lambda = 760.0f * (1.0f - idx) + 340.0f * idx;
-                float x = fmaf(1.056f, cie_g(lambda, 599.8f, 0.0264f, 0.0323f),
-                               fmaf(0.362f, cie_g(lambda, 442.0f, 0.0624f, 0.0374f),
-                                    -(0.065f * cie_g(lambda, 501.1f, 0.049f, 0.0382f))));
-                float y = fmaf(0.821f, cie_g(lambda, 568.8f, 0.0213f, 0.0248f),
-                               0.286f * cie_g(lambda, 530.9f, 0.0613f, 0.0322f));
-                float z = fmaf(1.217f, cie_g(lambda, 437.0f, 0.0845f, 0.0278f),
-                               0.681f * cie_g(lambda, 459.0f, 0.0385f, 0.0725f));
-                float r = (3.2406f * x + -1.5372f * y) + -0.4986f * z;
-                float g = (-0.9689f * x + 1.8758f * y) + 0.0415f * z;
-                float b = (0.0557f * x + -0.204f * y) + 1.057f * z;
-                *colour = v3(saturate(r) * 3.910f, saturate(g) * 4.411f, saturate(b) * 5.511f);
+        if constexpr (Q == kQualitySpectral) {
+            // u on the 2^-22 grid, so that the collapsed encoding u + 2 is exact
+            if (fresh) *picked = (*picked + 2.0f) - 2.0f;
+            const bool collapsed = *picked >= 2.0f;
+            const float u = collapsed ? *picked - 2.0f : *picked;
+            const float hero = fmaf(400.0f, u - floorf(u), 380.0f);
+            if (low == high) {
+                if (fresh) *colour = spec_mean_weight(u);
+                return low;
             }
+            if (!collapsed) {
+                const V3 h = spec_weight(hero);
+                if (fresh) {
+                    *colour = h;
+                } else {
+                    const V3 m = spec_mean_weight(u);
+                    *colour = v3(m.x > 0.0f ? h.x / m.x : 0.0f, m.y > 0.0f ? h.y / m.y : 0.0f,
+                                 m.z > 0.0f ? h.z / m.z : 0.0f);
+                }
+                *picked = u + 2.0f;
+            }
+            return cauchy_ior(low, high, hero);
+        } else {
+            float idx = *picked;
+            if constexpr (Q == kQualityHigh) {
+                if (fresh) {
+                    const V3 c = cie_rgb(760.0f * (1.0f - idx) + 340.0f * idx);
+                    *colour = v3(c.x * 3.910f, c.y * 4.411f, c.z * 5.511f);
+                }
+            }
+            return low * (1.0f - idx) + high * idx;
         }
-        return low * (1.0f - idx) + high * idx;
     }
 }
 

@@ -674,6 +674,8 @@ hipError_t launch_guided(const SceneDev& s, const CameraDev& cam, const FrameDev
             return launch_guided_q<kQualityLow>(s, cam, f, g, p, stream, timer);
         case kQualityHigh:
             return launch_guided_q<kQualityHigh>(s, cam, f, g, p, stream, timer);
+        case kQualitySpectral:
+            return launch_guided_q<kQualitySpectral>(s, cam, f, g, p, stream, timer);
         default:
             return launch_guided_q<kQualityMedium>(s, cam, f, g, p, stream, timer);
     }

@@ -251,6 +251,8 @@ hipError_t launch_trace_nrc(const SceneDev& s, const CameraDev& cam, const Frame
             return launch_nrc_q<kQualityLow>(s, cam, f, p, d, np, stream, timer);
         case kQualityHigh:
             return launch_nrc_q<kQualityHigh>(s, cam, f, p, d, np, stream, timer);
+        case kQualitySpectral:
+            return launch_nrc_q<kQualitySpectral>(s, cam, f, p, d, np, stream, timer);
         default:
             return launch_nrc_q<kQualityMedium>(s, cam, f, p, d, np, stream, timer);
     }

@@ -263,6 +263,8 @@ hipError_t launch_trace(const SceneDev& s, const CameraDev& cam, const FrameDev&
             return launch_plain<kQualityLow>(s, cam, f, p, stream, timer);
         case kQualityHigh:
             return launch_plain<kQualityHigh>(s, cam, f, p, stream, timer);
+        case kQualitySpectral:
+            return launch_plain<kQualitySpectral>(s, cam, f, p, stream, timer);
         default:
             return launch_plain<kQualityMedium>(s, cam, f, p, stream, timer);
     }

@@ -8,8 +8,9 @@
 * ``procedural``   - N triangles with centres uniform in [-10, 10]^3 and vertex
   offsets of length LogNormal(-2, 0.5) in uniform directions; per-triangle
   materials 60% diffuse (albedo U[0.1, 0.9]), 30% rough GGX metal (roughness
-  U[0.05, 1]), 10% dielectric (IOR 1.5); white sky; camera at (0, 0, -25)
-  looking at the origin, 45 deg, 16:9.
+  U[0.05, 1]), 10% dielectric (IOR 1.5, optionally a fraction of them
+  dispersive: Cauchy A = 1.5, B = 0.004 um^2); white sky; camera at
+  (0, 0, -25) looking at the origin, 45 deg, 16:9.
 * ``furnace_box``  - closed cube, all faces one material, camera inside: the
   radiance of every path is known in closed form.
 
@@ -150,11 +151,21 @@ def _pack_f16_pair(a: np.ndarray, b: np.ndarray) -> np.ndarray:
 ATLAS = 1024
 
 
-def procedural_materials(kinds: np.ndarray) -> np.ndarray:
+CAUCHY_A, CAUCHY_B = 1.5, 0.004  # um^2
+
+
+def cauchy_range(a: float = CAUCHY_A, b: float = CAUCHY_B) -> tuple:
+    """(n(760 nm), n(340 nm)) of n = a + b / lambda^2: the IOR-range endpoints
+    of a dispersive dielectric (lib.rs:132-143 ior argument)."""
+    return a + b / 0.76 ** 2, a + b / 0.34 ** 2
+
+
+def procedural_materials(kinds: np.ndarray, dispersive: Optional[np.ndarray] = None) -> np.ndarray:
     """Material words for triangle i: its own texel in the 1024 x 1024 atlases.
 
     Equivalent to Material.new(uv, uv, uv, [x, y], 0, None, attr, scale 0, ior,
-    ty) with uv the texel centre; the equivalence is checked by a test."""
+    ty) with uv the texel centre; the equivalence is checked by a test.
+    ``dispersive`` marks dielectrics whose IOR range is cauchy_range()."""
     n = len(kinds)
     i = np.arange(n, dtype=np.uint64)
     tx = (i % ATLAS).astype(np.uint32)
@@ -174,11 +185,18 @@ def procedural_materials(kinds: np.ndarray) -> np.ndarray:
     one = _pack_f16_pair(np.float32(1.0), np.float32(1.0))
     glass = _pack_f16_pair(np.float32(1.5), np.float32(1.5))
     words[:, 7] = np.where(kinds == 2, glass, one)
+    if dispersive is not None:
+        lo, hi = cauchy_range()
+        words[:, 7] = np.where((kinds == 2) & dispersive,
+                               _pack_f16_pair(np.float32(lo), np.float32(hi)), words[:, 7])
     return words
 
 
 def procedural(n_tri: int = 1_000_000, seed: int = 2, width: int = 1920,
-               height: int = 1080) -> SceneSpec:
+               height: int = 1080, dispersive: float = 0.0) -> SceneSpec:
+    """``dispersive``: fraction of the dielectrics given the Cauchy IOR range
+    (A = 1.5, B = 0.004 um^2); drawn after every other random number, so the
+    geometry and the other materials do not depend on it."""
     if n_tri > ATLAS * ATLAS:
         raise ValueError("at most 2^20 triangles fit the material atlases")
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -203,7 +221,8 @@ def procedural(n_tri: int = 1_000_000, seed: int = 2, width: int = 1920,
     attributes[:n_tri, 0] = r8
     attributes[:n_tri, 1] = r8
     attributes[:, 3] = 0
-    mats = procedural_materials(kinds)
+    disp = rng.uniform(size=n_tri) < dispersive if dispersive > 0.0 else None
+    mats = procedural_materials(kinds, disp)
     proj = api.perspective(45.0, width / height, 0.1, 200.0)
     view = api.look_to_rh((0.0, 0.0, -25.0), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0))
     return SceneSpec(v, None, np.arange(n_tri, dtype=np.uint32), mats, [_rgba(1, 1, 1)], [],

@@ -407,3 +407,15 @@ def nrc_adam(w, m, v, grad, lr, beta1, beta2, eps, bias1, bias2):
     """In-place Adam step on float32 arrays (the GPU's float arithmetic)."""
     g = np.ascontiguousarray(grad, np.float32)
     _lib.orc_nrc_adam(_f(w), _f(m), _f(v), _f(g), len(w), lr, beta1, beta2, eps, bias1, bias2)
+
+
+_sig("orc_spectral_pick", [_f32p, ctypes.c_float, ctypes.c_float, ctypes.c_uint32, _f32p],
+     ctypes.c_float)
+
+
+def spectral_pick(picked: float, lo: float, hi: float, seed: int):
+    """Spectral quality at one vertex: (ior, colour weight, new picked)."""
+    p = np.array([picked], np.float32)
+    c = np.empty(3, np.float32)
+    ri = _lib.orc_spectral_pick(_f(p), lo, hi, seed, _f(c))
+    return float(ri), c, float(p[0])

@@ -314,26 +314,84 @@ float cie_g(float x, float u, float t1, float t2) {
     float d = x - u;
     return std::exp(-((d * d) * (tau * tau)) * 0.5f);
 }
+// high.wesl:38-60: saturated sRGB of the multi-lobe CIE fit at lambda (nm)
+F3 cie_rgb(float lambda) {
+    float x = std::fma(1.056f, cie_g(lambda, 599.8f, 0.0264f, 0.0323f),
+                       std::fma(0.362f, cie_g(lambda, 442.0f, 0.0624f, 0.0374f),
+                                -(0.065f * cie_g(lambda, 501.1f, 0.049f, 0.0382f))));
+    float y = std::fma(0.821f, cie_g(lambda, 568.8f, 0.0213f, 0.0248f),
+                       0.286f * cie_g(lambda, 530.9f, 0.0613f, 0.0322f));
+    float z = std::fma(1.217f, cie_g(lambda, 437.0f, 0.0845f, 0.0278f),
+                       0.681f * cie_g(lambda, 459.0f, 0.0385f, 0.0725f));
+    float r = (3.2406f * x + -1.5372f * y) + -0.4986f * z;
+    float g = (-0.9689f * x + 1.8758f * y) + 0.0415f * z;
+    float b = (0.0557f * x + -0.204f * y) + 1.057f * z;
+    return mk(saturate(r), saturate(g), saturate(b));
+}
+
+// Spectral quality (no reference counterpart): four wavelengths per path,
+// lambda_j = 380 + 400 fract(u + j/4) nm, u the hero draw.  The colour
+// weight is the mean of the four normalised CIE weights until the path meets
+// a dispersive interface (IOR range lo != hi); there the hero wavelength
+// alone continues (the others cannot follow its refraction) and the weight
+// becomes the hero's.  IOR: Cauchy n = A + B / lambda^2 (lambda in um)
+// through n(760 nm) = lo and n(340 nm) = hi.  picked >= 2 marks "collapsed".
+constexpr float kSpecK0 = 3.7238771f, kSpecK1 = 4.2015504f, kSpecK2 = 5.2524004f;
+F3 spec_weight(float lambda) {
+    F3 c = cie_rgb(lambda);
+    return mk(c.x * kSpecK0, c.y * kSpecK1, c.z * kSpecK2);
+}
+F3 spec_mean_weight(float u) {
+    F3 acc = splat(0.0f);
+    for (int j = 0; j < 4; ++j) {
+        float v = u + 0.25f * (float)j;
+        v = v - std::floor(v);
+        acc = add(acc, spec_weight(std::fma(400.0f, v, 380.0f)));
+    }
+    return scale(acc, 0.25f);
+}
+float cauchy_ior(float lo, float hi, float lambda) {
+    const float inv_hi = 1.0f / (0.34f * 0.34f), inv_lo = 1.0f / (0.76f * 0.76f);
+    const float B = (hi - lo) / (inv_hi - inv_lo);
+    const float A = lo - B * inv_lo;
+    const float um = lambda * 0.001f;
+    return A + B / (um * um);
+}
+
 // returns IOR; *colour = wavelength weight; *picked < 0 = not picked yet
 float pick_ref_idx(uint32_t q, float* picked, float lo, float hi, uint32_t seed, F3* colour) {
     *colour = splat(1.0f);
     if (q == 0) return lo;
     bool fresh = *picked < 0.0f;
     if (fresh) *picked = orc_rand_f32(seed);
+    if (q == 3) {
+        // u on the 2^-22 grid, so that the collapsed encoding u + 2 is exact
+        if (fresh) *picked = (*picked + 2.0f) - 2.0f;
+        const bool collapsed = *picked >= 2.0f;
+        const float u = collapsed ? *picked - 2.0f : *picked;
+        const float hero = std::fma(400.0f, u - std::floor(u), 380.0f);
+        if (lo == hi) {
+            if (fresh) *colour = spec_mean_weight(u);
+            return lo;
+        }
+        if (!collapsed) {
+            const F3 h = spec_weight(hero);
+            if (fresh) {
+                *colour = h;
+            } else {
+                const F3 m = spec_mean_weight(u);
+                *colour = mk(m.x > 0.0f ? h.x / m.x : 0.0f, m.y > 0.0f ? h.y / m.y : 0.0f,
+                             m.z > 0.0f ? h.z / m.z : 0.0f);
+            }
+            *picked = u + 2.0f;
+        }
+        return cauchy_ior(lo, hi, hero);
+    }
     float u = *picked;
     if (q == 2 && fresh) {
         float lambda = 760.0f * (1.0f - u) + 340.0f * u;
-        float x = std::fma(1.056f, cie_g(lambda, 599.8f, 0.0264f, 0.0323f),
-                           std::fma(0.362f, cie_g(lambda, 442.0f, 0.0624f, 0.0374f),
-                                    -(0.065f * cie_g(lambda, 501.1f, 0.049f, 0.0382f))));
-        float y = std::fma(0.821f, cie_g(lambda, 568.8f, 0.0213f, 0.0248f),
-                           0.286f * cie_g(lambda, 530.9f, 0.0613f, 0.0322f));
-        float z = std::fma(1.217f, cie_g(lambda, 437.0f, 0.0845f, 0.0278f),
-                           0.681f * cie_g(lambda, 459.0f, 0.0385f, 0.0725f));
-        float r = (3.2406f * x + -1.5372f * y) + -0.4986f * z;
-        float g = (-0.9689f * x + 1.8758f * y) + 0.0415f * z;
-        float b = (0.0557f * x + -0.204f * y) + 1.057f * z;
-        *colour = mk(saturate(r) * 3.910f, saturate(g) * 4.411f, saturate(b) * 5.511f);
+        F3 c = cie_rgb(lambda);
+        *colour = mk(c.x * 3.910f, c.y * 4.411f, c.z * 5.511f);
     }
     return lo * (1.0f - u) + hi * u;
 }
@@ -1203,6 +1261,16 @@ void orc_high_colour(uint32_t seed, float* out) {
     out[1] = c.y;
     out[2] = c.z;
     out[3] = picked;
+}
+// Spectral-quality hook: pick at a vertex with IOR range (lo, hi) starting
+// from *picked (-1 = fresh); returns the IOR, writes the colour weight.
+float orc_spectral_pick(float* picked, float lo, float hi, uint32_t seed, float* colour) {
+    F3 c;
+    float ri = pick_ref_idx(3, picked, lo, hi, seed, &c);
+    colour[0] = c.x;
+    colour[1] = c.y;
+    colour[2] = c.z;
+    return ri;
 }
 void orc_background(void* h, const float* d, float* out) {
     F3 c = background(*(const Scene*)h, mk(d[0], d[1], d[2]));

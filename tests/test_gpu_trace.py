@@ -32,7 +32,7 @@ def rel_mse(img, ref):
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
 
 
-@pytest.mark.parametrize("quality", [api.Low, api.Medium, api.High])
+@pytest.mark.parametrize("quality", [api.Low, api.Medium, api.High, api.Spectral])
 @pytest.mark.parametrize("ty", [api.MaterialType.Diffuse, api.MaterialType.Metallic])
 def test_furnace_closed_form(quality, ty):
     rho = 0.6
@@ -40,11 +40,11 @@ def test_furnace_closed_form(quality, ty):
     color, normal, albedo, verts = _trace(spec, quality, samples=3, bounces=6)
     r = float(np.float32(round(rho * 255)) / np.float32(255))
     expect = 1.5 * sum(r ** k for k in range(6))
-    if quality == api.High:
-        # the High quality weights each path by its wavelength's colour
+    if quality in (api.High, api.Spectral):
+        # the spectral qualities weight each path by its wavelengths' colour
         c_o, _, _, _ = oracle.Scene(spec).trace_plain(spec.camera().to_bytes(), spec.width,
-                                                      spec.height, quality=2, samples=3,
-                                                      bounces=6)
+                                                      spec.height, quality=int(quality),
+                                                      samples=3, bounces=6)
         np.testing.assert_allclose(color, c_o, rtol=2e-5, atol=1e-6)
     else:
         exact = np.all(np.abs(color[..., :3] - expect) <= 2e-6 * expect, axis=-1)
@@ -57,7 +57,7 @@ def test_furnace_closed_form(quality, ty):
     np.testing.assert_allclose(np.linalg.norm(normal[..., :3], axis=-1), 1.0, rtol=1e-5)
 
 
-@pytest.mark.parametrize("quality", [api.Low, api.Medium, api.High])
+@pytest.mark.parametrize("quality", [api.Low, api.Medium, api.High, api.Spectral])
 def test_cornell_same_seeds_as_oracle(quality):
     spec = scenes.cornell_box(128, 128)
     color, normal, albedo, verts = _trace(spec, quality, samples=4, bounces=8, seed_offset=3)
@@ -77,6 +77,19 @@ def test_procedural_same_seeds_as_oracle():
     color, normal, albedo, verts = _trace(spec, api.Medium, samples=2, bounces=5, rr_start=3)
     c_o, n_o, a_o, v_o = oracle.Scene(spec).trace_plain(
         spec.camera().to_bytes(), 160, 90, quality=1, samples=2, bounces=5, rr_start=3)
+    np.testing.assert_array_equal(albedo, a_o)
+    close = np.all(np.abs(color - c_o) <= 1e-4 * (1 + np.abs(c_o)), axis=-1)
+    assert close.mean() > 0.97, f"only {close.mean():.4f} of pixels match the oracle"
+    assert abs(verts - v_o) <= 0.02 * v_o
+
+
+def test_dispersive_spectral_same_seeds_as_oracle():
+    """Four-wavelength quality on a scene whose dielectrics are all dispersive
+    (Cauchy A = 1.5, B = 0.004 um^2): same seeds, same images as the oracle."""
+    spec = scenes.procedural(50_000, seed=6, width=160, height=90, dispersive=1.0)
+    color, normal, albedo, verts = _trace(spec, api.Spectral, samples=2, bounces=6, rr_start=3)
+    c_o, n_o, a_o, v_o = oracle.Scene(spec).trace_plain(
+        spec.camera().to_bytes(), 160, 90, quality=3, samples=2, bounces=6, rr_start=3)
     np.testing.assert_array_equal(albedo, a_o)
     close = np.all(np.abs(color - c_o) <= 1e-4 * (1 + np.abs(c_o)), axis=-1)
     assert close.mean() > 0.97, f"only {close.mean():.4f} of pixels match the oracle"
