@@ -16,6 +16,8 @@ shaders, "port") on a bounded band of rows of the same frame.
 spectral_restir: the spectral + ReSTIR GI frame loop (guided mode, four
 wavelengths, Cauchy dispersion, temporal + spatial reuse) over a 60-frame
 camera orbit, in ms per frame.
+batched_uneven: 64 uneven scenes in one batch of persistent launches
+(hipGraph, autotuned launch shape) against one lmb_trace per image.
 time_to_relmse: the neural-cache tracer against the plain tracer (both 8
 bounces) on the same scene: GPU time for each to reach the relMSE that the
 plain tracer has at 16, 64 and 256 spp, judged against a 4096-spp plain
@@ -229,6 +231,74 @@ def spectral_leg(api, scenes, triangles: int, width: int, height: int, frames: i
     }
 
 
+def batch_leg(api, scenes, frames: int) -> dict:
+    """64 uneven procedural scenes (1k-500k triangles log-uniform, 128^2 to
+    1024^2 images) rendered as one batch: 4 spp, 6 bounces, RR from bounce 3,
+    Medium, plain.  Compared with rendering the same 64 images one lmb_trace
+    at a time; the batch's launch shape comes from an autotuner sweep."""
+    t0 = time.perf_counter()
+    specs = scenes.batch_workload()
+    items = [(sp.build(), sp.camera(), sp.width, sp.height) for sp in specs]
+    tris = sum(len(sp.vertices) // 3 for sp in specs)
+    pixels = sum(sp.width * sp.height for sp in specs)
+    log(f"batch leg: {len(items)} scenes, {tris} triangles, {pixels} pixels built in "
+        f"{time.perf_counter() - t0:.1f} s")
+    opts = api.PathTracerOpts(samples=4, bounces=6, rr_start=3, guiding=api.Guiding.OFF,
+                              restir_temporal=api.RestirTemporalMode.Disabled)
+    # one image at a time
+    pipe = api.RayTracer(api.Medium).create_pipeline(opts)
+    bufs = [api.DataBuffers(w, h, 1, api.BufferType(0)) for _, _, w, h in items]
+    for (scene, cam, _, _), b in zip(items, bufs):
+        pipe.dispatch(scene, cam, b, 0)
+    api.sync()
+    t0 = time.perf_counter()
+    for f in range(frames):
+        for (scene, cam, _, _), b in zip(items, bufs):
+            pipe.dispatch(scene, cam, b, 1 + f)
+    api.sync()
+    seq_ms = (time.perf_counter() - t0) / frames * 1e3
+    seq_verts = sum(b.path_vertices() for b in bufs)
+    del bufs
+
+    batch = api.Batch(items, opts, api.Medium)
+
+    def timed(cfg):
+        batch.set_config(cfg)
+        batch.trace(0)
+        api.sync()
+        t0 = time.perf_counter()
+        for f in range(frames):
+            batch.trace(1 + f)
+        api.sync()
+        return (time.perf_counter() - t0) / frames * 1e3
+
+    eager_ms = timed(api.BatchConfig(use_graph=False))
+    graph_ms = timed(api.BatchConfig())
+    sweep = [api.BatchConfig(tile_w=tw, tile_h=th, blocks_per_cu=k, refill=16, pass_pixels=pp)
+             for tw, th in ((8, 8), (64, 1)) for k in (4, 8, 16) for pp in (0, 1 << 22)]
+    t0 = time.perf_counter()
+    best, ms = batch.autotune(sweep, frames=1)
+    tune_s = time.perf_counter() - t0
+    best_ms = timed(best)
+    verts = batch.path_vertices()
+    log(f"batch leg: one-by-one {seq_ms:.1f} ms, batched {eager_ms:.1f} ms eager / "
+        f"{graph_ms:.1f} ms graph, autotuned {best_ms:.1f} ms ({tune_s:.1f} s sweep)")
+    return {
+        "workload": f"{len(items)} procedural scenes, {tris} triangles (1k-500k log-uniform), "
+                    f"{pixels} pixels (128^2-1024^2), 4 spp, 6 bounces, RR from 3, Medium, "
+                    f"2/3 diffuse 1/3 GGX",
+        "one_by_one_ms_per_frame": seq_ms,
+        "batched_eager_ms_per_frame": eager_ms,
+        "batched_graph_ms_per_frame": graph_ms,
+        "autotuned": {"config": best.__dict__, "ms_per_frame": best_ms, "sweep_seconds": tune_s,
+                      "sweep": [{"config": c.__dict__, "ms": m} for c, m in zip(sweep, ms)]},
+        "path_vertices_per_frame": verts,
+        "path_vertices_per_s": verts / (best_ms * 1e-3),
+        "speedup_vs_one_by_one": seq_ms / best_ms,
+        "vertices_match_one_by_one": verts == seq_verts,
+    }
+
+
 def calibrate_copy(gib: int = 1, launches: int = 3) -> int:
     """A plain device-to-device copy of known size (torch's copy kernel) for
     scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
@@ -255,6 +325,8 @@ def main() -> int:
     ap.add_argument("--no-cache-leg", action="store_true")
     ap.add_argument("--no-spectral-leg", action="store_true")
     ap.add_argument("--spectral-frames", type=int, default=60)
+    ap.add_argument("--no-batch-leg", action="store_true")
+    ap.add_argument("--batch-frames", type=int, default=3)
     ap.add_argument("--ref-spp", type=int, default=4096)
     ap.add_argument("--cache-frames", type=int, default=512)
     ap.add_argument("--cache-warmup", type=int, default=64)
@@ -364,6 +436,8 @@ def main() -> int:
         del scene
         out["spectral_restir"] = spectral_leg(api, scenes, args.triangles, args.width,
                                               args.height, args.spectral_frames)
+    if not args.no_batch_leg:
+        out["batched_uneven"] = batch_leg(api, scenes, args.batch_frames)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -405,6 +405,56 @@ lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera *c
                          lmb_data_buffers bufs, uint32_t seed_offset, lmb_nrc nrc,
                          void *stream);
 
+/* ------------------------------------------------------------------ */
+/* Batched tracer                                                      */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference (one rt_main dispatch per image).  Many
+ * scenes and resolutions rendered by one set of persistent wavefront
+ * launches: every ray carries its image index, so one traversal / shading
+ * launch serves all images and balances their uneven work.  A frame (all
+ * passes, samples and bounces) is captured once into a hipGraph and
+ * replayed.  Plain mode (guiding off) only; each image equals what lmb_trace
+ * renders for its scene and camera with the same options and seed. */
+typedef struct lmb_batch_s *lmb_batch;
+
+typedef struct lmb_batch_item {
+    lmb_scene scene;      /* not owned; must outlive the batch */
+    lmb_camera camera;
+    uint32_t width, height;
+} lmb_batch_item;
+
+typedef struct lmb_batch_config {
+    uint32_t blocks_per_cu;  /* persistent grid of every stage, per CU  (8)  */
+    uint32_t refill;         /* idle lanes that make a wave fetch rays  (16) */
+    uint32_t tile_w, tile_h; /* pixel tile of ray generation            (8x8) */
+    uint32_t pass_pixels;    /* pixels per wavefront pass (queue size); 0 = all */
+    uint32_t use_graph;      /* replay the frame as a captured hipGraph (1)  */
+} lmb_batch_config;
+
+void lmb_batch_config_default(lmb_batch_config *cfg);
+/* Writes up to `cap` configurations of the default autotuner sweep (tile
+ * shape x grid size x refill x pass size) and returns their number. */
+uint32_t lmb_batch_default_sweep(lmb_batch_config *out, uint32_t cap);
+lmb_status lmb_batch_create(const lmb_batch_item *items, uint32_t n,
+                            const lmb_tracer_opts *opts, const lmb_batch_config *cfg,
+                            lmb_batch *out);
+lmb_status lmb_batch_destroy(lmb_batch batch);
+lmb_status lmb_batch_set_config(lmb_batch batch, const lmb_batch_config *cfg);
+lmb_status lmb_batch_get_config(lmb_batch batch, lmb_batch_config *cfg);
+lmb_status lmb_batch_set_camera(lmb_batch batch, uint32_t item, const lmb_camera *cam);
+/* One frame of every image; seed_offset as lmb_trace's. */
+lmb_status lmb_batch_trace(lmb_batch batch, uint32_t seed_offset, void *stream);
+/* Colour image of item `item` (width * height * 4 floats), host memory. */
+lmb_status lmb_batch_read_output(lmb_batch batch, uint32_t item, float *host);
+/* Path vertices of the last frame (blocks). */
+lmb_status lmb_batch_path_vertices(lmb_batch batch, uint64_t *vertices);
+/* Times `frames` frames of each candidate (HIP events, after one warm-up
+ * frame that also builds the graph), writes the mean ms per frame of
+ * candidate i to ms[i], applies and returns the fastest. */
+lmb_status lmb_batch_autotune(lmb_batch batch, const lmb_batch_config *candidates,
+                              uint32_t n, uint32_t frames, lmb_batch_config *best,
+                              double *ms);
+
 /* Waits for all work on `stream` (NULL = device-wide). */
 lmb_status lmb_sync(void *stream);
 

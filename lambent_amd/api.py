@@ -125,6 +125,12 @@ class _KernelStatsC(ctypes.Structure):
                 ("cache_train_steps", ctypes.c_uint64)]
 
 
+class _BatchConfigC(ctypes.Structure):
+    _fields_ = [("blocks_per_cu", ctypes.c_uint32), ("refill", ctypes.c_uint32),
+                ("tile_w", ctypes.c_uint32), ("tile_h", ctypes.c_uint32),
+                ("pass_pixels", ctypes.c_uint32), ("use_graph", ctypes.c_uint32)]
+
+
 class _NrcOptsC(ctypes.Structure):
     _fields_ = [("learning_rate", ctypes.c_float), ("beta1", ctypes.c_float),
                 ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float),
@@ -214,6 +220,20 @@ _OPTIONAL = {
     "lmb_nrc_stats_get": ([_vp, ctypes.POINTER(_NrcStatsC)], ctypes.c_int32),
     "lmb_trace_nrc": ([_vp, _vp, ctypes.POINTER(_CameraC), _vp, ctypes.c_uint32, _vp, _vp],
                       ctypes.c_int32),
+    "lmb_batch_config_default": ([ctypes.POINTER(_BatchConfigC)], None),
+    "lmb_batch_default_sweep": ([ctypes.POINTER(_BatchConfigC), ctypes.c_uint32], ctypes.c_uint32),
+    "lmb_batch_create": ([_vp, ctypes.c_uint32, ctypes.POINTER(_TracerOptsC),
+                          ctypes.POINTER(_BatchConfigC), ctypes.POINTER(_vp)], ctypes.c_int32),
+    "lmb_batch_destroy": ([_vp], ctypes.c_int32),
+    "lmb_batch_set_config": ([_vp, ctypes.POINTER(_BatchConfigC)], ctypes.c_int32),
+    "lmb_batch_get_config": ([_vp, ctypes.POINTER(_BatchConfigC)], ctypes.c_int32),
+    "lmb_batch_set_camera": ([_vp, ctypes.c_uint32, ctypes.POINTER(_CameraC)], ctypes.c_int32),
+    "lmb_batch_trace": ([_vp, ctypes.c_uint32, _vp], ctypes.c_int32),
+    "lmb_batch_read_output": ([_vp, ctypes.c_uint32, _f32p], ctypes.c_int32),
+    "lmb_batch_path_vertices": ([_vp, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int32),
+    "lmb_batch_autotune": ([_vp, ctypes.POINTER(_BatchConfigC), ctypes.c_uint32, ctypes.c_uint32,
+                            ctypes.POINTER(_BatchConfigC), ctypes.POINTER(ctypes.c_double)],
+                           ctypes.c_int32),
 }
 for _name, (_args, _res) in _OPTIONAL.items():
     if hasattr(_lib, _name):
@@ -646,19 +666,24 @@ class PathTracerOpts:
     guiding: Guiding = Guiding.MARKOV_CHAIN
 
 
+def _tracer_opts_c(quality: Quality, opts: PathTracerOpts) -> _TracerOptsC:
+    c = _TracerOptsC()
+    c.quality = int(quality)
+    c.samples = int(opts.samples)
+    c.t_min = float(opts.tmin)
+    c.t_max = float(opts.tmax)
+    c.restir = int(opts.restir_temporal)
+    c.bounces = int(opts.bounces)
+    c.rr_start = 0xFFFFFFFF if opts.rr_start is None else int(opts.rr_start)
+    c.guiding = int(opts.guiding)
+    return c
+
+
 class Pipeline:
     """The ray-tracing pipeline (lib.rs:331-376 create_pipeline result)."""
 
     def __init__(self, quality: Quality, opts: PathTracerOpts):
-        c = _TracerOptsC()
-        c.quality = int(quality)
-        c.samples = int(opts.samples)
-        c.t_min = float(opts.tmin)
-        c.t_max = float(opts.tmax)
-        c.restir = int(opts.restir_temporal)
-        c.bounces = int(opts.bounces)
-        c.rr_start = 0xFFFFFFFF if opts.rr_start is None else int(opts.rr_start)
-        c.guiding = int(opts.guiding)
+        c = _tracer_opts_c(quality, opts)
         h = _vp()
         _check(_lib.lmb_tracer_create(ctypes.byref(c), ctypes.byref(h)))
         self._h = h
@@ -915,6 +940,112 @@ class NeuralCache:
     def close(self):
         if self._h:
             _lib.lmb_nrc_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- batched tracer
+class _BatchItemC(ctypes.Structure):
+    _fields_ = [("scene", _vp), ("camera", _CameraC), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32)]
+
+
+@dataclass
+class BatchConfig:
+    """lmb_batch_config: launch shape of the batched wavefronts."""
+    blocks_per_cu: int = 8
+    refill: int = 16
+    tile_w: int = 8
+    tile_h: int = 8
+    pass_pixels: int = 0
+    use_graph: bool = True
+
+    def _c(self) -> _BatchConfigC:
+        c = _BatchConfigC()
+        c.blocks_per_cu, c.refill = self.blocks_per_cu, self.refill
+        c.tile_w, c.tile_h, c.pass_pixels = self.tile_w, self.tile_h, self.pass_pixels
+        c.use_graph = 1 if self.use_graph else 0
+        return c
+
+    @staticmethod
+    def _from(c: _BatchConfigC) -> "BatchConfig":
+        return BatchConfig(c.blocks_per_cu, c.refill, c.tile_w, c.tile_h, c.pass_pixels,
+                           bool(c.use_graph))
+
+    @staticmethod
+    def default_sweep() -> list:
+        n = _lib.lmb_batch_default_sweep(None, 0)
+        arr = (_BatchConfigC * n)()
+        _lib.lmb_batch_default_sweep(arr, n)
+        return [BatchConfig._from(c) for c in arr]
+
+
+class Batch:
+    """Many (scene, camera, resolution) images rendered by one set of
+    persistent wavefront launches (no reference counterpart)."""
+
+    def __init__(self, items, opts: PathTracerOpts, quality: Quality = Quality.MEDIUM,
+                 config: Optional[BatchConfig] = None):
+        self._items = list(items)  # keeps the scenes alive
+        arr = (_BatchItemC * len(self._items))()
+        for k, (scene, cam, w, h) in enumerate(self._items):
+            arr[k].scene = scene.handle
+            arr[k].camera = cam._c
+            arr[k].width, arr[k].height = int(w), int(h)
+        oc = _tracer_opts_c(quality, opts)
+        cc = (config or BatchConfig())._c()
+        h = _vp()
+        _check(_lib.lmb_batch_create(arr, len(self._items), ctypes.byref(oc), ctypes.byref(cc),
+                                     ctypes.byref(h)))
+        self._h = h
+
+    def __len__(self):
+        return len(self._items)
+
+    @property
+    def config(self) -> BatchConfig:
+        c = _BatchConfigC()
+        _check(_lib.lmb_batch_get_config(self._h, ctypes.byref(c)))
+        return BatchConfig._from(c)
+
+    def set_config(self, config: BatchConfig) -> None:
+        c = config._c()
+        _check(_lib.lmb_batch_set_config(self._h, ctypes.byref(c)))
+
+    def set_camera(self, item: int, camera: Camera) -> None:
+        _check(_lib.lmb_batch_set_camera(self._h, item, ctypes.byref(camera._c)))
+
+    def trace(self, seed_offset: int = 0, stream=None) -> None:
+        _check(_lib.lmb_batch_trace(self._h, int(seed_offset) & 0xFFFFFFFF, stream))
+
+    def output(self, item: int) -> np.ndarray:
+        _, _, w, h = self._items[item]
+        out = np.empty((h, w, 4), np.float32)
+        _check(_lib.lmb_batch_read_output(self._h, item, _fptr(out)))
+        return out
+
+    def path_vertices(self) -> int:
+        n = ctypes.c_uint64()
+        _check(_lib.lmb_batch_path_vertices(self._h, ctypes.byref(n)))
+        return n.value
+
+    def autotune(self, candidates=None, frames: int = 2):
+        """(best config, ms per frame of every candidate)."""
+        cands = list(candidates) if candidates is not None else BatchConfig.default_sweep()
+        arr = (_BatchConfigC * len(cands))(*[c._c() for c in cands])
+        ms = (ctypes.c_double * len(cands))()
+        best = _BatchConfigC()
+        _check(_lib.lmb_batch_autotune(self._h, arr, len(cands), frames, ctypes.byref(best), ms))
+        return BatchConfig._from(best), list(ms)
+
+    def close(self):
+        if self._h:
+            _lib.lmb_batch_destroy(self._h)
             self._h = _vp()
 
     def __del__(self):

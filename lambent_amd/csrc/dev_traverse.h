@@ -12,6 +12,7 @@
 #include <stdint.h>
 
 #include "lmb_layout.h"
+#include "lmb_scene_dev.h"
 
 namespace lmb {
 
@@ -300,17 +301,45 @@ __device__ __forceinline__ Hit trace_closest(const Bvh8Node* __restrict__ nodes,
     return tv.h;
 }
 
+// Scene binding of the queue traversal: every ray of a OneScene queue uses
+// the same BVH (uniform pointers, kept in scalar registers); a SceneTable
+// queue tags each ray with a scene index and each lane binds that scene's
+// BVH when it starts the ray.
+struct OneScene {
+    const Bvh8Node* nodes_;
+    const LeafTri* tris_;
+    uint32_t num_nodes_;
+    __device__ __forceinline__ void bind(uint32_t) {}
+    __device__ __forceinline__ const Bvh8Node* nodes() const { return nodes_; }
+    __device__ __forceinline__ const LeafTri* tris() const { return tris_; }
+    __device__ __forceinline__ uint32_t num_nodes() const { return num_nodes_; }
+};
+
+struct SceneTable {
+    const SceneDev* table;
+    const Bvh8Node* nodes_ = nullptr;
+    const LeafTri* tris_ = nullptr;
+    uint32_t num_nodes_ = 0;
+    __device__ __forceinline__ void bind(uint32_t item) {
+        nodes_ = table[item].nodes;
+        tris_ = table[item].tris;
+        num_nodes_ = table[item].num_nodes;
+    }
+    __device__ __forceinline__ const Bvh8Node* nodes() const { return nodes_; }
+    __device__ __forceinline__ const LeafTri* tris() const { return tris_; }
+    __device__ __forceinline__ uint32_t num_nodes() const { return num_nodes_; }
+};
+
 // Persistent traversal over a ray queue with wave-level refill: whenever at
-// least REFILL lanes of the wave are idle, the wave takes that many rays from
-// the queue with one atomic and each idle lane starts one, so lanes do not
-// wait for the longest ray of the wave.  `fetch` counts handed-out rays.
-// Source must provide: Ray load(uint64_t slot), void store(uint64_t slot, Hit).
-template <int BLOCK, int REFILL, class Source>
-__device__ __forceinline__ void traverse_queue(const Bvh8Node* __restrict__ nodes,
-                                               const LeafTri* __restrict__ tris,
-                                               uint32_t num_nodes, uint64_t count,
-                                               uint32_t* fetch, ShortStack<BLOCK>& st,
-                                               Source& src) {
+// least `refill` lanes of the wave are idle, the wave takes that many rays
+// from the queue with one atomic and each idle lane starts one, so lanes do
+// not wait for the longest ray of the wave.  `fetch` counts handed-out rays.
+// Source must provide: Ray load(uint64_t slot, uint32_t* scene),
+// void store(uint64_t slot, Hit).
+template <int BLOCK, class Scenes, class Source>
+__device__ __forceinline__ void traverse_queue_t(Scenes& sc, uint64_t count, uint32_t* fetch,
+                                                 uint32_t refill, ShortStack<BLOCK>& st,
+                                                 Source& src) {
     const uint32_t lane = __lane_id();
     // A short queue is spread over every wave of the grid (a few rays each)
     // instead of filling the first waves: the traversal of a short queue is
@@ -325,7 +354,7 @@ __device__ __forceinline__ void traverse_queue(const Bvh8Node* __restrict__ node
         if (!exhausted) {
             const uint64_t idle = __ballot(!tv.active);
             const uint32_t n_idle = (uint32_t)__popcll(idle);
-            if (n_idle >= min((uint32_t)REFILL, quota)) {
+            if (n_idle >= min(refill, quota)) {
                 const uint32_t take = min(n_idle, quota);
                 const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
                 uint32_t base = 0;
@@ -337,7 +366,10 @@ __device__ __forceinline__ void traverse_queue(const Bvh8Node* __restrict__ node
                     const uint64_t mine = (uint64_t)base + rank;
                     if (mine < count) {
                         slot = mine;
-                        tv.init(src.load(slot), num_nodes);
+                        uint32_t item = 0;
+                        const Ray r = src.load(slot, &item);
+                        sc.bind(item);
+                        tv.init(r, sc.num_nodes());
                         st.sp = 0;
                         if (!tv.active) src.store(slot, tv.h);
                     }
@@ -345,11 +377,33 @@ __device__ __forceinline__ void traverse_queue(const Bvh8Node* __restrict__ node
             }
         }
         if (tv.active) {
-            tv.step<BLOCK>(nodes, tris, st);
+            tv.step<BLOCK>(sc.nodes(), sc.tris(), st);
             if (!tv.active) src.store(slot, tv.h);
         }
         if (exhausted && !__any(tv.active)) break;
     }
+}
+
+// Single-scene form; Source provides Ray load(uint64_t) and store().
+template <class Source>
+struct UntaggedSource {
+    Source& s;
+    __device__ __forceinline__ Ray load(uint64_t slot, uint32_t* item) const {
+        *item = 0;
+        return s.load(slot);
+    }
+    __device__ __forceinline__ void store(uint64_t slot, const Hit& h) const { s.store(slot, h); }
+};
+
+template <int BLOCK, int REFILL, class Source>
+__device__ __forceinline__ void traverse_queue(const Bvh8Node* __restrict__ nodes,
+                                               const LeafTri* __restrict__ tris,
+                                               uint32_t num_nodes, uint64_t count,
+                                               uint32_t* fetch, ShortStack<BLOCK>& st,
+                                               Source& src) {
+    OneScene sc{nodes, tris, num_nodes};
+    UntaggedSource<Source> us{src};
+    traverse_queue_t<BLOCK>(sc, count, fetch, (uint32_t)REFILL, st, us);
 }
 
 }  // namespace lmb

@@ -23,6 +23,7 @@
 #include "dev_common.h"
 #include "dev_shade.h"
 #include "dev_traverse.h"
+#include "lmb_batch.h"
 #include "lmb_frame_dev.h"
 #include "lmb_guided.h"
 
@@ -118,9 +119,12 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(SceneDev s, FrameDev f, Tra
                                       f.counters + kCounterFetch + bounce, st, src);
 }
 
-template <uint32_t Q>
-__global__ __launch_bounds__(kPtBlock) void k_shade_plain(SceneDev s, FrameDev f, TraceParams p,
-                                                          uint32_t bounce) {
+// kTable: the queue's rays carry a scene index (direction .w) into `table`
+// (the batched tracer); otherwise every ray belongs to `s1`.
+template <uint32_t Q, bool kTable>
+__global__ __launch_bounds__(kPtBlock) void k_shade_plain_t(SceneDev s1, const SceneDev* table,
+                                                            FrameDev f, TraceParams p,
+                                                            uint32_t bounce) {
     const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
     const uint32_t count = f.counters[bounce];
@@ -144,6 +148,8 @@ __global__ __launch_bounds__(kPtBlock) void k_shade_plain(SceneDev s, FrameDev f
             V3 scol = v3(ac.x, ac.y, ac.z);
             const V3 dir = v3(d4.x, d4.y, d4.z);
             const V3 org = v3(o4.x, o4.y, o4.z);
+            const uint32_t item = kTable ? __float_as_uint(d4.w) : 0u;
+            const SceneDev& s = kTable ? table[item] : s1;
             if (h.x != kMissPrim) {
                 const float t = __uint_as_float(h.y);
                 const Isect it = make_isect(s, h.x, t, __uint_as_float(h.z),
@@ -169,7 +175,7 @@ __global__ __launch_bounds__(kPtBlock) void k_shade_plain(SceneDev s, FrameDev f
                 if (alive) {
                     cont = true;
                     out_o = make_float4(new_pos.x, new_pos.y, new_pos.z, __uint_as_float(pid));
-                    out_d = make_float4(ndir.x, ndir.y, ndir.z, 0.f);
+                    out_d = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float(item));
                     f.ps_seed[pid] = seed;
                     f.ps_through[pid] = make_float4(color.x, color.y, color.z, picked);
                     f.ps_accum[pid] = make_float4(scol.x, scol.y, scol.z, 0.f);
@@ -247,12 +253,183 @@ static hipError_t launch_plain(const SceneDev& s, const CameraDev& cam, const Fr
             hipLaunchKernelGGL(k_extend, grid, block, 0, stream, s, f, p, b);
             if (tm) tm->end(kKernelExtend, stream);
             if (tm) tm->begin(kKernelShade, stream);
-            hipLaunchKernelGGL(k_shade_plain<Q>, grid, block, 0, stream, s, f, p, b);
+            hipLaunchKernelGGL((k_shade_plain_t<Q, false>), grid, block, 0, stream, s,
+                               (const SceneDev*)nullptr, f, p, b);
             if (tm) tm->end(kKernelShade, stream);
         }
     }
     hipLaunchKernelGGL(k_resolve, grid, block, 0, stream, f, p);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- batches
+// Zeroes the queue and fetch counters (and, at the start of a frame, the
+// frame's vertex counter) with a kernel, so a captured frame needs no memset.
+__global__ __launch_bounds__(kPtBlock) void k_batch_begin(FrameDev f, uint32_t frame_start) {
+    for (uint32_t i = threadIdx.x; i < kNumCounters; i += kPtBlock) {
+        const bool vertices = i >= kCounterVertices && i < kCounterFetch;
+        if (!vertices) f.counters[i] = 0;
+    }
+    if (frame_start && threadIdx.x == 0) *vertex_counter(f) = 0;
+}
+
+__device__ __forceinline__ uint32_t batch_item_of(const BatchDev& b, uint64_t tile) {
+    uint32_t lo = 0, hi = b.num_items - 1;
+    while (lo < hi) {  // last item whose first tile is <= tile
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (b.items[mid].tile_base <= tile)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Primary rays of the tiles [t0, t1): consecutive threads walk a
+// tile_w x tile_h pixel tile, so each wavefront of the queue holds a compact
+// screen tile of one image.  Seeds as general.wesl:516 with the image's own
+// x, y; rays carry their image index in direction .w.
+__global__ __launch_bounds__(kPtBlock) void k_batch_generate(BatchDev b, FrameDev f, TraceParams p,
+                                                             uint32_t sample, uint64_t t0,
+                                                             uint64_t t1) {
+    const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
+    const uint32_t tile_px = b.tile_w * b.tile_h;
+    const uint64_t n = (t1 - t0) * tile_px;
+    const uint64_t n_r = (n + 63) & ~63ull;
+    const uint32_t seed_offset = *b.seed;
+    const uint32_t lane = __lane_id();
+    for (uint64_t g = gtid; g < n_r; g += stride) {
+        bool valid = g < n;
+        uint32_t item = 0, x = 0, y = 0;
+        uint64_t pid = 0;
+        if (valid) {
+            const uint64_t tile = t0 + g / tile_px;
+            const uint32_t within = (uint32_t)(g % tile_px);
+            item = batch_item_of(b, tile);
+            const BatchItemDev& it = b.items[item];
+            const uint64_t lt = tile - it.tile_base;
+            x = (uint32_t)(lt % it.tiles_x) * b.tile_w + within % b.tile_w;
+            y = (uint32_t)(lt / it.tiles_x) * b.tile_h + within / b.tile_w;
+            valid = x < it.width && y < it.height;
+            pid = it.pix_base + (uint64_t)y * it.width + x;
+        }
+        float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4;
+        if (valid) {
+            const BatchItemDev& it = b.items[item];
+            uint32_t seed;
+            if (sample == 0) {
+                seed = rand_u32(x * y) + seed_offset + y;
+                f.pix_sum[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                seed = rand_u32(f.own_seed[pid]);
+            }
+            f.own_seed[pid] = seed;
+            float o[3], d[3];
+            create_ray(it.cam, x, y, it.width, it.height, o, d);
+            o4 = make_float4(o[0], o[1], o[2], __uint_as_float((uint32_t)pid));
+            d4 = make_float4(d[0], d[1], d[2], __uint_as_float(item));
+            f.ps_seed[pid] = seed;
+            f.ps_through[pid] = make_float4(1.f, 1.f, 1.f, -1.f);
+            f.ps_accum[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        const uint64_t mask = __ballot(valid);
+        if (mask) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&f.counters[0], (uint32_t)__popcll(mask));
+            base = __shfl(base, (int)leader);
+            if (valid) {
+                const uint64_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+                f.queue[0][2 * slot] = o4;
+                f.queue[0][2 * slot + 1] = d4;
+            }
+        }
+    }
+}
+
+// Queue rays tagged with their image index (direction .w).
+struct TaggedQueueSource {
+    const float4* __restrict__ q;
+    uint4* __restrict__ hits;
+    float tmin, tmax;
+    __device__ __forceinline__ Ray load(uint64_t slot, uint32_t* item) const {
+        const float4 o = q[2 * slot];
+        const float4 d = q[2 * slot + 1];
+        *item = __float_as_uint(d.w);
+        Ray r;
+        r.ox = o.x;
+        r.oy = o.y;
+        r.oz = o.z;
+        r.tmin = tmin;
+        r.dx = d.x;
+        r.dy = d.y;
+        r.dz = d.z;
+        r.tmax = tmax;
+        return r;
+    }
+    __device__ __forceinline__ void store(uint64_t slot, const Hit& h) const {
+        hits[slot] = make_uint4(h.leaf, __float_as_uint(h.t), __float_as_uint(h.u),
+                                __float_as_uint(h.v));
+    }
+};
+
+__global__ __launch_bounds__(kPtBlock) void k_batch_extend(const SceneDev* table, FrameDev f,
+                                                           TraceParams p, uint32_t bounce,
+                                                           uint32_t refill) {
+    __shared__ uint2 lds_stack[kStackLds * kPtBlock];
+    const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
+    const uint32_t count = f.counters[bounce];
+    if (gtid == 0) {
+        atomicAdd((unsigned long long*)vertex_counter(f), (unsigned long long)count);
+        atomicAdd((unsigned long long*)(f.counters + kCounterVerticesTotal),
+                  (unsigned long long)count);
+    }
+    ShortStack<kPtBlock> st;
+    st.lds = lds_stack + threadIdx.x;
+    st.spill = f.spill + gtid * (kStackMax - kStackLds);
+    SceneTable sc{table};
+    TaggedQueueSource src{f.queue[bounce & 1], f.hits, p.tmin, p.tmax};
+    traverse_queue_t<kPtBlock>(sc, count, f.counters + kCounterFetch + bounce, refill, st, src);
+}
+
+template <uint32_t Q>
+static hipError_t launch_batch_q(const BatchDev& b, const FrameDev& f, const TraceParams& p,
+                                 const BatchLaunch& l, hipStream_t stream) {
+    const dim3 grid(l.blocks), block(kPtBlock);
+    const SceneDev none{};
+    for (uint64_t t0 = 0; t0 < b.tiles_total; t0 += l.pass_tiles) {
+        const uint64_t t1 = t0 + l.pass_tiles < b.tiles_total ? t0 + l.pass_tiles : b.tiles_total;
+        for (uint32_t sample = 0; sample < p.samples; ++sample) {
+            hipLaunchKernelGGL(k_batch_begin, dim3(1), block, 0, stream, f,
+                               (uint32_t)(t0 == 0 && sample == 0));
+            hipLaunchKernelGGL(k_batch_generate, grid, block, 0, stream, b, f, p, sample, t0, t1);
+            for (uint32_t bn = 0; bn < p.bounces; ++bn) {
+                hipLaunchKernelGGL(k_batch_extend, grid, block, 0, stream, b.scenes, f, p, bn,
+                                   l.refill);
+                hipLaunchKernelGGL((k_shade_plain_t<Q, true>), grid, block, 0, stream, none,
+                                   b.scenes, f, p, bn);
+            }
+        }
+    }
+    hipLaunchKernelGGL(k_resolve, grid, block, 0, stream, f, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(const BatchDev& b, const FrameDev& f, const TraceParams& p,
+                        const BatchLaunch& l, hipStream_t stream) {
+    if (b.num_items == 0 || b.tiles_total == 0 || l.pass_tiles == 0 || l.blocks == 0)
+        return hipSuccess;
+    switch (p.quality) {
+        case kQualityLow:
+            return launch_batch_q<kQualityLow>(b, f, p, l, stream);
+        case kQualityHigh:
+            return launch_batch_q<kQualityHigh>(b, f, p, l, stream);
+        case kQualitySpectral:
+            return launch_batch_q<kQualitySpectral>(b, f, p, l, stream);
+        default:
+            return launch_batch_q<kQualityMedium>(b, f, p, l, stream);
+    }
 }
 
 hipError_t launch_trace(const SceneDev& s, const CameraDev& cam, const FrameDev& f,

@@ -193,10 +193,12 @@ def procedural_materials(kinds: np.ndarray, dispersive: Optional[np.ndarray] = N
 
 
 def procedural(n_tri: int = 1_000_000, seed: int = 2, width: int = 1920,
-               height: int = 1080, dispersive: float = 0.0) -> SceneSpec:
+               height: int = 1080, dispersive: float = 0.0,
+               mix: tuple = (0.6, 0.3, 0.1)) -> SceneSpec:
     """``dispersive``: fraction of the dielectrics given the Cauchy IOR range
     (A = 1.5, B = 0.004 um^2); drawn after every other random number, so the
-    geometry and the other materials do not depend on it."""
+    geometry and the other materials do not depend on it.  ``mix``: fractions
+    of diffuse, GGX metal and dielectric triangles."""
     if n_tri > ATLAS * ATLAS:
         raise ValueError("at most 2^20 triangles fit the material atlases")
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -208,7 +210,8 @@ def procedural(n_tri: int = 1_000_000, seed: int = 2, width: int = 1920,
     v = np.zeros((n_tri * 3, 4), np.float32)
     v[:, :3] = tri.reshape(-1, 3)
     u = rng.uniform(size=n_tri)
-    kinds = np.where(u < 0.6, 0, np.where(u < 0.9, 1, 2)).astype(np.uint32)
+    c0, c1 = mix[0], mix[0] + mix[1]
+    kinds = np.where(u < c0, 0, np.where(u < c1, 1, 2)).astype(np.uint32)
     albedo = rng.uniform(0.1, 0.9, size=(n_tri, 3))
     rough = rng.uniform(0.05, 1.0, size=n_tri)
     recolour = np.zeros((ATLAS * ATLAS, 4), np.uint8)
@@ -228,6 +231,23 @@ def procedural(n_tri: int = 1_000_000, seed: int = 2, width: int = 1920,
     return SceneSpec(v, None, np.arange(n_tri, dtype=np.uint32), mats, [_rgba(1, 1, 1)], [],
                      [attributes.reshape(ATLAS, ATLAS, 4)], recolour.reshape(ATLAS, ATLAS, 4),
                      api.uniform_background(1.0), proj, view, width, height)
+
+
+BATCH_RESOLUTIONS = (128, 256, 512, 1024)
+
+
+def batch_workload(n_scenes: int = 64, seed: int = 4, tri_min: int = 1_000,
+                   tri_max: int = 500_000, resolutions=BATCH_RESOLUTIONS) -> list:
+    """Uneven independent scenes for the batched tracer: triangle counts
+    log-uniform in [tri_min, tri_max], square resolutions drawn uniformly from
+    ``resolutions``, 2/3 diffuse and 1/3 GGX metal triangles (no
+    dielectrics), each a procedural scene of its own seed."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    counts = np.exp(rng.uniform(np.log(tri_min), np.log(tri_max), n_scenes)).astype(np.int64)
+    res = rng.choice(np.asarray(resolutions), n_scenes)
+    return [procedural(int(c), seed=seed * 1000 + i, width=int(r), height=int(r),
+                       mix=(2.0 / 3.0, 1.0 / 3.0, 0.0))
+            for i, (c, r) in enumerate(zip(counts, res))]
 
 
 def furnace_box(albedo: float, emission: float, ty: api.MaterialType = api.MaterialType.Diffuse,
