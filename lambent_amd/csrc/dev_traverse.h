@@ -178,9 +178,14 @@ struct Traversal {
         active = num_nodes != 0;
     }
 
-    template <int BLOCK>
+    // kTriangles: test the hit leaves' triangles here (one lane alone);
+    // otherwise hand them out as (*tri_base_out, *tri_mask_out) for the
+    // wave's shared triangle phase (shared_triangles).
+    template <int BLOCK, bool kTriangles = true>
     __device__ __forceinline__ void step(const Bvh8Node* __restrict__ nodes,
-                                         const LeafTri* __restrict__ tris, ShortStack<BLOCK>& st) {
+                                         const LeafTri* __restrict__ tris, ShortStack<BLOCK>& st,
+                                         uint32_t* tri_base_out = nullptr,
+                                         uint32_t* tri_mask_out = nullptr) {
         const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
         const uint4 w0 = np[0];
         const uint4 w1 = np[1];
@@ -248,6 +253,11 @@ struct Traversal {
                 }
             }
         }
+        if (!kTriangles) {
+            *tri_base_out = tri_base;
+            *tri_mask_out = tri_hits;
+            tri_hits = 0;
+        }
         while (tri_hits) {
             const uint32_t k = __builtin_ctz(tri_hits);
             tri_hits &= tri_hits - 1u;
@@ -313,6 +323,8 @@ struct OneScene {
     __device__ __forceinline__ const Bvh8Node* nodes() const { return nodes_; }
     __device__ __forceinline__ const LeafTri* tris() const { return tris_; }
     __device__ __forceinline__ uint32_t num_nodes() const { return num_nodes_; }
+    // triangle array of the ray of lane `owner` (all lanes call this)
+    __device__ __forceinline__ const LeafTri* tris_of(uint32_t) const { return tris_; }
 };
 
 struct SceneTable {
@@ -328,7 +340,151 @@ struct SceneTable {
     __device__ __forceinline__ const Bvh8Node* nodes() const { return nodes_; }
     __device__ __forceinline__ const LeafTri* tris() const { return tris_; }
     __device__ __forceinline__ uint32_t num_nodes() const { return num_nodes_; }
+    // Every lane binds its own scene: the owner's triangle array comes from
+    // the owner lane (all lanes call this).
+    __device__ __forceinline__ const LeafTri* tris_of(uint32_t owner) const {
+        const uint64_t p = reinterpret_cast<uint64_t>(tris_);
+        const uint32_t lo = __shfl((uint32_t)p, (int)owner);
+        const uint32_t hi = __shfl((uint32_t)(p >> 32), (int)owner);
+        return reinterpret_cast<const LeafTri*>(((uint64_t)hi << 32) | lo);
+    }
 };
+
+// Per-wave LDS of the shared triangle phase: 64 entries each (one per lane).
+struct TriShare {
+    unsigned long long* key;  // (orderable t << 32) | primitive id of the best hit
+    float2* uv;
+    uint32_t* leaf;
+};
+
+// Order-preserving map of floats to unsigned integers (NaN excluded).
+__device__ __forceinline__ uint32_t orderable(float t) {
+    const uint32_t u = __float_as_uint(t);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float from_orderable(uint32_t o) {
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
+// Position of the j-th set bit (j from 0) of a 24-bit mask.
+__device__ __forceinline__ uint32_t nth_set_bit(uint32_t m, uint32_t j) {
+    uint32_t b = 0, c;
+    c = __popc(m & 0xFFu);
+    if (j >= c) { j -= c; b = 8; }
+    c = __popc((m >> b) & 0xFFu);
+    if (j >= c) { j -= c; b += 8; }
+    c = __popc((m >> b) & 0xFu);
+    if (j >= c) { j -= c; b += 4; }
+    c = __popc((m >> b) & 0x3u);
+    if (j >= c) { j -= c; b += 2; }
+    c = (m >> b) & 1u;
+    if (j >= c) b += 1;
+    return b;
+}
+
+// Shared triangle phase, called by all 64 lanes of a wave.  Lane l owns the
+// triangles tbase_l + bit of tmask_l; the wave's (owner, triangle) pairs are
+// spread over all lanes, 64 per round, each tested against its owner's ray
+// (fetched by lane shuffles).  Per owner the closest hit is the minimum of
+// (t, primitive id) - the order the one-lane loop applies (nearer t first,
+// then the lower primitive id) - taken with a 64-bit LDS atomicMin; the lane
+// holding the winning triangle then writes its (u, v) and leaf index.
+template <class Scenes>
+__device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t tbase, uint32_t tmask,
+                                                 const Scenes& sc, const TriShare& ts,
+                                                 uint32_t lane) {
+    const uint32_t c = __popc(tmask);
+    if (!__ballot(c != 0u)) return;
+    if (!__ballot(c > 1u)) {
+        // at most one triangle per lane: each lane tests its own in place
+        if (c) {
+            const uint32_t leaf = tbase + (uint32_t)__builtin_ctz(tmask);
+            const float4* tp = reinterpret_cast<const float4*>(sc.tris() + leaf);
+            const float4 a = tp[0], b = tp[1], cc = tp[2];
+            float t, u, v;
+            if (intersect_triangle(tv.sh, tv.r.ox, tv.r.oy, tv.r.oz, a, b, cc, tv.r.tmin, tv.h.t,
+                                   &t, &u, &v)) {
+                const uint32_t prim = __float_as_uint(a.w);
+                if (t < tv.h.t || prim < tv.h.prim) {
+                    tv.h.t = t;
+                    tv.h.u = u;
+                    tv.h.v = v;
+                    tv.h.prim = prim;
+                    tv.h.leaf = leaf;
+                }
+            }
+        }
+        return;
+    }
+    uint32_t incl = c;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d);
+        if (lane >= d) incl += y;
+    }
+    const uint32_t off = incl - c;
+    const uint32_t total = __shfl(incl, 63);
+    const unsigned long long init = ((unsigned long long)orderable(tv.h.t) << 32) | tv.h.prim;
+    ts.key[lane] = init;
+    const uint32_t kxyz = (uint32_t)tv.sh.kx | ((uint32_t)tv.sh.ky << 2) | ((uint32_t)tv.sh.kz << 4);
+    for (uint32_t r0 = 0; r0 < total; r0 += 64) {
+        const uint32_t i = r0 + lane;
+        uint32_t owner = 0;
+#pragma unroll
+        for (uint32_t stp = 32; stp; stp >>= 1) {
+            const uint32_t m = owner + stp;
+            if (__shfl(off, (int)m) <= i) owner = m;
+        }
+        const uint32_t o_mask = __shfl(tmask, (int)owner);
+        const uint32_t o_base = __shfl(tbase, (int)owner);
+        const uint32_t o_off = __shfl(off, (int)owner);
+        const float ox = __shfl(tv.r.ox, (int)owner);
+        const float oy = __shfl(tv.r.oy, (int)owner);
+        const float oz = __shfl(tv.r.oz, (int)owner);
+        const float otmin = __shfl(tv.r.tmin, (int)owner);
+        Shear sh;
+        sh.sx = __shfl(tv.sh.sx, (int)owner);
+        sh.sy = __shfl(tv.sh.sy, (int)owner);
+        sh.sz = __shfl(tv.sh.sz, (int)owner);
+        const uint32_t ok = __shfl(kxyz, (int)owner);
+        const LeafTri* __restrict__ tris = sc.tris_of(owner);
+        sh.kx = (int)(ok & 3u);
+        sh.ky = (int)((ok >> 2) & 3u);
+        sh.kz = (int)((ok >> 4) & 3u);
+        bool hit = false;
+        unsigned long long mykey = 0;
+        float t = 0.f, u = 0.f, v = 0.f;
+        uint32_t leaf = 0;
+        if (i < total) {
+            leaf = o_base + nth_set_bit(o_mask, i - o_off);
+            const float4* tp = reinterpret_cast<const float4*>(tris + leaf);
+            const float4 a = tp[0], b = tp[1], cc = tp[2];
+            const float tmax = from_orderable((uint32_t)(ts.key[owner] >> 32));
+            if (intersect_triangle(sh, ox, oy, oz, a, b, cc, otmin, tmax, &t, &u, &v)) {
+                hit = true;
+                mykey = ((unsigned long long)orderable(t) << 32) | __float_as_uint(a.w);
+                atomicMin(ts.key + owner, mykey);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (hit && ts.key[owner] == mykey) {
+            ts.uv[owner] = make_float2(u, v);
+            ts.leaf[owner] = leaf;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    const unsigned long long k = ts.key[lane];
+    if (k != init) {
+        tv.h.t = from_orderable((uint32_t)(k >> 32));
+        tv.h.prim = (uint32_t)k;
+        const float2 uv = ts.uv[lane];
+        tv.h.u = uv.x;
+        tv.h.v = uv.y;
+        tv.h.leaf = ts.leaf[lane];
+    }
+}
 
 // Persistent traversal over a ray queue with wave-level refill: whenever at
 // least `refill` lanes of the wave are idle, the wave takes that many rays
@@ -339,7 +495,7 @@ struct SceneTable {
 template <int BLOCK, class Scenes, class Source>
 __device__ __forceinline__ void traverse_queue_t(Scenes& sc, uint64_t count, uint32_t* fetch,
                                                  uint32_t refill, ShortStack<BLOCK>& st,
-                                                 Source& src) {
+                                                 const TriShare& ts, Source& src) {
     const uint32_t lane = __lane_id();
     // A short queue is spread over every wave of the grid (a few rays each)
     // instead of filling the first waves: the traversal of a short queue is
@@ -376,10 +532,11 @@ __device__ __forceinline__ void traverse_queue_t(Scenes& sc, uint64_t count, uin
                 }
             }
         }
-        if (tv.active) {
-            tv.step<BLOCK>(sc.nodes(), sc.tris(), st);
-            if (!tv.active) src.store(slot, tv.h);
-        }
+        const bool was_active = tv.active;
+        uint32_t tbase = 0, tmask = 0;
+        if (was_active) tv.step<BLOCK, false>(sc.nodes(), sc.tris(), st, &tbase, &tmask);
+        shared_triangles(tv, tbase, tmask, sc, ts, lane);
+        if (was_active && !tv.active) src.store(slot, tv.h);
         if (exhausted && !__any(tv.active)) break;
     }
 }
@@ -400,10 +557,19 @@ __device__ __forceinline__ void traverse_queue(const Bvh8Node* __restrict__ node
                                                const LeafTri* __restrict__ tris,
                                                uint32_t num_nodes, uint64_t count,
                                                uint32_t* fetch, ShortStack<BLOCK>& st,
-                                               Source& src) {
+                                               const TriShare& ts, Source& src) {
     OneScene sc{nodes, tris, num_nodes};
     UntaggedSource<Source> us{src};
-    traverse_queue_t<BLOCK>(sc, count, fetch, (uint32_t)REFILL, st, us);
+    traverse_queue_t<BLOCK>(sc, count, fetch, (uint32_t)REFILL, st, ts, us);
 }
+
+// LDS of the shared triangle phase for a BLOCK-thread block, and this
+// thread's wave slice of it.
+#define LMB_TRI_SHARE(BLOCK, name)                                                   \
+    __shared__ unsigned long long name##_key[BLOCK];                                 \
+    __shared__ float2 name##_uv[BLOCK];                                              \
+    __shared__ uint32_t name##_leaf[BLOCK];                                          \
+    const TriShare name{name##_key + (threadIdx.x & ~63u), name##_uv + (threadIdx.x & ~63u), \
+                        name##_leaf + (threadIdx.x & ~63u)}
 
 }  // namespace lmb

@@ -101,7 +101,12 @@ struct QueueSource {
 
 constexpr int kRefill = 16;
 
-__global__ __launch_bounds__(kPtBlock) void k_extend(SceneDev s, FrameDev f, TraceParams p,
+// Minimum waves per SIMD the traversal kernels are compiled for (register cap).
+#ifndef LMB_EXTEND_WAVES
+#define LMB_EXTEND_WAVES 1
+#endif
+
+__global__ __launch_bounds__(kPtBlock, LMB_EXTEND_WAVES) void k_extend(SceneDev s, FrameDev f, TraceParams p,
                                                      uint32_t bounce) {
     __shared__ uint2 lds_stack[kStackLds * kPtBlock];
     const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
@@ -114,9 +119,10 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(SceneDev s, FrameDev f, Tra
     ShortStack<kPtBlock> st;
     st.lds = lds_stack + threadIdx.x;
     st.spill = f.spill + gtid * (kStackMax - kStackLds);
+    LMB_TRI_SHARE(kPtBlock, ts);
     QueueSource src{f.queue[bounce & 1], f.hits, p.tmin, p.tmax};
     traverse_queue<kPtBlock, kRefill>(s.nodes, s.tris, s.num_nodes, count,
-                                      f.counters + kCounterFetch + bounce, st, src);
+                                      f.counters + kCounterFetch + bounce, st, ts, src);
 }
 
 // kTable: the queue's rays carry a scene index (direction .w) into `table`
@@ -374,7 +380,7 @@ struct TaggedQueueSource {
     }
 };
 
-__global__ __launch_bounds__(kPtBlock) void k_batch_extend(const SceneDev* table, FrameDev f,
+__global__ __launch_bounds__(kPtBlock, LMB_EXTEND_WAVES) void k_batch_extend(const SceneDev* table, FrameDev f,
                                                            TraceParams p, uint32_t bounce,
                                                            uint32_t refill) {
     __shared__ uint2 lds_stack[kStackLds * kPtBlock];
@@ -388,9 +394,11 @@ __global__ __launch_bounds__(kPtBlock) void k_batch_extend(const SceneDev* table
     ShortStack<kPtBlock> st;
     st.lds = lds_stack + threadIdx.x;
     st.spill = f.spill + gtid * (kStackMax - kStackLds);
+    LMB_TRI_SHARE(kPtBlock, ts);
     SceneTable sc{table};
     TaggedQueueSource src{f.queue[bounce & 1], f.hits, p.tmin, p.tmax};
-    traverse_queue_t<kPtBlock>(sc, count, f.counters + kCounterFetch + bounce, refill, st, src);
+    traverse_queue_t<kPtBlock>(sc, count, f.counters + kCounterFetch + bounce, refill, st, ts,
+                               src);
 }
 
 template <uint32_t Q>

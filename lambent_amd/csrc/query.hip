@@ -83,8 +83,9 @@ __global__ __launch_bounds__(kQueryBlock) void k_intersect(SceneDev s, const Ray
     ShortStack<kQueryBlock> st;
     st.lds = lds_stack + threadIdx.x;
     st.spill = spill + gtid * (kStackMax - kStackLds);
+    LMB_TRI_SHARE(kQueryBlock, ts);
     RaySource src{rays, hits};
-    traverse_queue<kQueryBlock, kQueryRefill>(s.nodes, s.tris, s.num_nodes, n, fetch, st, src);
+    traverse_queue<kQueryBlock, kQueryRefill>(s.nodes, s.tris, s.num_nodes, n, fetch, st, ts, src);
 }
 
 __global__ __launch_bounds__(kQueryBlock) void k_primary(SceneDev s, CameraDev cam,
@@ -99,9 +100,10 @@ __global__ __launch_bounds__(kQueryBlock) void k_primary(SceneDev s, CameraDev c
     ShortStack<kQueryBlock> st;
     st.lds = lds_stack + threadIdx.x;
     st.spill = spill + gtid * (kStackMax - kStackLds);
+    LMB_TRI_SHARE(kQueryBlock, ts);
     CameraSource src{cam, width, height, tmin, tmax, prim_out, t_out};
     traverse_queue<kQueryBlock, kQueryRefill>(s.nodes, s.tris, s.num_nodes,
-                                              (uint64_t)width * height, fetch, st, src);
+                                              (uint64_t)width * height, fetch, st, ts, src);
 }
 
 hipError_t launch_intersect(const SceneDev& s, const void* rays, uint64_t n, void* hits,
