@@ -313,6 +313,14 @@ lmb_status lmb_scene_create(const lmb_scene_desc* desc, lmb_scene* out) {
         return fail(LMB_ERR_BVH_TOO_DEEP, "BVH depth " + std::to_string(bvh.max_depth) +
                                               " exceeds the traversal stack (" +
                                               std::to_string(kStackMax) + ")");
+    // The material index of each triangle rides in its leaf record (v1.w), so
+    // shading needs no separate material_idx lookup.
+    for (LeafTri& lt : bvh.tris) {
+        uint32_t id;
+        std::memcpy(&id, &lt.v0[3], 4);
+        const uint32_t mat = desc->material_idx[id];
+        std::memcpy(&lt.v1[3], &mat, 4);
+    }
     lmb_scene s = new lmb_scene_s();
     lmb_status st = scene_upload(s, desc, ntri, bvh);
     if (st) {

@@ -5,7 +5,8 @@
 // 1. Binned-SAH binary BVH (16 bins per axis, leaves of <= 3 triangles),
 //    subtrees above 64k primitives built on separate threads.
 // 2. Collapse to 8-wide: repeatedly open the child of largest surface area
-//    until the node has 8 children or only leaves remain.
+//    (weighted by 1 + the triangles of its leaf children) until the node has
+//    8 children or only leaves remain.
 // 3. Slot assignment by direction octant (greedy maximum alignment of the
 //    child centroid offset with the slot's octant direction).
 // 4. 8-bit quantization of the child boxes against a per-node origin that is
@@ -278,7 +279,13 @@ int build_bvh8(const float* vertices, uint64_t num_vertices, const uint32_t* ind
                 for (int i = 0; i < nch; ++i) {
                     const BNode& c = bn[ch[i]];
                     if (c.count > 0) continue;
-                    float a = half_area(c.lo, c.hi);
+                    // Opening a node whose children are leaves lifts their
+                    // triangles into this node (no separate visit): weight its
+                    // area by 1 + the triangles of its leaf children.  Fuller
+                    // nodes (about a third fewer on triangle soups) and fewer
+                    // visits per ray than opening by area alone.
+                    const float a = half_area(c.lo, c.hi) *
+                                    (float)(1u + bn[c.left].count + bn[c.left + 1].count);
                     if (a > best) {
                         best = a;
                         pick = i;

@@ -165,7 +165,7 @@ __device__ __forceinline__ Isect make_isect(const SceneDev& s, uint32_t leaf, fl
     it.t = t;
     const V3 v0 = v3(a.x, a.y, a.z), v1 = v3(b.x, b.y, b.z), v2 = v3(c.x, c.y, c.z);
     it.front_face = dot3(dir, cross3(v1 - v0, v2 - v0)) < 0.0f;
-    const Material m = s.materials[s.prim_material[prim]];
+    const Material m = s.materials[__float_as_uint(b.w)];  // material index (leaf record v1.w)
     const uint32_t idx_diffuse = m.tex_idx_diffuse_emission & 0xFFFFu;
     const uint32_t idx_emission = m.tex_idx_diffuse_emission >> 16;
     const uint32_t idx_attr = m.tex_idx_attributes_ty & 0xFFFFu;

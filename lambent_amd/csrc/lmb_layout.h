@@ -47,8 +47,9 @@ constexpr uint8_t kMetaInternal = 0xFF;
 constexpr int kStackLds = 8;
 constexpr int kStackMax = 48;
 
-// Triangle in BVH leaf order: three float4, the w of the first holds the
-// original primitive index bits.  48 bytes.
+// Triangle in BVH leaf order: three float4; v0.w holds the original
+// primitive index bits, v1.w its material index (bindings.wesl:59-61).
+// 48 bytes.
 struct alignas(16) LeafTri {
     float v0[4];
     float v1[4];
