@@ -430,6 +430,12 @@ def look_to_rh(eye, direction, up) -> np.ndarray:
     return m
 
 
+def look_at_rh(eye, centre, up) -> np.ndarray:
+    """cgmath::Matrix4::look_at_rh: look_to_rh towards centre - eye."""
+    e = np.asarray(eye, np.float32)
+    return look_to_rh(e, np.asarray(centre, np.float32) - e, up)
+
+
 @dataclass(frozen=True)
 class DispatchSize:
     width: int

@@ -326,3 +326,25 @@ def test_high_quality_colour_matches_float64_cie_fit():
     # normalisation constants are the reciprocals of the integrals)
     cols = np.array([oracle.high_colour(s)[0] for s in range(20000)])
     np.testing.assert_allclose(cols.mean(axis=0), 1.0, atol=0.03)
+
+
+def test_reference_test_scene_fixtures_decode():
+    """The copied PNG fixtures of the reference's renderer test decode to
+    64 x 64 RGBA8 through the TextureLoader path, and the oracle renders the
+    scene: every primary ray hits one of its four cube faces (the camera is
+    inside the cube), and the albedo output holds only texel values."""
+    import ref_scene
+    from lambent_amd import api
+    loader = api.TextureLoader()
+    for name in ("diffuse.png", "pure_white.png", "emission_partial.png", "emission.png",
+                 "attributes.png", "plus_x.png", "minus_z.png"):
+        loader.load(str(ref_scene.TEX / name))
+        assert loader.images[-1].shape == (64, 64, 4)
+    spec = ref_scene.reference_test_scene(64, 36)
+    c, n, a, v = oracle.Scene(spec).trace_plain(spec.camera().to_bytes(), 64, 36, quality=1,
+                                                samples=2, bounces=5)
+    assert (a[..., 3] > 0).all()
+    texels = {tuple(t) for im in spec.diffuse for t in im.reshape(-1, 4)[:, :3]}
+    for px in np.unique(a[..., :3].reshape(-1, 3), axis=0):
+        assert tuple(np.round(px * 255).astype(np.uint8)) in texels
+    assert np.isfinite(c).all() and v >= 64 * 36 * 2

@@ -5,12 +5,15 @@
 * same seeds: the GPU image and the oracle image of the Cornell box agree per
   pixel except where a transcendental-function ulp flips a branch;
 * converged images: relMSE against the oracle's float64-accumulated image;
-* Russian roulette is unbiased (mean radiance within the Monte-Carlo error).
+* Russian roulette is unbiased (mean radiance within the Monte-Carlo error);
+* the reference's own test scene (its PNG textures and sky) under every
+  quality, same seeds as the oracle.
 """
 import numpy as np
 import pytest
 
 import oracle
+import ref_scene
 from lambent_amd import api, scenes
 
 pytestmark = pytest.mark.gpu
@@ -126,3 +129,77 @@ def test_russian_roulette_unbiased():
     assert v1 < v0
     m0, m1 = plain[..., :3].mean(), rr[..., :3].mean()
     assert abs(m1 - m0) / m0 < 0.02, (m0, m1)
+
+
+def _textured_quad(width=96, height=96):
+    """A unit quad facing the camera with per-vertex UVs (two materials, one
+    UV triple per triangle, lib.rs:132-143) and a random 16 x 16 diffuse
+    texture, plus an emissive textured ceiling quad; nearest sampling with
+    clamping (textures.rs default sampler)."""
+    rng = np.random.default_rng(21)
+    tex = rng.integers(0, 256, (16, 16, 4), dtype=np.uint8)
+    tex[..., 3] = 255
+    emit = rng.integers(0, 256, (4, 4, 4), dtype=np.uint8)
+    emit[..., 3] = 255
+    v = np.array([[0, 0, 1, 1], [1, 0, 1, 1], [1, 1, 1, 1], [0, 1, 1, 1],
+                  [0, 1.2, 0, 1], [1, 1.2, 0, 1], [1, 1.2, 1, 1], [0, 1.2, 1, 1]], np.float32)
+    idx = np.array([0, 1, 2, 0, 2, 3, 4, 5, 6, 4, 6, 7], np.uint32)
+    D = api.MaterialType.Diffuse
+    m0 = api.Material.new([0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0, 0], 0, None, None,
+                          api.EmissionScale.from_emission_scale(0.0), None, D)
+    m1 = api.Material.new([0.0, 0.0], [1.0, 1.0], [0.0, 1.0], [0, 0], 0, None, None,
+                          api.EmissionScale.from_emission_scale(0.0), None, D)
+    light = api.Material.new([0.1, 0.1], [0.9, 0.1], [0.9, 0.9], [0, 0], 1, 0, None,
+                             api.EmissionScale.from_emission_scale(5.0), None, D)
+    light2 = api.Material.new([0.1, 0.1], [0.9, 0.9], [0.1, 0.9], [0, 0], 1, 0, None,
+                              api.EmissionScale.from_emission_scale(5.0), None, D)
+    proj = api.perspective(50.0, width / height, 0.1, 100.0)
+    view = api.look_to_rh((0.5, 0.5, -0.6), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0))
+    return scenes.SceneSpec(v, idx, np.array([0, 1, 2, 3], np.uint32),
+                            api.materials_array([m0, m1, light, light2]),
+                            [tex, scenes._rgba(1, 1, 1)], [emit], [], None,
+                            api.uniform_background(0.2), proj, view, width, height)
+
+
+def test_textured_quad_same_seeds_as_oracle():
+    spec = _textured_quad()
+    color, normal, albedo, verts = _trace(spec, api.Medium, samples=2, bounces=4)
+    c_o, n_o, a_o, v_o = oracle.Scene(spec).trace_plain(
+        spec.camera().to_bytes(), spec.width, spec.height, quality=1, samples=2, bounces=4)
+    # the albedo output is the texel under the barycentric UV: exact
+    np.testing.assert_array_equal(albedo, a_o)
+    hit = albedo[..., 3] > 0
+    assert len(np.unique(albedo[hit][:, :3], axis=0)) > 100  # many different texels seen
+    close = np.all(np.abs(color - c_o) <= 1e-4 * (1 + np.abs(c_o)), axis=-1)
+    assert close.mean() > 0.97, close.mean()
+    assert abs(verts - v_o) <= 0.01 * v_o
+
+
+@pytest.mark.parametrize("quality", [api.Low, api.Medium, api.High, api.Spectral])
+@pytest.mark.parametrize("frame", [0, 5])
+def test_reference_test_scene_same_seeds_as_oracle(quality, frame):
+    """The reference's own renderer-test scene (src/tests/mod.rs): textured
+    diffuse, emissive, metallic and transparent faces under a PNG cube-map
+    sky, 4 samples, the frame number as the seed."""
+    spec = ref_scene.reference_test_scene(320, 180)
+    color, normal, albedo, verts = _trace(spec, quality, samples=4, bounces=5, seed_offset=frame)
+    c_o, n_o, a_o, v_o = oracle.Scene(spec).trace_plain(
+        spec.camera().to_bytes(), 320, 180, quality=int(quality), samples=4, bounces=5,
+        seed_offset=frame)
+    np.testing.assert_array_equal(albedo, a_o)
+    np.testing.assert_allclose(normal, n_o, atol=1e-6)
+    close = np.all(np.abs(color - c_o) <= 1e-4 * (1 + np.abs(c_o)), axis=-1)
+    assert close.mean() > 0.97, f"only {close.mean():.4f} of pixels match the oracle"
+    assert abs(verts - v_o) <= 0.01 * v_o
+
+
+def test_reference_test_scene_full_size():
+    """Same scene at the reference test's 1280 x 720, one sample."""
+    spec = ref_scene.reference_test_scene(1280, 720)
+    color, normal, albedo, verts = _trace(spec, api.Medium, samples=1, bounces=5, seed_offset=1)
+    c_o, n_o, a_o, v_o = oracle.Scene(spec).trace_plain(
+        spec.camera().to_bytes(), 1280, 720, quality=1, samples=1, bounces=5, seed_offset=1)
+    np.testing.assert_array_equal(albedo, a_o)
+    close = np.all(np.abs(color - c_o) <= 1e-4 * (1 + np.abs(c_o)), axis=-1)
+    assert close.mean() > 0.97, f"only {close.mean():.4f} of pixels match the oracle"
+    assert abs(verts - v_o) <= 0.01 * v_o
