@@ -80,7 +80,8 @@ def rel_mse(img, ref) -> float:
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
 
 
-def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int) -> dict:
+def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
+              lr: float = 1e-3) -> dict:
     """Time-to-relMSE of the cached tracer against the plain tracer, both with
     8-bounce paths (RR from bounce 3), judged against a plain reference of
     ref_spp samples accumulated in fp64.  Times are GPU wall times of the
@@ -113,7 +114,7 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int) -> di
         plain_pts.append({"spp": spp, "seconds": dt, "relmse": rel_mse(img, ref)})
         log(f"cache leg: plain {spp} spp {dt * 1e3:.1f} ms relMSE {plain_pts[-1]['relmse']:.5f}")
 
-    cache = api.NeuralCache(api.NeuralCacheOpts(learning_rate=3e-3))
+    cache = api.NeuralCache(api.NeuralCacheOpts(learning_rate=lr))
     pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=1, **popts))
     b = api.DataBuffers(W, H, 1, api.BufferType(0))
     b.set_profiling(True)
@@ -153,7 +154,8 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int) -> di
     train_tf = train_flops / (ks["cache_train_ms"] * 1e-3) / 1e12 if ks["cache_train_ms"] else 0.0
     return {
         "workload": f"procedural {W}x{H}, 8 bounces, RR from 3; cache: paths end at the second "
-                    f"vertex, 65536 training paths and 4 Adam steps (lr 3e-3) per frame, "
+                    f"vertex (past specular scatters: the vertex after the first rough one), "
+                    f"65536 training paths and 4 Adam steps (lr {lr:g}) per frame, "
                     f"{warm} warm-up frames (timed, not accumulated)",
         "reference": f"plain tracer {ref_spp} spp, fp64 accumulation of {chunk}-spp dispatches",
         "plain": plain_pts,
@@ -330,6 +332,7 @@ def main() -> int:
     ap.add_argument("--ref-spp", type=int, default=4096)
     ap.add_argument("--cache-frames", type=int, default=512)
     ap.add_argument("--cache-warmup", type=int, default=64)
+    ap.add_argument("--cache-lr", type=float, default=1e-3)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (rocprofv3 counter calibration)")
     args = ap.parse_args()
@@ -431,7 +434,7 @@ def main() -> int:
             log(f"cpu baseline failed: {exc!r}")
     if not args.no_cache_leg:
         out["time_to_relmse"] = cache_leg(api, scene, spec, cam, args.ref_spp, args.cache_frames,
-                                          args.cache_warmup)
+                                          args.cache_warmup, args.cache_lr)
     if not args.no_spectral_leg:
         del scene
         out["spectral_restir"] = spectral_leg(api, scenes, args.triangles, args.width,

@@ -358,6 +358,12 @@ typedef struct lmb_nrc_opts {
     float ema_decay;       /* inference uses an exponential moving average
                               of the weights with this decay per step; 0 = the
                               trained weights themselves               (0)  */
+    uint32_t defer_specular; /* 1: a path whose scatters so far were all
+                              specular (smooth glass, mirrors: roughness
+                              below 0.1, or any at LOW quality) does not end
+                              at its second vertex but at the vertex after
+                              its first rough scatter; 0: always the second
+                              vertex                                    (1)  */
 } lmb_nrc_opts;
 
 void lmb_nrc_opts_default(lmb_nrc_opts *opts);
@@ -395,7 +401,8 @@ typedef struct lmb_nrc_stats {
 lmb_status lmb_nrc_stats_get(lmb_nrc nrc, lmb_nrc_stats *stats);
 
 /* One frame of the cached tracer: for each of the tracer's samples, every
- * path is traced to its second vertex, where it ends with
+ * path is traced to its second vertex (a later one past specular scatters,
+ * see defer_specular), where it ends with
  * throughput * cache(vertex); about train_paths of them (chosen at random
  * there) continue to full depth instead and become training records (their
  * full-path estimate also goes to the image).  Then train_steps Adam steps

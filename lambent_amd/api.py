@@ -135,7 +135,8 @@ class _NrcOptsC(ctypes.Structure):
     _fields_ = [("learning_rate", ctypes.c_float), ("beta1", ctypes.c_float),
                 ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float),
                 ("train_paths", ctypes.c_uint32), ("train_steps", ctypes.c_uint32),
-                ("seed", ctypes.c_uint32), ("ema_decay", ctypes.c_float)]
+                ("seed", ctypes.c_uint32), ("ema_decay", ctypes.c_float),
+                ("defer_specular", ctypes.c_uint32)]
 
 
 class _NrcStatsC(ctypes.Structure):
@@ -869,6 +870,7 @@ class NeuralCacheOpts:
     train_steps: int = 4
     seed: int = 1
     ema_decay: float = 0.0
+    defer_specular: bool = True
 
 
 def _nrc_opts_c(o: NeuralCacheOpts) -> _NrcOptsC:
@@ -876,6 +878,7 @@ def _nrc_opts_c(o: NeuralCacheOpts) -> _NrcOptsC:
     c.learning_rate, c.beta1, c.beta2, c.epsilon = (o.learning_rate, o.beta1, o.beta2, o.epsilon)
     c.train_paths, c.train_steps, c.seed = o.train_paths, o.train_steps, o.seed
     c.ema_decay = o.ema_decay
+    c.defer_specular = int(bool(o.defer_specular))
     return c
 
 

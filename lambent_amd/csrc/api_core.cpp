@@ -354,7 +354,7 @@ lmb_status lmb_scene_intersect_device(lmb_scene scene, const void* rays_dev, uin
     if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
     lmb_status st = ensure_spill(&scene->spill, query_threads(n));
     if (st) return st;
-    if (!scene->fetch.ptr && (st = scene->fetch.alloc(256))) return st;
+    if (!scene->fetch.ptr && (st = scene->fetch.alloc(kFetchWords * 4))) return st;
     e = launch_intersect(scene->dev, rays_dev, n, hits_dev, scene->spill.ptr,
                          scene->fetch.as<uint32_t>(), (hipStream_t)stream);
     if (e != hipSuccess) return fail_hip(e, "intersect kernel launch");
@@ -396,7 +396,7 @@ lmb_status lmb_primary_hits(lmb_scene scene, const lmb_camera* cam, uint32_t wid
     if (st) return st;
     st = ensure_spill(&scene->spill, query_threads(n));
     if (st) return st;
-    if (!scene->fetch.ptr && (st = scene->fetch.alloc(256))) return st;
+    if (!scene->fetch.ptr && (st = scene->fetch.alloc(kFetchWords * 4))) return st;
     CameraDev cd;
     std::memcpy(&cd, cam, sizeof cd);
     e = launch_primary(scene->dev, cd, width, height, t_min, t_max, (uint32_t*)dp.ptr,

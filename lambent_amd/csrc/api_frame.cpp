@@ -487,10 +487,11 @@ lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera* c
         np.inv_ext[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
     }
     np.train_expect = (float)nrc->opts.train_paths / (float)std::max<uint32_t>(1, o.samples);
+    np.defer_specular = nrc->opts.defer_specular;
     const hipStream_t hs = (hipStream_t)stream;
-    hipError_t e = launch_fill_zero(nrc->dev.ctl + kNrcCtlRecords, 4, hs);
-    if (e == hipSuccess)
-        e = launch_trace_nrc(scene->dev, cd, bufs->dev, p, nrc->dev, np, hs, bufs->timer.get());
+    // The first shading launch zeroes the record count (after counting the
+    // previous trace's last sample for the selection probability).
+    hipError_t e = launch_trace_nrc(scene->dev, cd, bufs->dev, p, nrc->dev, np, hs, bufs->timer.get());
     if (e != hipSuccess) return fail_hip(e, "cached trace launch");
     if (nrc->opts.train_paths == 0) return LMB_OK;
     return nrc_run_steps(nrc, nrc->dev.rec_in, nrc->dev.rec_target, nrc->dev.ctl + kNrcCtlRecords,

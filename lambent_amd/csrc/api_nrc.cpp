@@ -219,6 +219,7 @@ void lmb_nrc_opts_default(lmb_nrc_opts* o) {
     o->train_steps = 4;
     o->seed = 1;
     o->ema_decay = 0.0f;
+    o->defer_specular = 1u;
 }
 
 static lmb_status validate(const lmb_nrc_opts& o) {
@@ -231,6 +232,8 @@ static lmb_status validate(const lmb_nrc_opts& o) {
         return fail(LMB_ERR_INVALID_ARGUMENT, "train_paths must be <= 2^24");
     if (!(o.ema_decay >= 0.0f && o.ema_decay < 1.0f))
         return fail(LMB_ERR_INVALID_ARGUMENT, "ema_decay must be in [0, 1)");
+    if (o.defer_specular > 1u)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "defer_specular must be 0 or 1");
     return LMB_OK;
 }
 

@@ -489,7 +489,9 @@ __device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t tbase, 
 // Persistent traversal over a ray queue with wave-level refill: whenever at
 // least `refill` lanes of the wave are idle, the wave takes that many rays
 // from the queue with one atomic and each idle lane starts one, so lanes do
-// not wait for the longest ray of the wave.  `fetch` counts handed-out rays.
+// not wait for the longest ray of the wave.  `fetch`: kFetchSegments zeroed
+// counters kFetchStride words apart, one per queue segment, counting the rays
+// handed out there.
 // Source must provide: Ray load(uint64_t slot, uint32_t* scene),
 // void store(uint64_t slot, Hit).
 template <int BLOCK, class Scenes, class Source>
@@ -497,11 +499,14 @@ __device__ __forceinline__ void traverse_queue_t(Scenes& sc, uint64_t count, uin
                                                  uint32_t refill, ShortStack<BLOCK>& st,
                                                  const TriShare& ts, Source& src) {
     const uint32_t lane = __lane_id();
+    if (count == 0) return;
     // A short queue is spread over every wave of the grid (a few rays each)
     // instead of filling the first waves: the traversal of a short queue is
     // latency-bound, and a wave lasts as long as its longest ray.
     const uint64_t waves = (uint64_t)gridDim.x * BLOCK / 64u;
     const uint32_t quota = (uint32_t)min<uint64_t>(64u, max<uint64_t>(1u, (count + waves - 1) / waves));
+    const uint64_t seg_len = (count + kFetchSegments - 1) / kFetchSegments;
+    uint32_t seg = blockIdx.x % kFetchSegments, drained = 0;  // wave-uniform
     Traversal tv;
     tv.active = false;
     uint64_t slot = 0;
@@ -513,14 +518,19 @@ __device__ __forceinline__ void traverse_queue_t(Scenes& sc, uint64_t count, uin
             if (n_idle >= min(refill, quota)) {
                 const uint32_t take = min(n_idle, quota);
                 const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
+                const uint64_t lo = (uint64_t)seg * seg_len;
+                const uint64_t hi = min(lo + seg_len, count);
                 uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(fetch, take);
+                if (lane == leader) base = atomicAdd(fetch + seg * kFetchStride, take);
                 base = __shfl(base, (int)leader);
-                if ((uint64_t)base + take >= count) exhausted = true;
+                if (lo + base + take >= hi) {  // this segment is drained: go on to the next
+                    seg = (seg + 1u) % kFetchSegments;
+                    if (++drained == kFetchSegments) exhausted = true;
+                }
                 const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
                 if (!tv.active && rank < take) {
-                    const uint64_t mine = (uint64_t)base + rank;
-                    if (mine < count) {
+                    const uint64_t mine = lo + base + rank;
+                    if (mine < hi) {
                         slot = mine;
                         uint32_t item = 0;
                         const Ray r = src.load(slot, &item);

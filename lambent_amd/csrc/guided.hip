@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kGBlock) void k_g_generate(CameraDev cam, FrameDev 
     if (gtid == 0) {
         f.counters[0] = (uint32_t)f.npix;
         for (uint32_t i = 1; i <= kMaxBounces; ++i) f.counters[i] = 0;
-        for (uint32_t i = 0; i <= kMaxBounces; ++i) f.counters[kCounterFetch + i] = 0;
+        for (uint32_t k = 0; k < 2 * kFetchSegments; ++k) f.counters[kCounterFetch + k * kFetchStride] = 0;
         if (sample == 0) *reinterpret_cast<uint64_t*>(f.counters + kCounterVertices) = 0;
     }
     for (uint64_t pix = gtid; pix < f.npix; pix += stride) {

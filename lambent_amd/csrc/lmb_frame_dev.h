@@ -16,9 +16,16 @@ constexpr uint32_t kMaxBounces = 64;
 // [kCounterVerticesTotal] (64-bit) path vertices since the last stats read.
 constexpr uint32_t kCounterVertices = kMaxBounces + 2;
 constexpr uint32_t kCounterVerticesTotal = kMaxBounces + 4;
-// [kCounterFetch + b]: rays of bounce b handed out to traversal lanes.
-constexpr uint32_t kCounterFetch = kMaxBounces + 6;
-constexpr uint32_t kNumCounters = kCounterFetch + kMaxBounces + 1;
+// [kCounterFetch + s * kFetchWords]: the fetch counters of the traversal of
+// bounce b (s = b & 1; the other set is zeroed by that launch for the next
+// bounce), 256-byte aligned.
+constexpr uint32_t kCounterFetch = 128;
+constexpr uint32_t kNumCounters = kCounterFetch + 2 * kFetchWords;
+static_assert(kCounterVerticesTotal + 2 <= kCounterFetch, "counter layout");
+
+__host__ __device__ inline uint32_t* fetch_set(uint32_t* counters, uint32_t bounce) {
+    return counters + kCounterFetch + (bounce & 1u) * kFetchWords;
+}
 
 struct FrameDev {
     uint32_t width, height;

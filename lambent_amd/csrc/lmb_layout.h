@@ -47,6 +47,17 @@ constexpr uint8_t kMetaInternal = 0xFF;
 constexpr int kStackLds = 8;
 constexpr int kStackMax = 48;
 
+// Work distribution of the persistent traversal: a queue of rays is cut into
+// kFetchSegments contiguous segments, each with its own fetch counter on a
+// 256-byte line (kFetchStride words apart).  Blocks are dispatched to the 8
+// XCDs in turn, so block b starts on segment b % 8 and takes the rest of the
+// queue only once that segment is drained: 8 atomic streams instead of one
+// queue of atomics on a single address, and each XCD's L2 sees a contiguous
+// part of the queue.
+constexpr uint32_t kFetchSegments = 8;
+constexpr uint32_t kFetchStride = 64;
+constexpr uint32_t kFetchWords = kFetchSegments * kFetchStride;
+
 // Triangle in BVH leaf order: three float4; v0.w holds the original
 // primitive index bits, v1.w its material index (bindings.wesl:59-61).
 // 48 bytes.

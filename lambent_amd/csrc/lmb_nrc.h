@@ -19,6 +19,7 @@
 #pragma once
 
 #include <hip/hip_runtime_api.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "lmb_frame_dev.h"
@@ -106,14 +107,20 @@ struct NrcQuery {
     float spare[2];
 };
 static_assert(sizeof(NrcQuery) == 64, "NrcQuery is 64 bytes");
+static_assert(offsetof(NrcQuery, pid) == 4 * kNrcInputs, "pid follows the inputs");
 
-// Control words of a cache (device u32).
-constexpr uint32_t kNrcCtlQueries = 0;   // queries of the current sample
-constexpr uint32_t kNrcCtlRecords = 1;   // training records of the current trace
-constexpr uint32_t kNrcCtlHits = 2;      // paths reaching the cache vertex, this sample
-constexpr uint32_t kNrcCtlHitsPrev = 3;  // the same for the previous sample
-constexpr uint32_t kNrcCtlHostCount = 4; // record count of the host-array calls
-constexpr uint32_t kNrcCtlWords = 8;
+// Control words of a cache (device u32), each on a 256-byte line of its own:
+// every wave of the tracer's shading does an atomic on the first two, and
+// atomics on one line queue behind each other in L2.
+constexpr uint32_t kNrcCtlStride = 64;
+constexpr uint32_t kNrcCtlQueries = 0 * kNrcCtlStride;      // queries of the current sample
+constexpr uint32_t kNrcCtlRecords = 1 * kNrcCtlStride;      // records reserved in the current trace
+                                                            // (may pass rec_cap; those are dropped)
+constexpr uint32_t kNrcCtlRecordsBase = 2 * kNrcCtlStride;  // min(records, rec_cap) at the sample's start
+constexpr uint32_t kNrcCtlHitsPrev = 3 * kNrcCtlStride;     // cache vertices of the last sample
+                                                            // traced (kept across traces)
+constexpr uint32_t kNrcCtlHostCount = 4 * kNrcCtlStride;    // record count of the host-array calls
+constexpr uint32_t kNrcCtlWords = 5 * kNrcCtlStride;
 
 struct NrcDev {
     float* master;  // kNrcNumWeights fp32
@@ -167,6 +174,7 @@ struct NrcTraceParams {
     float inv_ext[3];
     float train_expect;     // expected training records per sample
     uint32_t sample;
+    uint32_t defer_specular;  // cache vertices only after a rough scatter
 };
 hipError_t launch_trace_nrc(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
                             const TraceParams& p, const NrcDev& d, NrcTraceParams np,

@@ -14,11 +14,19 @@
 //     T * max(cache(inputs), 0) from the inference kernel.
 // The emission of the second vertex belongs to S (the cache predicts
 // scattered radiance only).
+// With defer_specular, a vertex is a cache vertex only when the scatter
+// before it spread the path (a diffuse lobe or one of roughness >= 0.1): a
+// path seen through smooth glass or a mirror goes on to the vertex after
+// its first rough scatter (the spread rule of neural radiance caching), so
+// the cache's error there is blurred by a lobe instead of shown sharply.
+// Every path still meets at most one cache vertex; training continuations
+// (rslot != 0) never meet another.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
 #include "dev_common.h"
+#include "dev_compact.h"
 #include "dev_shade.h"
 #include "lmb_frame_dev.h"
 #include "lmb_nrc.h"
@@ -26,18 +34,26 @@
 namespace lmb {
 
 constexpr int kNrcTraceBlock = 256;
-constexpr uint32_t kCacheBounce = 1;  // the second path vertex
+constexpr uint32_t kCacheBounce = 1;  // the second path vertex: the earliest cache vertex
+constexpr uint32_t kArmedBit = 0x80000000u;  // ps_accum.w: the last scatter spread the path
+constexpr float kSpreadRoughness = 0.1f;
 
-// Wave-level compaction: every lane with `pred` gets a distinct index from
-// `*counter`; one atomic per wave.  Called by all lanes of the wave.
-__device__ __forceinline__ uint32_t wave_reserve(bool pred, uint32_t* counter, uint32_t lane) {
-    const uint64_t mask = __ballot(pred);
-    if (!mask) return 0;
-    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)leader);
-    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+// Whether a scatter off `it` spreads the path (see defer_specular above).
+template <uint32_t Q>
+__device__ __forceinline__ bool scatter_spreads(const Isect& it) {
+    if (it.ty == kMatDiffuse) return true;
+    if (Q == kQualityLow) return false;  // Low has no roughness: mirrors and smooth glass
+    if (it.ty == kMatMetallic) return fmaxf(it.rough_x, it.rough_y) >= kSpreadRoughness;
+    return fmaxf(it.rough_tx, it.rough_ty) >= kSpreadRoughness;
+}
+
+// A query / training record as four 16-byte stores (the layout of NrcQuery).
+__device__ __forceinline__ void store_query(NrcQuery* q, const float (&in)[kNrcInputs], uint32_t pid) {
+    float4* o = reinterpret_cast<float4*>(q);
+    o[0] = make_float4(in[0], in[1], in[2], in[3]);
+    o[1] = make_float4(in[4], in[5], in[6], in[7]);
+    o[2] = make_float4(in[8], in[9], in[10], in[11]);
+    o[3] = make_float4(in[12], __uint_as_float(pid), 0.0f, 0.0f);
 }
 
 __device__ __forceinline__ float train_draw(uint32_t pid, uint32_t seed_offset, uint32_t sample) {
@@ -52,20 +68,28 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
     const uint64_t gtid = (uint64_t)blockIdx.x * kNrcTraceBlock + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * kNrcTraceBlock;
     const uint32_t count = f.counters[bounce];
-    const uint64_t count_r = ((uint64_t)count + 63) & ~63ull;
+    const uint64_t count_r = block_uniform_count<kNrcTraceBlock>(count);
     const float4* __restrict__ qin = f.queue[bounce & 1];
     float4* __restrict__ qout = f.queue[(bounce + 1) & 1];
-    const uint32_t lane = __lane_id();
+    __shared__ ReserveLds<kNrcTraceBlock, 1> rl_rec;
+    __shared__ ReserveLds<kNrcTraceBlock, 2> rl_out;
     if (bounce == 0 && gtid == 0) {
-        d.ctl[kNrcCtlQueries] = 0;
-        const uint32_t hits = d.ctl[kNrcCtlHits];
+        // The last sample's cache vertices: its queries plus the records it
+        // stored (a record that found no room became a query).  No lane
+        // reserves queries or records at bounce 0.
+        const uint32_t rec = min(d.ctl[kNrcCtlRecords], d.rec_cap);
+        const uint32_t hits = d.ctl[kNrcCtlQueries] + (rec - min(d.ctl[kNrcCtlRecordsBase], rec));
         if (hits) d.ctl[kNrcCtlHitsPrev] = hits;
-        d.ctl[kNrcCtlHits] = 0;
+        d.ctl[kNrcCtlQueries] = 0;
+        if (np.sample == 0) d.ctl[kNrcCtlRecords] = 0;  // a new trace
+        d.ctl[kNrcCtlRecordsBase] = np.sample == 0 ? 0u : rec;
     }
     float prob = 0.0f;
-    if (bounce == kCacheBounce) {
+    if (bounce >= kCacheBounce) {
+        // Selection probability for about train_expect records per sample;
+        // before any sample has been counted, the paths reaching bounce 1.
         const uint32_t prev = d.ctl[kNrcCtlHitsPrev];
-        const uint32_t ref = prev ? prev : max(count, 1u);
+        const uint32_t ref = prev ? prev : max(f.counters[kCacheBounce], 1u);
         prob = fminf(np.train_expect / (float)ref, 1.0f);
     }
     for (uint64_t slot = gtid; slot < count_r; slot += stride) {
@@ -74,6 +98,7 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
         float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4;
         uint4 h = make_uint4(kMissPrim, 0u, 0u, 0u);
         uint32_t pid = 0, seed = 0, rslot = 0;
+        bool armed = false;
         V3 color = v3s(0.f), scol = v3s(0.f);
         float picked = -1.0f;
         if (active) {
@@ -88,6 +113,8 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
             picked = th.w;
             scol = v3(ac.x, ac.y, ac.z);
             rslot = __float_as_uint(ac.w);
+            armed = (rslot & kArmedBit) != 0u;
+            rslot &= ~kArmedBit;
         }
         const bool hit = active && h.x != kMissPrim;
         const V3 dir = v3(d4.x, d4.y, d4.z);
@@ -105,7 +132,7 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
                 f.out_albedo[pid] = make_float4(it.color.x, it.color.y, it.color.z, 1.f);
             }
             scol = fma3(color, it.emission, scol);  // general.wesl:714
-            if (bounce == kCacheBounce) {
+            if (bounce >= kCacheBounce && armed && rslot == 0u) {
                 at_cache = true;
                 in[0] = (new_pos.x - np.lo[0]) * np.inv_ext[0];
                 in[1] = (new_pos.y - np.lo[1]) * np.inv_ext[1];
@@ -123,8 +150,10 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
                 cand = train_draw(pid, p.seed_offset, np.sample) < prob;
             }
         }
-        // ---- phase B (whole wave): training-record slots
-        const uint32_t r = wave_reserve(cand, d.ctl + kNrcCtlRecords, lane);
+        // ---- phase B (whole block): training-record slots
+        uint32_t r_idx[1];
+        block_reserve<kNrcTraceBlock, 1>({cand}, {d.ctl + kNrcCtlRecords}, rl_rec, r_idx);
+        const uint32_t r = r_idx[0];
         if (cand && r >= d.rec_cap) cand = false;
         const bool is_query = at_cache && !cand;
         // ---- phase C: scattering, continuation or the end of the path
@@ -134,10 +163,7 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
         if (active) {
             if (hit) {
                 if (cand) {
-                    NrcQuery* rq = d.rec_in + r;
-#pragma unroll
-                    for (int k = 0; k < kNrcInputs; ++k) rq->in[k] = in[k];
-                    rq->pid = pid;
+                    store_query(d.rec_in + r, in, pid);
                     d.rec_aux[2 * (uint64_t)r] = make_float4(color.x, color.y, color.z, scol.x);
                     d.rec_aux[2 * (uint64_t)r + 1] = make_float4(scol.y, scol.z, 0.f, 0.f);
                     rslot = r + 1u;
@@ -165,7 +191,9 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
                         out_d = make_float4(ndir.x, ndir.y, ndir.z, 0.f);
                         f.ps_seed[pid] = seed;
                         f.ps_through[pid] = make_float4(color.x, color.y, color.z, picked);
-                        f.ps_accum[pid] = make_float4(scol.x, scol.y, scol.z, __uint_as_float(rslot));
+                        const bool arm = !np.defer_specular || scatter_spreads<Q>(it);
+                        f.ps_accum[pid] = make_float4(scol.x, scol.y, scol.z,
+                                                      __uint_as_float(rslot | (arm ? kArmedBit : 0u)));
                     } else {
                         finish = true;
                     }
@@ -191,19 +219,15 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
                 f.pix_sum[pid] = make_float4(ps.x + pix.x, ps.y + pix.y, ps.z + pix.z, 0.f);
             }
         }
-        // ---- phase D (whole wave): queries, hit count, continuation queue
-        const uint32_t qi = wave_reserve(is_query, d.ctl + kNrcCtlQueries, lane);
+        // ---- phase D (whole block): queries and the continuation queue
+        uint32_t o_idx[2];
+        block_reserve<kNrcTraceBlock, 2>({is_query, cont}, {d.ctl + kNrcCtlQueries, f.counters + bounce + 1},
+                                         rl_out, o_idx);
+        const uint32_t qi = o_idx[0], ci = o_idx[1];
         if (is_query) {
-            NrcQuery* qq = d.queries + qi;
-#pragma unroll
-            for (int k = 0; k < kNrcInputs; ++k) qq->in[k] = in[k];
-            qq->pid = pid;
+            store_query(d.queries + qi, in, pid);
             d.query_weight[qi] = make_float4(color.x, color.y, color.z, 0.f);
         }
-        const uint64_t at_mask = __ballot(at_cache);
-        if (at_mask && lane == (uint32_t)__ffsll((unsigned long long)at_mask) - 1u)
-            atomicAdd(d.ctl + kNrcCtlHits, (uint32_t)__popcll(at_mask));
-        const uint32_t ci = wave_reserve(cont, f.counters + bounce + 1, lane);
         if (cont) {
             qout[2 * (uint64_t)ci] = out_o;
             qout[2 * (uint64_t)ci + 1] = out_d;

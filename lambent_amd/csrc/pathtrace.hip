@@ -21,6 +21,7 @@
 #include <algorithm>
 
 #include "dev_common.h"
+#include "dev_compact.h"
 #include "dev_shade.h"
 #include "dev_traverse.h"
 #include "lmb_batch.h"
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(kPtBlock) void k_generate(CameraDev cam, FrameDev f
     if (gtid == 0) {
         f.counters[0] = (uint32_t)f.npix;
         for (uint32_t i = 1; i <= kMaxBounces; ++i) f.counters[i] = 0;
-        for (uint32_t i = 0; i <= kMaxBounces; ++i) f.counters[kCounterFetch + i] = 0;
+        for (uint32_t k = 0; k < 2 * kFetchSegments; ++k) f.counters[kCounterFetch + k * kFetchStride] = 0;
         if (sample == 0) *vertex_counter(f) = 0;
     }
     for (uint64_t pix = gtid; pix < f.npix; pix += stride) {
@@ -122,6 +123,9 @@ __global__ __launch_bounds__(kPtBlock, LMB_EXTEND_WAVES) void k_extend(SceneDev 
         atomicAdd((unsigned long long*)vertex_counter(f), (unsigned long long)count);
         atomicAdd((unsigned long long*)(f.counters + kCounterVerticesTotal),
                   (unsigned long long)count);
+        // The next bounce's fetch set; its last user (bounce - 1) has finished.
+        uint32_t* next = fetch_set(f.counters, bounce + 1);
+        for (uint32_t k = 0; k < kFetchSegments; ++k) next[k * kFetchStride] = 0;
     }
     ShortStack<kPtBlock> st;
     st.lds = lds_stack + threadIdx.x;
@@ -129,7 +133,7 @@ __global__ __launch_bounds__(kPtBlock, LMB_EXTEND_WAVES) void k_extend(SceneDev 
     LMB_TRI_SHARE(kPtBlock, ts);
     QueueSource src{f.queue[bounce & 1], f.hits, p.tmin, p.tmax};
     traverse_queue<kPtBlock, kRefill>(s.nodes, s.tris, s.num_nodes, count,
-                                      f.counters + kCounterFetch + bounce, st, ts, src);
+                                      fetch_set(f.counters, bounce), st, ts, src);
 }
 
 // kTable: the queue's rays carry a scene index (direction .w) into `table`
@@ -141,10 +145,10 @@ __global__ __launch_bounds__(kPtBlock) void k_shade_plain_t(SceneDev s1, const S
     const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
     const uint32_t count = f.counters[bounce];
-    const uint64_t count_r = ((uint64_t)count + 63) & ~63ull;  // wave-uniform trip count
+    const uint64_t count_r = block_uniform_count<kPtBlock>(count);
+    __shared__ ReserveLds<kPtBlock, 1> rl;
     const float4* __restrict__ qin = f.queue[bounce & 1];
     float4* __restrict__ qout = f.queue[(bounce + 1) & 1];
-    const uint32_t lane = __lane_id();
     for (uint64_t slot = gtid; slot < count_r; slot += stride) {
         bool cont = false;
         float4 out_o = make_float4(0.f, 0.f, 0.f, 0.f), out_d = out_o;
@@ -208,18 +212,12 @@ __global__ __launch_bounds__(kPtBlock) void k_shade_plain_t(SceneDev s1, const S
                 f.pix_sum[pid] = make_float4(ps.x + scol.x, ps.y + scol.y, ps.z + scol.z, 0.f);
             }
         }
-        // Compact the survivors into the next queue: one atomic per wave.
-        const uint64_t mask = __ballot(cont);
-        if (mask) {
-            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&f.counters[bounce + 1], (uint32_t)__popcll(mask));
-            base = __shfl(base, (int)leader);
-            if (cont) {
-                const uint32_t idx = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-                qout[2 * (uint64_t)idx] = out_o;
-                qout[2 * (uint64_t)idx + 1] = out_d;
-            }
+        // Compact the survivors into the next queue: one atomic per block.
+        uint32_t idx[1];
+        block_reserve<kPtBlock, 1>({cont}, {f.counters + bounce + 1}, rl, idx);
+        if (cont) {
+            qout[2 * (uint64_t)idx[0]] = out_o;
+            qout[2 * (uint64_t)idx[0] + 1] = out_d;
         }
     }
 }
@@ -397,6 +395,9 @@ __global__ __launch_bounds__(kPtBlock, LMB_EXTEND_WAVES) void k_batch_extend(con
         atomicAdd((unsigned long long*)vertex_counter(f), (unsigned long long)count);
         atomicAdd((unsigned long long*)(f.counters + kCounterVerticesTotal),
                   (unsigned long long)count);
+        // The next bounce's fetch set; its last user (bounce - 1) has finished.
+        uint32_t* next = fetch_set(f.counters, bounce + 1);
+        for (uint32_t k = 0; k < kFetchSegments; ++k) next[k * kFetchStride] = 0;
     }
     ShortStack<kPtBlock> st;
     st.lds = lds_stack + threadIdx.x;
@@ -404,8 +405,7 @@ __global__ __launch_bounds__(kPtBlock, LMB_EXTEND_WAVES) void k_batch_extend(con
     LMB_TRI_SHARE(kPtBlock, ts);
     SceneTable sc{table};
     TaggedQueueSource src{f.queue[bounce & 1], f.hits, p.tmin, p.tmax};
-    traverse_queue_t<kPtBlock>(sc, count, f.counters + kCounterFetch + bounce, refill, st, ts,
-                               src);
+    traverse_queue_t<kPtBlock>(sc, count, fetch_set(f.counters, bounce), refill, st, ts, src);
 }
 
 template <uint32_t Q>

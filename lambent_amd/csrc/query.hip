@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kQueryBlock) void k_primary(SceneDev s, CameraDev c
 hipError_t launch_intersect(const SceneDev& s, const void* rays, uint64_t n, void* hits,
                             uint2* spill, uint32_t* fetch, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    hipError_t e = hipMemsetAsync(fetch, 0, sizeof(uint32_t), stream);
+    hipError_t e = hipMemsetAsync(fetch, 0, kFetchWords * sizeof(uint32_t), stream);
     if (e != hipSuccess) return e;
     uint64_t threads = query_threads(n);
     hipLaunchKernelGGL(k_intersect, dim3((uint32_t)(threads / kQueryBlock)), dim3(kQueryBlock),
@@ -122,7 +122,7 @@ hipError_t launch_primary(const SceneDev& s, const CameraDev& cam, uint32_t widt
                           uint2* spill, uint32_t* fetch, hipStream_t stream) {
     uint64_t n = (uint64_t)width * height;
     if (n == 0) return hipSuccess;
-    hipError_t e = hipMemsetAsync(fetch, 0, sizeof(uint32_t), stream);
+    hipError_t e = hipMemsetAsync(fetch, 0, kFetchWords * sizeof(uint32_t), stream);
     if (e != hipSuccess) return e;
     uint64_t threads = query_threads(n);
     hipLaunchKernelGGL(k_primary, dim3((uint32_t)(threads / kQueryBlock)), dim3(kQueryBlock), 0,
