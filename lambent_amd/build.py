@@ -54,6 +54,8 @@ def sources() -> list[Path]:
 def _compile(hipcc: str, src: Path) -> Path:
     obj = OUT_DIR / (src.name + ".o")
     flags = COMMON + (HIP_FLAGS if src.suffix == ".hip" else [])
+    # Tuning builds: e.g. LMB_EXTRA_FLAGS="-DLMB_FRAME_BLOCKS_PER_CU=4".
+    flags += os.environ.get("LMB_EXTRA_FLAGS", "").split()
     cmd = [hipcc, *flags, "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:

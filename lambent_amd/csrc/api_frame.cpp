@@ -194,7 +194,7 @@ lmb_status alloc_window(lmb_data_buffers b, uint32_t width, uint32_t height) {
     if ((st = alloc_zero(b->info, n * sizeof(CompressedInfo)))) return st;
     if ((st = alloc_zero(b->mc, b->mc_count * sizeof(MarkovChain)))) return st;
     if ((st = alloc_zero(b->mc_old, b->mc_count * sizeof(MarkovChain)))) return st;
-    if ((st = ensure_spill(&b->spill, frame_threads(n)))) return st;
+    if ((st = ensure_spill(&b->spill, extend_threads(n)))) return st;
     b->width = width;
     b->height = height;
     return LMB_OK;

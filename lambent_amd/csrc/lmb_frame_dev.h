@@ -72,7 +72,8 @@ struct TraceParams {
 };
 
 // Threads of the persistent wavefront kernels for a frame of npix paths.
-uint64_t frame_threads(uint64_t npix);
+uint64_t frame_threads(uint64_t npix);   // generation / shading grid
+uint64_t extend_threads(uint64_t npix);  // persistent traversal grid (sizes the stack spill)
 
 // Optional per-launch timing hooks (HIP events recorded on the launch stream).
 enum KernelKind {

@@ -54,7 +54,10 @@ constexpr int kStackMax = 48;
 // queue only once that segment is drained: 8 atomic streams instead of one
 // queue of atomics on a single address, and each XCD's L2 sees a contiguous
 // part of the queue.
-constexpr uint32_t kFetchSegments = 8;
+#ifndef LMB_FETCH_SEGMENTS
+#define LMB_FETCH_SEGMENTS 8
+#endif
+constexpr uint32_t kFetchSegments = LMB_FETCH_SEGMENTS;
 constexpr uint32_t kFetchStride = 64;
 constexpr uint32_t kFetchWords = kFetchSegments * kFetchStride;
 

@@ -32,19 +32,29 @@ namespace lmb {
 
 constexpr int kPtBlock = 256;
 
-// Persistent grid of the wavefront stages, in 256-thread blocks per CU.
-// Measured on the 1M-triangle 1080p frame: 3 is fastest (1 / 2 / 3 / 4 / 8:
-// 9.15 / 6.37 / 6.31 / 6.35 / 6.53 ms per frame).
+// Grids of the wavefront stages, in 256-thread blocks per CU: the
+// grid-stride generation / shading kernels, and the persistent traversal
+// (k_extend, whose 94 VGPRs allow 5 blocks of 4 waves per CU).  Measured on
+// the 1M-triangle 1080p frame (5 bounces): shading 3 / 4 / 5 blocks 0.478 /
+// 0.471 / 0.531 ms per frame; traversal 3 / 4 / 5 blocks 3.53 / 3.18 /
+// 3.04 ms per frame (6 blocks at 80 VGPRs spill: 3.41 ms).
 #ifndef LMB_FRAME_BLOCKS_PER_CU
-#define LMB_FRAME_BLOCKS_PER_CU 3
+#define LMB_FRAME_BLOCKS_PER_CU 4
+#endif
+#ifndef LMB_EXTEND_BLOCKS_PER_CU
+#define LMB_EXTEND_BLOCKS_PER_CU 5
 #endif
 
-uint64_t frame_threads(uint64_t npix) {
+static uint64_t grid_threads(uint64_t npix, uint64_t blocks_per_cu) {
     uint64_t blocks = (npix + kPtBlock - 1) / kPtBlock;
-    uint64_t cap = (uint64_t)device_cu_count() * LMB_FRAME_BLOCKS_PER_CU;
+    uint64_t cap = (uint64_t)device_cu_count() * blocks_per_cu;
     blocks = std::max<uint64_t>(1, std::min(blocks, cap));
     return blocks * kPtBlock;
 }
+
+uint64_t frame_threads(uint64_t npix) { return grid_threads(npix, LMB_FRAME_BLOCKS_PER_CU); }
+
+uint64_t extend_threads(uint64_t npix) { return grid_threads(npix, LMB_EXTEND_BLOCKS_PER_CU); }
 
 __device__ __forceinline__ uint64_t* vertex_counter(const FrameDev& f) {
     return reinterpret_cast<uint64_t*>(f.counters + kCounterVertices);
@@ -107,7 +117,10 @@ struct QueueSource {
     }
 };
 
-constexpr int kRefill = 16;
+#ifndef LMB_REFILL
+#define LMB_REFILL 16
+#endif
+constexpr int kRefill = LMB_REFILL;  // idle lanes that make a wave fetch more rays
 
 // Minimum waves per SIMD the traversal kernels are compiled for (register cap).
 #ifndef LMB_EXTEND_WAVES
@@ -235,7 +248,7 @@ __global__ __launch_bounds__(kPtBlock) void k_resolve(FrameDev f, TraceParams p)
 
 hipError_t launch_extend(const SceneDev& s, const FrameDev& f, const TraceParams& p,
                          uint32_t bounce, hipStream_t stream) {
-    const dim3 grid((uint32_t)(frame_threads(f.npix) / kPtBlock)), block(kPtBlock);
+    const dim3 grid((uint32_t)(extend_threads(f.npix) / kPtBlock)), block(kPtBlock);
     hipLaunchKernelGGL(k_extend, grid, block, 0, stream, s, f, p, bounce);
     return hipGetLastError();
 }
@@ -257,11 +270,12 @@ template <uint32_t Q>
 static hipError_t launch_plain(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
                                const TraceParams& p, hipStream_t stream, KernelTimer* tm) {
     const dim3 grid((uint32_t)(frame_threads(f.npix) / kPtBlock)), block(kPtBlock);
+    const dim3 xgrid((uint32_t)(extend_threads(f.npix) / kPtBlock));
     for (uint32_t sample = 0; sample < p.samples; ++sample) {
         hipLaunchKernelGGL(k_generate, grid, block, 0, stream, cam, f, p, sample);
         for (uint32_t b = 0; b < p.bounces; ++b) {
             if (tm) tm->begin(kKernelExtend, stream);
-            hipLaunchKernelGGL(k_extend, grid, block, 0, stream, s, f, p, b);
+            hipLaunchKernelGGL(k_extend, xgrid, block, 0, stream, s, f, p, b);
             if (tm) tm->end(kKernelExtend, stream);
             if (tm) tm->begin(kKernelShade, stream);
             hipLaunchKernelGGL((k_shade_plain_t<Q, false>), grid, block, 0, stream, s,
