@@ -81,11 +81,15 @@ def rel_mse(img, ref) -> float:
 
 
 def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
-              lr: float = 1e-3) -> dict:
+              lr: float = 1e-3, spp: int = 8, steps: int = 32) -> dict:
     """Time-to-relMSE of the cached tracer against the plain tracer, both with
     8-bounce paths (RR from bounce 3), judged against a plain reference of
-    ref_spp samples accumulated in fp64.  Times are GPU wall times of the
-    dispatches (host read-back of the images for scoring is not counted)."""
+    ref_spp samples accumulated in fp64.  The cache starts untrained: `warm`
+    1-spp frames (timed, not accumulated), then `frames` frames of `spp`
+    samples (the camera rays are traced once per frame, training paths come
+    from the first sample), `steps` Adam steps per frame over the frame's
+    65536 training records.  Times are wall times of the dispatches (host
+    read-back of the images for scoring is not counted)."""
     import numpy as np
     W, H = spec.width, spec.height
     popts = dict(bounces=8, rr_start=3, guiding=api.Guiding.OFF,
@@ -109,25 +113,25 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
     ref /= ref_spp // chunk
     log(f"cache leg: reference {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
     plain_pts = []
-    for spp in (16, 64, 256):
-        img, dt = plain(spp, 20_000 + spp)
-        plain_pts.append({"spp": spp, "seconds": dt, "relmse": rel_mse(img, ref)})
-        log(f"cache leg: plain {spp} spp {dt * 1e3:.1f} ms relMSE {plain_pts[-1]['relmse']:.5f}")
+    for n_spp in (16, 64, 256):
+        img, dt = plain(n_spp, 20_000 + n_spp)
+        plain_pts.append({"spp": n_spp, "seconds": dt, "relmse": rel_mse(img, ref)})
+        log(f"cache leg: plain {n_spp} spp {dt * 1e3:.1f} ms relMSE {plain_pts[-1]['relmse']:.5f}")
 
-    cache = api.NeuralCache(api.NeuralCacheOpts(learning_rate=lr))
-    pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=1, **popts))
+    cache = api.NeuralCache(api.NeuralCacheOpts(learning_rate=lr, train_steps=steps))
+    warm_pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=1, **popts))
+    pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=spp, **popts))
     b = api.DataBuffers(W, H, 1, api.BufferType(0))
-    b.set_profiling(True)
-    b.kernel_stats()
     t = 0.0
     for f in range(warm):
         api.sync()
         t0 = time.perf_counter()
-        pipe.dispatch_cached(scene, cam, b, cache, f)
+        warm_pipe.dispatch_cached(scene, cam, b, cache, f)
         api.sync()
         t += time.perf_counter() - t0
+    t_warm = t
     acc = np.zeros((H, W, 3))
-    checks = {1, 2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 320, 384, 448, frames}
+    checks = set(range(1, 65, 2)) | set(range(64, frames + 1, 8)) | {frames}
     pts = []
     for n in range(1, frames + 1):
         api.sync()
@@ -137,38 +141,49 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
         t += time.perf_counter() - t0
         acc += b.output(api.Output.COLOR)[..., :3]
         if n in checks:
-            pts.append({"frames": n, "seconds": t, "relmse": rel_mse(acc / n, ref)})
+            pts.append({"frames": n, "samples": n * spp, "seconds": t,
+                        "relmse": rel_mse(acc / n, ref)})
+    log(f"cache leg: {warm} + {frames} frames in {t:.2f} s, final relMSE {pts[-1]['relmse']:.5f}")
+    # Matrix-core figures from a few more frames with per-launch events (kept
+    # out of the timed run: the events add about a quarter to a frame).
+    b.set_profiling(True)
+    b.kernel_stats()
+    prof_frames = 4
+    for f in range(prof_frames):
+        pipe.dispatch_cached(scene, cam, b, cache, warm + frames + 1 + f)
     ks = b.kernel_stats()
     st = cache.stats()
-    total = warm + frames
-    log(f"cache leg: {total} frames in {t:.2f} s, final relMSE {pts[-1]['relmse']:.5f}")
     targets = []
     for p in plain_pts:
         hit = next((q for q in pts if q["relmse"] <= p["relmse"]), None)
         targets.append({"plain_spp": p["spp"], "relmse": p["relmse"], "plain_seconds": p["seconds"],
                         "cache_seconds": hit["seconds"] if hit else None,
                         "speedup": p["seconds"] / hit["seconds"] if hit else None})
-    infer_flops = st["queries"] * NRC_FLOP_FWD * ks["cache_infer_launches"]
-    train_flops = st["records"] * NRC_FLOP_TRAIN * total
-    infer_tf = infer_flops / (ks["cache_infer_ms"] * 1e-3) / 1e12 if ks["cache_infer_ms"] else 0.0
-    train_tf = train_flops / (ks["cache_train_ms"] * 1e-3) / 1e12 if ks["cache_train_ms"] else 0.0
+    infer_launches = ks["cache_infer_launches"]
+    infer_ms = ks["cache_infer_ms"]
+    train_ms = ks["cache_train_ms"]
+    infer_flops = st["queries"] * NRC_FLOP_FWD * infer_launches
+    train_flops = st["records"] * NRC_FLOP_TRAIN * prof_frames
+    infer_tf = infer_flops / (infer_ms * 1e-3) / 1e12 if infer_ms else 0.0
+    train_tf = train_flops / (train_ms * 1e-3) / 1e12 if train_ms else 0.0
     return {
         "workload": f"procedural {W}x{H}, 8 bounces, RR from 3; cache: paths end at the second "
-                    f"vertex (past specular scatters: the vertex after the first rough one), "
-                    f"65536 training paths and 4 Adam steps (lr {lr:g}) per frame, "
-                    f"{warm} warm-up frames (timed, not accumulated)",
+                    f"vertex (a specular scatter moves it one vertex on), 65536 training paths "
+                    f"and {steps} Adam steps (lr {lr:g}) per frame; {warm} warm-up frames of "
+                    f"1 spp (timed, not accumulated), then frames of {spp} spp",
         "reference": f"plain tracer {ref_spp} spp, fp64 accumulation of {chunk}-spp dispatches",
         "plain": plain_pts,
         "cache_points": pts,
-        "cache_ms_per_frame": t / total * 1e3,
+        "warm_ms_per_frame": t_warm / max(warm, 1) * 1e3,
+        "cache_ms_per_frame": (t - t_warm) / frames * 1e3,
         "targets": targets,
         "headline": targets[-1],
         "mfma": {
             "infer_tflops": infer_tf, "infer_frac": infer_tf / MFMA_BF16_PEAK_TFLOPS,
-            "infer_ms_per_frame": ks["cache_infer_ms"] / total,
-            "queries_per_frame": st["queries"],
+            "infer_ms_per_launch": infer_ms / max(infer_launches, 1),
+            "queries_per_launch": st["queries"],
             "train_tflops": train_tf, "train_frac": train_tf / MFMA_BF16_PEAK_TFLOPS,
-            "train_ms_per_frame": ks["cache_train_ms"] / total,
+            "train_ms_per_frame": train_ms / prof_frames,
             "records_per_frame": st["records"],
             "peak_tflops": MFMA_BF16_PEAK_TFLOPS, "flop_per_query": NRC_FLOP_FWD,
         },
@@ -330,9 +345,11 @@ def main() -> int:
     ap.add_argument("--no-batch-leg", action="store_true")
     ap.add_argument("--batch-frames", type=int, default=3)
     ap.add_argument("--ref-spp", type=int, default=4096)
-    ap.add_argument("--cache-frames", type=int, default=512)
-    ap.add_argument("--cache-warmup", type=int, default=64)
+    ap.add_argument("--cache-frames", type=int, default=96)
+    ap.add_argument("--cache-warmup", type=int, default=32)
     ap.add_argument("--cache-lr", type=float, default=1e-3)
+    ap.add_argument("--cache-spp", type=int, default=8)
+    ap.add_argument("--cache-steps", type=int, default=32)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (rocprofv3 counter calibration)")
     args = ap.parse_args()
@@ -434,7 +451,8 @@ def main() -> int:
             log(f"cpu baseline failed: {exc!r}")
     if not args.no_cache_leg:
         out["time_to_relmse"] = cache_leg(api, scene, spec, cam, args.ref_spp, args.cache_frames,
-                                          args.cache_warmup, args.cache_lr)
+                                          args.cache_warmup, args.cache_lr, args.cache_spp,
+                                          args.cache_steps)
     if not args.no_spectral_leg:
         del scene
         out["spectral_restir"] = spectral_leg(api, scenes, args.triangles, args.width,

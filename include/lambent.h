@@ -358,12 +358,12 @@ typedef struct lmb_nrc_opts {
     float ema_decay;       /* inference uses an exponential moving average
                               of the weights with this decay per step; 0 = the
                               trained weights themselves               (0)  */
-    uint32_t defer_specular; /* 1: a path whose scatters so far were all
+    uint32_t defer_specular; /* n > 0: a path whose scatters so far were all
                               specular (smooth glass, mirrors: roughness
                               below 0.1, or any at LOW quality) does not end
                               at its second vertex but at the vertex after
-                              its first rough scatter; 0: always the second
-                              vertex                                    (1)  */
+                              its first rough scatter, at most n vertices
+                              later; 0: always the second vertex (<= 64) (1) */
 } lmb_nrc_opts;
 
 void lmb_nrc_opts_default(lmb_nrc_opts *opts);

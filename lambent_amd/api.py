@@ -870,7 +870,7 @@ class NeuralCacheOpts:
     train_steps: int = 4
     seed: int = 1
     ema_decay: float = 0.0
-    defer_specular: bool = True
+    defer_specular: int = 1
 
 
 def _nrc_opts_c(o: NeuralCacheOpts) -> _NrcOptsC:
@@ -878,7 +878,7 @@ def _nrc_opts_c(o: NeuralCacheOpts) -> _NrcOptsC:
     c.learning_rate, c.beta1, c.beta2, c.epsilon = (o.learning_rate, o.beta1, o.beta2, o.epsilon)
     c.train_paths, c.train_steps, c.seed = o.train_paths, o.train_steps, o.seed
     c.ema_decay = o.ema_decay
-    c.defer_specular = int(bool(o.defer_specular))
+    c.defer_specular = int(o.defer_specular)
     return c
 
 

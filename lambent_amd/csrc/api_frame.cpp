@@ -91,6 +91,7 @@ struct lmb_data_buffers_s {
     DeviceBuffer out_color, out_normal, out_albedo;
     DeviceBuffer own_seed, pix_sum, ps_seed, ps_through, ps_accum;
     DeviceBuffer queue0, queue1, hits, counters;
+    DeviceBuffer hits_primary;  // allocated by the first trace of several samples
     DeviceBuffer gi, gi_old, info, mc, mc_old, wmc, wmc_old;
     SpillBuffer spill;
     uint64_t gi_count = 0, mc_count = 0, wmc_count = 0;
@@ -186,6 +187,7 @@ lmb_status alloc_window(lmb_data_buffers b, uint32_t width, uint32_t height) {
     if ((st = alloc_zero(b->queue0, n * 32))) return st;
     if ((st = alloc_zero(b->queue1, n * 32))) return st;
     if ((st = alloc_zero(b->hits, n * 16))) return st;
+    b->hits_primary.release();
     if ((st = alloc_zero(b->counters, kNumCounters * 4))) return st;
     b->gi_count = (b->flags & LMB_BUFFER_SPATIAL_RESAMPLING) ? n : 1;
     b->mc_count = (b->flags & LMB_BUFFER_MARKOV_CHAIN_SCREEN_SPACE) ? n : 1;
@@ -216,6 +218,7 @@ void refresh_view(lmb_data_buffers b) {
     f.queue[0] = b->queue0.as<float4>();
     f.queue[1] = b->queue1.as<float4>();
     f.hits = b->hits.as<uint4>();
+    f.hits_primary = b->hits_primary.as<uint4>();
     f.counters = b->counters.as<uint32_t>();
     f.spill = b->spill.ptr;
     f.gi = b->gi.as<Reservoir>();
@@ -426,10 +429,23 @@ lmb_status lmb_data_buffers_kernel_stats(lmb_data_buffers b, lmb_kernel_stats* s
     return LMB_OK;
 }
 
+// The bounce-0 hit buffer that a trace of several samples reuses.
+static lmb_status ensure_primary_hits(lmb_data_buffers b, uint32_t samples) {
+    const uint64_t bytes = (uint64_t)b->width * b->height * 16;
+    if (samples < 2 || b->hits_primary.bytes >= bytes) return LMB_OK;
+    lmb_status st = b->hits_primary.alloc(bytes);
+    refresh_view(b);
+    return st;
+}
+
 lmb_status lmb_trace(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
                      lmb_data_buffers bufs, uint32_t seed_offset, void* stream) {
     if (!tracer || !scene || !cam || !bufs) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
     const lmb_tracer_opts& o = tracer->opts;
+    if (o.guiding != LMB_GUIDING_MARKOV_CHAIN) {
+        lmb_status st = ensure_primary_hits(bufs, o.samples);
+        if (st) return st;
+    }
     TraceParams p;
     p.quality = o.quality;
     p.samples = o.samples;
@@ -462,6 +478,8 @@ lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera* c
     const lmb_tracer_opts& o = tracer->opts;
     if (o.guiding != LMB_GUIDING_OFF)
         return fail(LMB_ERR_INVALID_ARGUMENT, "the cached tracer needs guiding off");
+    lmb_status pst = ensure_primary_hits(bufs, o.samples);
+    if (pst) return pst;
     TraceParams p;
     p.quality = o.quality;
     p.samples = o.samples;
@@ -486,7 +504,7 @@ lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera* c
         np.lo[k] = scene->bounds_lo[k];
         np.inv_ext[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
     }
-    np.train_expect = (float)nrc->opts.train_paths / (float)std::max<uint32_t>(1, o.samples);
+    np.train_expect = (float)nrc->opts.train_paths;  // all in the trace's first sample
     np.defer_specular = nrc->opts.defer_specular;
     const hipStream_t hs = (hipStream_t)stream;
     // The first shading launch zeroes the record count (after counting the

@@ -232,8 +232,8 @@ static lmb_status validate(const lmb_nrc_opts& o) {
         return fail(LMB_ERR_INVALID_ARGUMENT, "train_paths must be <= 2^24");
     if (!(o.ema_decay >= 0.0f && o.ema_decay < 1.0f))
         return fail(LMB_ERR_INVALID_ARGUMENT, "ema_decay must be in [0, 1)");
-    if (o.defer_specular > 1u)
-        return fail(LMB_ERR_INVALID_ARGUMENT, "defer_specular must be 0 or 1");
+    if (o.defer_specular > kMaxBounces)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "defer_specular must be at most 64");
     return LMB_OK;
 }
 

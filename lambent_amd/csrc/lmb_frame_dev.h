@@ -44,6 +44,10 @@ struct FrameDev {
     // wavefront queues: slot -> 2 float4 (origin | path id bits, direction)
     float4* queue[2];
     uint4* hits;  // slot -> (leaf, t, u, v)
+    // Bounce-0 hits kept over the samples of one trace (null: not kept).
+    // Primary rays carry no jitter, so every sample of a pixel has the same
+    // camera ray and the same closest hit: it is traced once per trace.
+    uint4* hits_primary;
     uint32_t* counters;
     uint2* spill;
     // persistent state (data_buffer/mod.rs:103-114)
@@ -72,6 +76,10 @@ struct TraceParams {
 };
 
 // Threads of the persistent wavefront kernels for a frame of npix paths.
+// The frame as bounce 0 of a trace sees it: hits in hits_primary when the
+// trace has several samples and that buffer exists (the first sample traces
+// the camera rays into it, later samples shade from it without tracing).
+FrameDev primary_view(const FrameDev& f, const TraceParams& p);
 uint64_t frame_threads(uint64_t npix);   // generation / shading grid
 uint64_t extend_threads(uint64_t npix);  // persistent traversal grid (sizes the stack spill)
 

@@ -174,7 +174,7 @@ struct NrcTraceParams {
     float inv_ext[3];
     float train_expect;     // expected training records per sample
     uint32_t sample;
-    uint32_t defer_specular;  // cache vertices only after a rough scatter
+    uint32_t defer_specular;  // most vertices a specular chain moves the cache vertex
 };
 hipError_t launch_trace_nrc(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
                             const TraceParams& p, const NrcDev& d, NrcTraceParams np,

@@ -14,11 +14,13 @@
 //     T * max(cache(inputs), 0) from the inference kernel.
 // The emission of the second vertex belongs to S (the cache predicts
 // scattered radiance only).
-// With defer_specular, a vertex is a cache vertex only when the scatter
-// before it spread the path (a diffuse lobe or one of roughness >= 0.1): a
-// path seen through smooth glass or a mirror goes on to the vertex after
-// its first rough scatter (the spread rule of neural radiance caching), so
-// the cache's error there is blurred by a lobe instead of shown sharply.
+// With defer_specular = n > 0, a vertex is a cache vertex only when the
+// scatter before it spread the path (a diffuse lobe or one of roughness
+// >= 0.1) or when it is n vertices past the second: a path seen through
+// smooth glass or a mirror goes on to the vertex after its first rough
+// scatter (the spread rule of neural radiance caching), so the cache's error
+// there is blurred by a lobe instead of shown sharply, and specular chains
+// stop n vertices late at most (their short queues are latency-bound).
 // Every path still meets at most one cache vertex; training continuations
 // (rslot != 0) never meet another.
 #include <hip/hip_runtime.h>
@@ -191,7 +193,8 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
                         out_d = make_float4(ndir.x, ndir.y, ndir.z, 0.f);
                         f.ps_seed[pid] = seed;
                         f.ps_through[pid] = make_float4(color.x, color.y, color.z, picked);
-                        const bool arm = !np.defer_specular || scatter_spreads<Q>(it);
+                        const bool arm = bounce + 1u >= kCacheBounce + np.defer_specular ||
+                                         scatter_spreads<Q>(it);
                         f.ps_accum[pid] = make_float4(scol.x, scol.y, scol.z,
                                                       __uint_as_float(rslot | (arm ? kArmedBit : 0u)));
                     } else {
@@ -241,17 +244,30 @@ static hipError_t launch_nrc_q(const SceneDev& s, const CameraDev& cam, const Fr
                                hipStream_t stream, KernelTimer* tm) {
     const dim3 grid((uint32_t)(frame_threads(f.npix) / kNrcTraceBlock)), block(kNrcTraceBlock);
     hipError_t e;
+    const FrameDev f0 = primary_view(f, p);
+    const float train_expect = np.train_expect;
+    // Without training paths every path has ended by the bounce after its
+    // latest possible cache vertex.
+    const uint32_t short_bounces = std::min(p.bounces, kCacheBounce + np.defer_specular + 1u);
     for (uint32_t sample = 0; sample < p.samples; ++sample) {
+        // Training paths only in the first sample: later samples end at
+        // their cache vertices and skip the long latency-bound tail of the
+        // training paths.
         np.sample = sample;
+        np.train_expect = sample == 0 ? train_expect : 0.0f;
         e = launch_generate(cam, f, p, sample, stream);
         if (e != hipSuccess) return e;
-        for (uint32_t b = 0; b < p.bounces; ++b) {
-            if (tm) tm->begin(kKernelExtend, stream);
-            e = launch_extend(s, f, p, b, stream);
-            if (tm) tm->end(kKernelExtend, stream);
-            if (e != hipSuccess) return e;
+        const uint32_t bounces = np.train_expect > 0.0f ? p.bounces : short_bounces;
+        for (uint32_t b = 0; b < bounces; ++b) {
+            const FrameDev& fb = b == 0 ? f0 : f;
+            if (!(b == 0 && sample > 0 && f0.hits != f.hits)) {
+                if (tm) tm->begin(kKernelExtend, stream);
+                e = launch_extend(s, fb, p, b, stream);
+                if (tm) tm->end(kKernelExtend, stream);
+                if (e != hipSuccess) return e;
+            }
             if (tm) tm->begin(kKernelShade, stream);
-            hipLaunchKernelGGL(k_shade_nrc<Q>, grid, block, 0, stream, s, f, p, d, np, b);
+            hipLaunchKernelGGL(k_shade_nrc<Q>, grid, block, 0, stream, s, fb, p, d, np, b);
             if (tm) tm->end(kKernelShade, stream);
             e = hipGetLastError();
             if (e != hipSuccess) return e;

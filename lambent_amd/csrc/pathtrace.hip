@@ -246,6 +246,12 @@ __global__ __launch_bounds__(kPtBlock) void k_resolve(FrameDev f, TraceParams p)
     }
 }
 
+FrameDev primary_view(const FrameDev& f, const TraceParams& p) {
+    FrameDev f0 = f;
+    if (p.samples > 1 && f.hits_primary) f0.hits = f.hits_primary;
+    return f0;
+}
+
 hipError_t launch_extend(const SceneDev& s, const FrameDev& f, const TraceParams& p,
                          uint32_t bounce, hipStream_t stream) {
     const dim3 grid((uint32_t)(extend_threads(f.npix) / kPtBlock)), block(kPtBlock);
@@ -271,15 +277,19 @@ static hipError_t launch_plain(const SceneDev& s, const CameraDev& cam, const Fr
                                const TraceParams& p, hipStream_t stream, KernelTimer* tm) {
     const dim3 grid((uint32_t)(frame_threads(f.npix) / kPtBlock)), block(kPtBlock);
     const dim3 xgrid((uint32_t)(extend_threads(f.npix) / kPtBlock));
+    const FrameDev f0 = primary_view(f, p);
     for (uint32_t sample = 0; sample < p.samples; ++sample) {
         hipLaunchKernelGGL(k_generate, grid, block, 0, stream, cam, f, p, sample);
         for (uint32_t b = 0; b < p.bounces; ++b) {
-            if (tm) tm->begin(kKernelExtend, stream);
-            hipLaunchKernelGGL(k_extend, xgrid, block, 0, stream, s, f, p, b);
-            if (tm) tm->end(kKernelExtend, stream);
+            const FrameDev& fb = b == 0 ? f0 : f;
+            if (!(b == 0 && sample > 0 && f0.hits != f.hits)) {
+                if (tm) tm->begin(kKernelExtend, stream);
+                hipLaunchKernelGGL(k_extend, xgrid, block, 0, stream, s, fb, p, b);
+                if (tm) tm->end(kKernelExtend, stream);
+            }
             if (tm) tm->begin(kKernelShade, stream);
             hipLaunchKernelGGL((k_shade_plain_t<Q, false>), grid, block, 0, stream, s,
-                               (const SceneDev*)nullptr, f, p, b);
+                               (const SceneDev*)nullptr, fb, p, b);
             if (tm) tm->end(kKernelShade, stream);
         }
     }

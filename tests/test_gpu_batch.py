@@ -32,7 +32,9 @@ def single(items, seed, quality=api.Medium):
         b = api.DataBuffers(w, h, 1, api.BufferType(0))
         pipe.dispatch(scene, cam, b, seed)
         outs.append(b.output(api.Output.COLOR))
-        verts += b.path_vertices()
+        # the single-scene tracer reuses the first sample's camera-ray hits,
+        # the batched tracer traces every sample's
+        verts += b.path_vertices() + (OPTS.samples - 1) * w * h
     return outs, verts
 
 

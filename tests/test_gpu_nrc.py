@@ -218,3 +218,22 @@ def test_set_opts_freezes_training_in_the_tracer():
     assert np.isfinite(bufs.output(api.Output.COLOR)).all()
     with pytest.raises(api.LambentError):
         cache.set_opts(api.NeuralCacheOpts(train_steps=0))
+
+
+def test_cached_tracer_several_samples_per_trace():
+    """Training paths come from the first sample only; later samples reuse
+    its camera-ray hits and end at their cache vertices."""
+    spec = scenes.cornell_box(48, 48)
+    scene = spec.build()
+    pipe = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=4, bounces=8, rr_start=3, guiding=api.Guiding.OFF))
+    bufs = api.DataBuffers(48, 48, 1)
+    cache = api.NeuralCache(api.NeuralCacheOpts(train_paths=512, train_steps=2, seed=4))
+    for f in range(40):
+        pipe.dispatch_cached(scene, spec.camera(), bufs, cache, f)
+    st = cache.stats()
+    assert st["steps"] == 80
+    assert 0.6 * 512 < st["records"] < 1.4 * 512, st
+    assert st["queries"] > 48 * 48 * 0.5, st  # the last sample: no training paths
+    img = bufs.output(api.Output.COLOR)[..., :3]
+    assert np.isfinite(img).all() and img.mean() > 0

@@ -27,8 +27,12 @@ def _trace(spec, quality=api.Medium, samples=1, bounces=5, rr_start=None, seed_o
                            guiding=api.Guiding.OFF))
     bufs = api.DataBuffers(spec.width, spec.height, 1)
     pipe.dispatch(scene, spec.camera(), bufs, seed_offset)
+    # path_vertices counts the closest-hit queries issued: the camera rays of
+    # samples after the first reuse the first sample's hits.  The oracle
+    # traces them every sample.
+    reused = (samples - 1) * spec.width * spec.height
     return (bufs.output(api.Output.COLOR), bufs.output(api.Output.NORMAL),
-            bufs.output(api.Output.ALBEDO), bufs.path_vertices())
+            bufs.output(api.Output.ALBEDO), bufs.path_vertices() + reused)
 
 
 def rel_mse(img, ref):
