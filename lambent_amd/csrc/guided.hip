@@ -31,6 +31,7 @@
 #include <algorithm>
 
 #include "dev_common.h"
+#include "dev_compact.h"
 #include "dev_guide.h"
 #include "dev_restir.h"
 #include "dev_shade.h"
@@ -294,10 +295,10 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
     const uint64_t gtid = (uint64_t)blockIdx.x * kGBlock + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * kGBlock;
     const uint32_t count = f.counters[bounce];
-    const uint64_t count_r = ((uint64_t)count + 63) & ~63ull;
+    const uint64_t count_r = block_uniform_count<kGBlock>(count);
     const float4* __restrict__ qin = f.queue[bounce & 1];
     float4* __restrict__ qout = f.queue[(bounce + 1) & 1];
-    const uint32_t lane = __lane_id();
+    __shared__ ReserveLds<kGBlock, 1> rl;
     const float jitter_size = kGridSize * 1.5f;  // general.wesl:387
     for (uint64_t slot = gtid; slot < count_r; slot += stride) {
         bool cont = false;
@@ -476,17 +477,12 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
                 }
             }
         }
-        const uint64_t mask = __ballot(cont);
-        if (mask) {
-            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&f.counters[bounce + 1], (uint32_t)__popcll(mask));
-            base = __shfl(base, (int)leader);
-            if (cont) {
-                const uint32_t idx = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-                qout[2 * (uint64_t)idx] = out_o;
-                qout[2 * (uint64_t)idx + 1] = out_d;
-            }
+        // Compact the survivors into the next queue: one atomic per block.
+        uint32_t idx[1];
+        block_reserve<kGBlock, 1>({cont}, {f.counters + bounce + 1}, rl, idx);
+        if (cont) {
+            qout[2 * (uint64_t)idx[0]] = out_o;
+            qout[2 * (uint64_t)idx[0] + 1] = out_d;
         }
     }
 }

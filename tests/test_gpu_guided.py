@@ -38,9 +38,16 @@ def test_guided_matches_oracle_statistically():
     o, _, _, _ = oracle.trace_guided(oracle.Scene(spec), spec.camera().to_bytes(), 64, 64, st,
                                      samples=64, bounces=8, seed_offset=4242)
     o = o[..., :3]
-    e_ref, e_gpu = rel_mse(a, o), rel_mse(a, b)
     assert np.isfinite(a).all()
-    assert e_ref < 1.5 * e_gpu + 1e-3, (e_ref, e_gpu)
+    # Without next-event estimation the light is found by chance: a pixel's
+    # 64-sample mean can hold one sample hundreds of times the image mean, and
+    # such single events, in a, b or o, decide the plain relMSE.  The
+    # comparison is made on images clipped at 4x the oracle's mean (the same
+    # for all three); the means below are unclipped.
+    clip = 4.0 * float(o.mean())
+    ac, bc, oc = (np.minimum(x, clip) for x in (a, b, o))
+    e_ref, e_gpu = rel_mse(ac, oc), rel_mse(ac, bc)
+    assert e_ref < 1.5 * e_gpu + 1e-3, (e_ref, e_gpu, rel_mse(a, o), rel_mse(a, b))
     # heavy-tailed noise (small light, no next-event estimation): 64 spp means
     # of independent runs scatter by a few percent
     assert abs(a.mean() - o.mean()) / o.mean() < 0.06
