@@ -199,6 +199,16 @@ lmb_status lmb_scene_intersect_device(lmb_scene scene, const void *rays_dev,
  * n = A + B / lambda^2 through the material's IOR-range endpoints
  * n(760 nm), n(340 nm); see lmb_material_new's ior argument. */
 #define LMB_QUALITY_SPECTRAL 3u
+/* src/debug/mod.rs:1-58 debugging shaders: the camera ray's closest hit
+ * only (t_min / t_max from the options; samples, bounces, restir, Russian
+ * roulette and guiding are ignored); the colour output holds the value, the
+ * normal and albedo outputs zero, w = 1. */
+#define LMB_QUALITY_DEBUG_FRONT_FACE 16u  /* debug::FrontFace (front_face.wesl):
+                                             front (0, 1/(t+1), 0), back (1/(t+1), 0, 0) */
+#define LMB_QUALITY_DEBUG_REFLECTANCE 17u /* debug::Reflectance (reflectance.wesl):
+                                             the material's IOR pair / 5 in x, y */
+#define LMB_QUALITY_DEBUG_TANGENT 18u     /* debug::Tangent (tangent.wesl):
+                                             (correct_tangent(normalize(v0 - v1)) + 1) / 2 */
 
 /* path_tracing/mod.rs:25-34 RestirTemporalMode */
 #define LMB_RESTIR_DISABLED 0u

@@ -643,9 +643,21 @@ class Quality(enum.IntEnum):
     MEDIUM = 1
     HIGH = 2
     SPECTRAL = 3
+    DEBUG_FRONT_FACE = 16
+    DEBUG_REFLECTANCE = 17
+    DEBUG_TANGENT = 18
 
 
 Low, Medium, High, Spectral = Quality.LOW, Quality.MEDIUM, Quality.HIGH, Quality.SPECTRAL
+
+
+class debug:  # noqa: N801 - mirrors the Rust module name
+    """src/debug/mod.rs:1-58: shaders that show the camera ray's closest hit
+    (front face / back face shaded by 1 / (t + 1), the material's IOR pair,
+    the triangle's tangent)."""
+    FrontFace = Quality.DEBUG_FRONT_FACE
+    Reflectance = Quality.DEBUG_REFLECTANCE
+    Tangent = Quality.DEBUG_TANGENT
 
 
 class RestirTemporalMode(enum.IntEnum):

@@ -265,7 +265,10 @@ void lmb_tracer_opts_default(lmb_tracer_opts* o) {
 
 lmb_status lmb_tracer_create(const lmb_tracer_opts* opts, lmb_tracer* out) {
     if (!opts || !out) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (opts->quality > LMB_QUALITY_SPECTRAL) return fail(LMB_ERR_INVALID_ARGUMENT, "bad quality");
+    const bool debug_shader = opts->quality >= LMB_QUALITY_DEBUG_FRONT_FACE &&
+                              opts->quality <= LMB_QUALITY_DEBUG_TANGENT;
+    if (opts->quality > LMB_QUALITY_SPECTRAL && !debug_shader)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "bad quality");
     if (opts->samples == 0)  // NonZeroU32 in the reference
         return fail(LMB_ERR_INVALID_ARGUMENT, "samples must be non-zero");
     if (opts->bounces == 0 || opts->bounces > kMaxBounces)
@@ -459,7 +462,9 @@ lmb_status lmb_trace(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
     CameraDev cd;
     std::memcpy(&cd, cam, sizeof cd);
     hipError_t e;
-    if (o.guiding == LMB_GUIDING_MARKOV_CHAIN) {
+    if (o.quality > LMB_QUALITY_SPECTRAL) {
+        e = launch_debug(scene->dev, cd, bufs->dev, p, (hipStream_t)stream, bufs->timer.get());
+    } else if (o.guiding == LMB_GUIDING_MARKOV_CHAIN) {
         lmb_status st = ensure_guided(bufs);
         if (st) return st;
         e = launch_guided(scene->dev, cd, bufs->dev, bufs->guided, p, (hipStream_t)stream,
@@ -478,6 +483,8 @@ lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera* c
     const lmb_tracer_opts& o = tracer->opts;
     if (o.guiding != LMB_GUIDING_OFF)
         return fail(LMB_ERR_INVALID_ARGUMENT, "the cached tracer needs guiding off");
+    if (o.quality > LMB_QUALITY_SPECTRAL)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "the cached tracer has no debugging shaders");
     lmb_status pst = ensure_primary_hits(bufs, o.samples);
     if (pst) return pst;
     TraceParams p;

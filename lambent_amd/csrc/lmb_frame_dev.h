@@ -108,6 +108,9 @@ hipError_t launch_extend(const SceneDev& s, const FrameDev& f, const TraceParams
 hipError_t launch_resolve(const FrameDev& f, const TraceParams& p, hipStream_t stream);
 
 // One full trace (all samples, all bounces) on `stream`.
+// The debugging shaders (debug.hip; p.quality = LMB_QUALITY_DEBUG_*).
+hipError_t launch_debug(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
+                        const TraceParams& p, hipStream_t stream, KernelTimer* timer);
 hipError_t launch_trace(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
                         const TraceParams& p, hipStream_t stream, KernelTimer* timer);
 

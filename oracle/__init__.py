@@ -55,6 +55,8 @@ _sig("orc_scene_create", [_f32p, ctypes.c_uint64, _u32p, ctypes.c_uint64, _u32p,
                           ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                           ctypes.c_uint32], _vp)
 _sig("orc_scene_destroy", [_vp])
+_sig("orc_trace_debug", [_vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                         ctypes.c_float, ctypes.c_float, _f32p], None)
 _sig("orc_trace_plain", [_vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
                          ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
@@ -202,6 +204,15 @@ class Scene:
                                      0xFFFFFFFF if rr_start is None else rr_start, seed_offset,
                                      1 if fp64 else 0, _f(c), _f(n), _f(a))
         return c, n, a, verts
+
+    def trace_debug(self, camera_bytes: bytes, width: int, height: int, kind: int,
+                    tmin: float = 1e-4, tmax: float = 1000.0) -> np.ndarray:
+        """The debugging shaders (src/debug/*.wesl): kind 0 front face,
+        1 reflectance, 2 tangent; (height, width, 4) colour."""
+        cam = camera_floats(camera_bytes)
+        c = np.empty((height, width, 4), np.float32)
+        _lib.orc_trace_debug(self._h, _f(cam), width, height, kind, tmin, tmax, _f(c))
+        return c
 
     def background(self, direction) -> np.ndarray:
         d = np.ascontiguousarray(direction, np.float32)

@@ -1229,6 +1229,40 @@ uint64_t orc_trace_plain(void* h, const float* cam, uint32_t width, uint32_t hei
     return total;
 }
 
+// The debugging shaders (src/debug/front_face.wesl:52-63,
+// reflectance.wesl:49-57 + 92-96, tangent.wesl:52-72 + 111-122): the camera
+// ray's closest hit only.  kind 0 front face: front (0, 1/(t+1), 0), back
+// (1/(t+1), 0, 0); kind 1 reflectance: (ior_lo / 5, ior_hi / 5, 0); kind 2
+// tangent: (correct_tangent(normalize(v0 - v1)) + 1) / 2, the zero vector on
+// a miss.  Misses are 0 for kinds 0 and 1.  rgba per pixel, w = 1.
+void orc_trace_debug(void* h, const float* cam, uint32_t width, uint32_t height, uint32_t kind,
+                     float tmin, float tmax, float* out_color) {
+    const Scene& s = *(const Scene*)h;
+    const int64_t n = (int64_t)width * height;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < n; ++k) {
+        uint32_t x = (uint32_t)(k % width), y = (uint32_t)(k / width);
+        float o3[3], d3[3];
+        orc_create_ray(cam, x, y, width, height, o3, d3);
+        Hit hit = ray_hit(s, mk(o3[0], o3[1], o3[2]), mk(d3[0], d3[1], d3[2]), tmin, tmax);
+        F3 c = splat(0.0f);
+        if (kind == 0 && hit.hit) {
+            float shade = 1.0f / (hit.t + 1.0f);
+            c = hit.front_face ? mk(0.0f, shade, 0.0f) : mk(shade, 0.0f, 0.0f);
+        } else if (kind == 1 && hit.hit) {
+            c = mk(hit.n_lo / 5.0f, hit.n_hi / 5.0f, 0.0f);
+        } else if (kind == 2) {
+            F3 t = hit.hit ? hit.tangent : splat(0.0f);
+            c = mk((t.x + 1.0f) / 2.0f, (t.y + 1.0f) / 2.0f, (t.z + 1.0f) / 2.0f);
+        }
+        float* oc = out_color + 4 * k;
+        oc[0] = c.x;
+        oc[1] = c.y;
+        oc[2] = c.z;
+        oc[3] = 1.0f;
+    }
+}
+
 // Test hooks for single functions.
 void orc_oren_nayar(const float* incoming, const float* reflected, float roughness,
                     const float* color, float* out) {

@@ -348,3 +348,81 @@ def test_reference_test_scene_fixtures_decode():
     for px in np.unique(a[..., :3].reshape(-1, 3), axis=0):
         assert tuple(np.round(px * 255).astype(np.uint8)) in texels
     assert np.isfinite(c).all() and v >= 64 * 36 * 2
+
+
+def _debug_scene(v, idx, ior=None):
+    """Triangles in front of a camera at the origin looking down +z (60 deg,
+    24 x 24 px), one material."""
+    v4 = np.zeros((len(v), 4), np.float32)
+    v4[:, :3] = v
+    m = api.Material.new([0, 0], [1, 0], [1, 1], [0, 0], 0, None, None,
+                         api.EmissionScale.from_emission_scale(0.0), ior,
+                         api.MaterialType.Diffuse)
+    proj = api.perspective(60.0, 1.0, 0.1, 100.0)
+    view = api.look_to_rh((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0))
+    return scenes.SceneSpec(v4, np.array(idx, np.uint32), np.zeros(len(idx) // 3, np.uint32),
+                            api.materials_array([m]), [scenes._rgba(1, 1, 1)], [], [], None,
+                            api.uniform_background(0.0), proj, view, 24, 24)
+
+
+def _debug_quad(front: bool, ior=None):
+    """A 3 x 3 quad in the plane z = 3; `front` picks the winding
+    (counter-clockwise seen from the camera = front face)."""
+    v = [[-1.5, -1.5, 3], [1.5, -1.5, 3], [1.5, 1.5, 3], [-1.5, 1.5, 3]]
+    return _debug_scene(v, [0, 2, 1, 0, 3, 2] if front else [0, 1, 2, 0, 2, 3], ior)
+
+
+@pytest.mark.parametrize("front", [True, False])
+def test_debug_front_face_closed_form(front):
+    """front_face.wesl:52-63: colour 1 / (t + 1) in green on a front face, in
+    red on a back face, t the parametric distance (z = 3 over the direction's
+    z), 0 on a miss."""
+    spec = _debug_quad(front)
+    cam = spec.camera()
+    c = oracle.Scene(spec).trace_debug(cam.to_bytes(), 24, 24, 0)
+    rays = oracle.create_rays(cam.to_bytes(), 24, 24).reshape(24, 24, 8).astype(np.float64)
+    t = (3.0 - rays[..., 2]) / rays[..., 6]
+    inside = (np.abs(rays[..., 0] + t * rays[..., 4]) < 1.5) & \
+             (np.abs(rays[..., 1] + t * rays[..., 5]) < 1.5)
+    lit = c[..., 1] if front else c[..., 0]
+    dark = c[..., 0] if front else c[..., 1]
+    assert inside.mean() > 0.5 and (~inside).any()
+    np.testing.assert_allclose(lit[inside], 1.0 / (t[inside] + 1.0), rtol=2e-6)
+    assert (dark == 0).all() and (c[..., 2] == 0).all() and (c[..., 3] == 1).all()
+    assert (lit[~inside] == 0).all()
+
+
+def test_debug_reflectance_closed_form():
+    """reflectance.wesl:49-57: the material's refractive-index pair unpacked
+    from f16, over 5, in red and green."""
+    spec = _debug_quad(True, ior=api.refractive_indices.DIAMOND)
+    c = oracle.Scene(spec).trace_debug(spec.camera().to_bytes(), 24, 24, 1)
+    lo, hi = (np.float32(np.float16(x)) for x in api.refractive_indices.DIAMOND)
+    hit = c[..., 0] > 0
+    assert 0.5 < hit.mean() < 1.0
+    assert (c[hit][:, 0] == lo / np.float32(5)).all() and (c[hit][:, 1] == hi / np.float32(5)).all()
+    assert (c[~hit][:, :3] == 0).all()
+
+
+@pytest.mark.parametrize("d,expect", [
+    ((-3.0, 1.0, 0.5), (3.0, -1.0, -0.5)),   # x largest and negative: reversed
+    ((0.5, -4.0, 1.0), (-0.5, 4.0, -1.0)),   # y largest and negative: reversed
+    ((1.0, -1.0, 0.0), (1.0, -1.0, 0.0)),    # |x| = |y|: neither is largest, z >= 0 decides
+    ((1.0, 1.0, -1.0), (-1.0, -1.0, 1.0)),   # all equal: z < 0 reverses
+])
+def test_debug_tangent_closed_form(d, expect):
+    """tangent.wesl:52-72: correct_tangent(normalize(v0 - v1)) as
+    (d + 1) / 2 on a triangle whose v0 - v1 is d (the largest axis made
+    positive; ties leave the decision to z); a miss shows (0.5, 0.5, 0.5)."""
+    d = np.asarray(d, np.float64)
+    v0 = np.array([0.0, 0.0, 3.0])
+    side = np.cross(d, [0.0, 0.0, 1.0])
+    v = [v0, v0 - 0.5 * d / np.linalg.norm(d[:2]), v0 + side / np.linalg.norm(side)]
+    spec = _debug_scene(v, [0, 1, 2])
+    c = oracle.Scene(spec).trace_debug(spec.camera().to_bytes(), 24, 24, 2)
+    e = np.asarray(expect) / np.linalg.norm(expect)
+    want = (e + 1.0) / 2.0
+    hit = np.all(np.abs(c[..., :3] - want) < 2e-7, axis=-1)
+    miss = np.all(c[..., :3] == 0.5, axis=-1)
+    assert hit.sum() >= 4 and miss.any()
+    assert (hit | miss).all()

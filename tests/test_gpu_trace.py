@@ -207,3 +207,38 @@ def test_reference_test_scene_full_size():
     close = np.all(np.abs(color - c_o) <= 1e-4 * (1 + np.abs(c_o)), axis=-1)
     assert close.mean() > 0.97, f"only {close.mean():.4f} of pixels match the oracle"
     assert abs(verts - v_o) <= 0.01 * v_o
+
+
+@pytest.mark.parametrize("kind", [api.debug.FrontFace, api.debug.Reflectance, api.debug.Tangent],
+                         ids=["front_face", "reflectance", "tangent"])
+@pytest.mark.parametrize("scene_name", ["reference_test", "cornell", "procedural"])
+def test_debug_shaders_bit_exact(kind, scene_name):
+    """The debugging shaders (src/debug/*.wesl) against the oracle's
+    restatement: the camera ray's closest hit is bit-exact, and each shader
+    maps it by float arithmetic in the same order, so the images are equal."""
+    spec = {"reference_test": lambda: ref_scene.reference_test_scene(320, 180),
+            "cornell": lambda: scenes.cornell_box(96, 96),
+            "procedural": lambda: scenes.procedural(50_000, seed=5, width=160, height=90)}[scene_name]()
+    scene = spec.build()
+    pipe = api.RayTracer(kind).create_pipeline(api.PathTracerOpts(samples=3, bounces=5))
+    bufs = api.DataBuffers(spec.width, spec.height, 1)
+    pipe.dispatch(scene, spec.camera(), bufs, 7)
+    expect = oracle.Scene(spec).trace_debug(spec.camera().to_bytes(), spec.width, spec.height,
+                                            int(kind) - int(api.debug.FrontFace))
+    np.testing.assert_array_equal(bufs.output(api.Output.COLOR), expect)
+    zero = np.zeros((spec.height, spec.width, 4), np.float32)
+    zero[..., 3] = 1.0
+    np.testing.assert_array_equal(bufs.output(api.Output.NORMAL), zero)
+    np.testing.assert_array_equal(bufs.output(api.Output.ALBEDO), zero)
+    assert bufs.path_vertices() == spec.width * spec.height
+    distinct = len(np.unique(expect.reshape(-1, 4), axis=0))
+    assert distinct > (0 if kind == api.debug.Reflectance else 2)  # Cornell: one IOR pair
+
+
+def test_debug_shader_rejected_by_the_cached_tracer():
+    spec = scenes.cornell_box(16, 16)
+    pipe = api.RayTracer(api.debug.Tangent).create_pipeline(
+        api.PathTracerOpts(samples=1, guiding=api.Guiding.OFF))
+    with pytest.raises(api.LambentError):
+        pipe.dispatch_cached(spec.build(), spec.camera(), api.DataBuffers(16, 16, 1),
+                             api.NeuralCache(), 0)
