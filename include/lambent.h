@@ -154,7 +154,16 @@ typedef struct lmb_scene_desc {
     const lmb_material *materials;
     uint64_t num_materials;
     lmb_textures textures;       /* may be NULL: no textures, black sky */
+    uint32_t flags;              /* LMB_SCENE_* (0: opaque geometry)         */
 } lmb_scene_desc;
+
+/* Geometry without the OPAQUE flag (main.rs:469 builds opaque geometry):
+ * a candidate triangle becomes the hit only when its diffuse texel's alpha,
+ * sampled at the hit's texture coordinates, is not near zero (|a| >= 1e-4)
+ * or its material is TRANSPARENT (general.wesl:1057-1066); the debugging
+ * shaders test the alpha alone (front_face.wesl:82-91).  Costs a material
+ * and a texel fetch per candidate hit; batches take opaque scenes only. */
+#define LMB_SCENE_ALPHA_TEST 1u
 
 lmb_status lmb_scene_create(const lmb_scene_desc *desc, lmb_scene *out);
 lmb_status lmb_scene_destroy(lmb_scene scene);

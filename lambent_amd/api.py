@@ -107,7 +107,7 @@ class _SceneDescC(ctypes.Structure):
     _fields_ = [("vertices", _f32p), ("num_vertices", ctypes.c_uint64),
                 ("indices", _u32p), ("num_indices", ctypes.c_uint64),
                 ("material_idx", _u32p), ("materials", ctypes.POINTER(_MaterialC)),
-                ("num_materials", ctypes.c_uint64), ("textures", _vp)]
+                ("num_materials", ctypes.c_uint64), ("textures", _vp), ("flags", ctypes.c_uint32)]
 
 
 class _BvhStatsC(ctypes.Structure):
@@ -557,13 +557,19 @@ class BvhStats:
     device_bytes: int
 
 
+SCENE_ALPHA_TEST = 1  # LMB_SCENE_ALPHA_TEST
+
+
 class Scene:
     """Geometry + materials + textures on the GPU with a compressed 8-wide
-    BVH (replaces the BLAS/TLAS build and bind groups 0 and 2)."""
+    BVH (replaces the BLAS/TLAS build and bind groups 0 and 2).  ``opaque``:
+    the OPAQUE geometry flag of the reference's BLAS (main.rs:469); False
+    makes every candidate hit pass the any-hit alpha test of
+    general.wesl:1057-1066."""
 
     def __init__(self, vertices: np.ndarray, indices: Optional[np.ndarray],
                  material_idx: np.ndarray, materials: Sequence[Material] | np.ndarray,
-                 textures: Optional[TextureSet] = None):
+                 textures: Optional[TextureSet] = None, opaque: bool = True):
         v = np.ascontiguousarray(vertices, np.float32)
         if v.ndim != 2 or v.shape[1] not in (3, 4):
             raise ValueError("vertices must be (n, 3) or (n, 4) float32")
@@ -585,6 +591,7 @@ class Scene:
         d.materials = mats.ctypes.data_as(ctypes.POINTER(_MaterialC))
         d.num_materials = len(mats)
         d.textures = None if textures is None else textures.handle
+        d.flags = 0 if opaque else SCENE_ALPHA_TEST
         h = _vp()
         _check(_lib.lmb_scene_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h

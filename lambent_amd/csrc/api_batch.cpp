@@ -222,6 +222,9 @@ lmb_status lmb_batch_create(const lmb_batch_item* items, uint32_t n, const lmb_t
     uint64_t npix = 0;
     for (uint32_t i = 0; i < n; ++i) {
         if (!items[i].scene) return fail(LMB_ERR_INVALID_ARGUMENT, "item " + std::to_string(i) + " has no scene");
+        if (items[i].scene->dev.alpha_test)
+            return fail(LMB_ERR_INVALID_ARGUMENT,
+                        "item " + std::to_string(i) + ": batches take opaque scenes only");
         if (items[i].width == 0 || items[i].height == 0)
             return fail(LMB_ERR_INVALID_ARGUMENT, "item " + std::to_string(i) + " has an empty image");
         npix += (uint64_t)items[i].width * items[i].height;

@@ -300,6 +300,7 @@ lmb_status lmb_scene_create(const lmb_scene_desc* desc, lmb_scene* out) {
     if (ntri >= 0xFFFFFFFFull) return fail(LMB_ERR_INVALID_ARGUMENT, "too many triangles");
     if (desc->num_materials == 0 || !desc->materials)
         return fail(LMB_ERR_INVALID_ARGUMENT, "at least one material is required");
+    if (desc->flags & ~LMB_SCENE_ALPHA_TEST) return fail(LMB_ERR_INVALID_ARGUMENT, "unknown scene flags");
     for (uint64_t t = 0; t < ntri; ++t) {
         if (desc->material_idx[t] >= desc->num_materials)
             return fail(LMB_ERR_INVALID_ARGUMENT,
@@ -328,6 +329,7 @@ lmb_status lmb_scene_create(const lmb_scene_desc* desc, lmb_scene* out) {
         delete s;
         return st;
     }
+    s->dev.alpha_test = (desc->flags & LMB_SCENE_ALPHA_TEST) ? 1u : 0u;
     *out = s;
     return LMB_OK;
 }

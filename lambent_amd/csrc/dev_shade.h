@@ -79,14 +79,32 @@ __device__ __forceinline__ float4 unorm4(uint32_t t) {
     return make_float4((float)(t & 0xFFu) / 255.0f, (float)((t >> 8) & 0xFFu) / 255.0f,
                        (float)((t >> 16) & 0xFFu) / 255.0f, (float)(t >> 24) / 255.0f);
 }
-__device__ __forceinline__ float4 tex_sample(const SceneDev& s, uint32_t ti, float u, float v) {
-    const TexDesc d = s.tex[ti];
+__device__ __forceinline__ uint32_t tex_texel(const TexDesc* __restrict__ tex,
+                                              const uint32_t* __restrict__ texels, uint32_t ti,
+                                              float u, float v) {
+    const TexDesc d = tex[ti];
     float fx = floorf(u * (float)d.width);
     float fy = floorf(v * (float)d.height);
     fx = fminf(fmaxf(fx, 0.0f), (float)(d.width - 1));
     fy = fminf(fmaxf(fy, 0.0f), (float)(d.height - 1));
     uint32_t ix = (uint32_t)fx, iy = (uint32_t)fy;
-    return unorm4(s.texels[d.offset + iy * d.width + ix]);
+    return texels[d.offset + iy * d.width + ix];
+}
+__device__ __forceinline__ float4 tex_sample(const SceneDev& s, uint32_t ti, float u, float v) {
+    return unorm4(tex_texel(s.tex, s.texels, ti, u, v));
+}
+
+// Texture coordinates of a hit (general.wesl:1060-1061, 1118-1119):
+// barycentrics (u, v) weight vertices 1 and 2, so
+// tex = tex_pos_1 (1 - u - v) + tex_pos_2 u + tex_pos_3 v.
+__device__ __forceinline__ void hit_tex_coords(const Material& m, float bu, float bv, float* tu,
+                                               float* tv) {
+    const float bz = (1.0f - bu) - bv;
+    const float t1u = f16_to_f32(m.tex_pos_1 & 0xFFFFu), t1v = f16_to_f32(m.tex_pos_1 >> 16);
+    const float t2u = f16_to_f32(m.tex_pos_2 & 0xFFFFu), t2v = f16_to_f32(m.tex_pos_2 >> 16);
+    const float t3u = f16_to_f32(m.tex_pos_3 & 0xFFFFu), t3v = f16_to_f32(m.tex_pos_3 >> 16);
+    *tu = fmaf(t1u, bz, fmaf(t2u, bu, t3u * bv));
+    *tv = fmaf(t1v, bz, fmaf(t2v, bu, t3v * bv));
 }
 __device__ __forceinline__ float4 tex_load(const SceneDev& s, uint32_t ti, uint32_t x,
                                            uint32_t y) {
@@ -176,13 +194,8 @@ __device__ __forceinline__ Isect make_isect(const SceneDev& s, uint32_t leaf, fl
     V3 normal = normalize3(cross3(tangent, v0 - v2));
     it.normal = dot3(dir, normal) > 0.0f ? -normal : normal;
     it.tangent = tangent;
-    // bary = (u, v, 1 - u - v); tex = tp1 * bary.z + tp2 * bary.x + tp3 * bary.y
-    const float bz = (1.0f - bu) - bv;
-    const float t1u = f16_to_f32(m.tex_pos_1 & 0xFFFFu), t1v = f16_to_f32(m.tex_pos_1 >> 16);
-    const float t2u = f16_to_f32(m.tex_pos_2 & 0xFFFFu), t2v = f16_to_f32(m.tex_pos_2 >> 16);
-    const float t3u = f16_to_f32(m.tex_pos_3 & 0xFFFFu), t3v = f16_to_f32(m.tex_pos_3 >> 16);
-    const float tu = fmaf(t1u, bz, fmaf(t2u, bu, t3u * bv));
-    const float tv = fmaf(t1v, bz, fmaf(t2v, bu, t3v * bv));
+    float tu, tv;
+    hit_tex_coords(m, bu, bv, &tu, &tv);
     const float4 dc = tex_sample(s, s.diffuse_base + idx_diffuse, tu, tv);
     const float4 rc = tex_load(s, s.recolour, m.tex_pos_recolour & 0xFFFFu,
                                m.tex_pos_recolour >> 16);

@@ -316,6 +316,7 @@ __device__ __forceinline__ Hit trace_closest(const Bvh8Node* __restrict__ nodes,
 // queue tags each ray with a scene index and each lane binds that scene's
 // BVH when it starts the ray.
 struct OneScene {
+    static constexpr bool kAlphaTest = false;  // opaque: every candidate hit counts
     const Bvh8Node* nodes_;
     const LeafTri* tris_;
     uint32_t num_nodes_;
@@ -328,6 +329,7 @@ struct OneScene {
 };
 
 struct SceneTable {
+    static constexpr bool kAlphaTest = false;
     const SceneDev* table;
     const Bvh8Node* nodes_ = nullptr;
     const LeafTri* tris_ = nullptr;
@@ -405,7 +407,9 @@ __device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t tbase, 
             if (intersect_triangle(tv.sh, tv.r.ox, tv.r.oy, tv.r.oz, a, b, cc, tv.r.tmin, tv.h.t,
                                    &t, &u, &v)) {
                 const uint32_t prim = __float_as_uint(a.w);
-                if (t < tv.h.t || prim < tv.h.prim) {
+                bool take = t < tv.h.t || prim < tv.h.prim;
+                if constexpr (Scenes::kAlphaTest) take = take && sc.accept(b, u, v);
+                if (take) {
                     tv.h.t = t;
                     tv.h.u = u;
                     tv.h.v = v;
@@ -460,7 +464,9 @@ __device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t tbase, 
             const float4* tp = reinterpret_cast<const float4*>(tris + leaf);
             const float4 a = tp[0], b = tp[1], cc = tp[2];
             const float tmax = from_orderable((uint32_t)(ts.key[owner] >> 32));
-            if (intersect_triangle(sh, ox, oy, oz, a, b, cc, otmin, tmax, &t, &u, &v)) {
+            bool found = intersect_triangle(sh, ox, oy, oz, a, b, cc, otmin, tmax, &t, &u, &v);
+            if constexpr (Scenes::kAlphaTest) found = found && sc.accept(b, u, v);
+            if (found) {
                 hit = true;
                 mykey = ((unsigned long long)orderable(t) << 32) | __float_as_uint(a.w);
                 atomicMin(ts.key + owner, mykey);

@@ -23,6 +23,7 @@ struct SceneDev {
     uint32_t attr_base, num_attr;
     uint32_t recolour;              // TexDesc index of the recolour texture
     uint32_t bg_base;               // six TexDesc entries: +x -x +y -y +z -z
+    uint32_t alpha_test;            // non-opaque geometry: any-hit alpha test
 };
 
 // Number of compute units of the current device (cached).

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 
+#include "dev_anyhit.h"
 #include "dev_common.h"
 #include "dev_compact.h"
 #include "dev_shade.h"
@@ -127,6 +128,8 @@ constexpr int kRefill = LMB_REFILL;  // idle lanes that make a wave fetch more r
 #define LMB_EXTEND_WAVES 1
 #endif
 
+// kAlpha: the scene's geometry is not opaque (any-hit alpha test).
+template <bool kAlpha>
 __global__ __launch_bounds__(kPtBlock, LMB_EXTEND_WAVES) void k_extend(SceneDev s, FrameDev f, TraceParams p,
                                                      uint32_t bounce) {
     __shared__ uint2 lds_stack[kStackLds * kPtBlock];
@@ -145,8 +148,23 @@ __global__ __launch_bounds__(kPtBlock, LMB_EXTEND_WAVES) void k_extend(SceneDev 
     st.spill = f.spill + gtid * (kStackMax - kStackLds);
     LMB_TRI_SHARE(kPtBlock, ts);
     QueueSource src{f.queue[bounce & 1], f.hits, p.tmin, p.tmax};
-    traverse_queue<kPtBlock, kRefill>(s.nodes, s.tris, s.num_nodes, count,
-                                      fetch_set(f.counters, bounce), st, ts, src);
+    if constexpr (kAlpha) {
+        AlphaScene sc = alpha_scene(s, p.quality <= kQualitySpectral);
+        UntaggedSource<QueueSource> us{src};
+        traverse_queue_t<kPtBlock>(sc, count, fetch_set(f.counters, bounce), (uint32_t)kRefill, st,
+                                   ts, us);
+    } else {
+        traverse_queue<kPtBlock, kRefill>(s.nodes, s.tris, s.num_nodes, count,
+                                          fetch_set(f.counters, bounce), st, ts, src);
+    }
+}
+
+static void launch_extend_kernel(const SceneDev& s, const FrameDev& f, const TraceParams& p,
+                                 uint32_t bounce, dim3 grid, hipStream_t stream) {
+    if (s.alpha_test)
+        hipLaunchKernelGGL(k_extend<true>, grid, dim3(kPtBlock), 0, stream, s, f, p, bounce);
+    else
+        hipLaunchKernelGGL(k_extend<false>, grid, dim3(kPtBlock), 0, stream, s, f, p, bounce);
 }
 
 // kTable: the queue's rays carry a scene index (direction .w) into `table`
@@ -254,8 +272,8 @@ FrameDev primary_view(const FrameDev& f, const TraceParams& p) {
 
 hipError_t launch_extend(const SceneDev& s, const FrameDev& f, const TraceParams& p,
                          uint32_t bounce, hipStream_t stream) {
-    const dim3 grid((uint32_t)(extend_threads(f.npix) / kPtBlock)), block(kPtBlock);
-    hipLaunchKernelGGL(k_extend, grid, block, 0, stream, s, f, p, bounce);
+    const dim3 grid((uint32_t)(extend_threads(f.npix) / kPtBlock));
+    launch_extend_kernel(s, f, p, bounce, grid, stream);
     return hipGetLastError();
 }
 
@@ -284,7 +302,7 @@ static hipError_t launch_plain(const SceneDev& s, const CameraDev& cam, const Fr
             const FrameDev& fb = b == 0 ? f0 : f;
             if (!(b == 0 && sample > 0 && f0.hits != f.hits)) {
                 if (tm) tm->begin(kKernelExtend, stream);
-                hipLaunchKernelGGL(k_extend, xgrid, block, 0, stream, s, fb, p, b);
+                launch_extend_kernel(s, fb, p, b, xgrid, stream);
                 if (tm) tm->end(kKernelExtend, stream);
             }
             if (tm) tm->begin(kKernelShade, stream);

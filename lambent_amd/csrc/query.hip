@@ -5,6 +5,7 @@
 #include <algorithm>
 
 #include "dev_common.h"
+#include "dev_anyhit.h"
 #include "dev_traverse.h"
 #include "lmb_scene_dev.h"
 
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(kQueryBlock) void k_intersect(SceneDev s, const Ray
     st.spill = spill + gtid * (kStackMax - kStackLds);
     LMB_TRI_SHARE(kQueryBlock, ts);
     RaySource src{rays, hits};
-    traverse_queue<kQueryBlock, kQueryRefill>(s.nodes, s.tris, s.num_nodes, n, fetch, st, ts, src);
+    traverse_scene<kQueryBlock, kQueryRefill>(s, n, fetch, st, ts, src, true);
 }
 
 __global__ __launch_bounds__(kQueryBlock) void k_primary(SceneDev s, CameraDev cam,
@@ -102,8 +103,7 @@ __global__ __launch_bounds__(kQueryBlock) void k_primary(SceneDev s, CameraDev c
     st.spill = spill + gtid * (kStackMax - kStackLds);
     LMB_TRI_SHARE(kQueryBlock, ts);
     CameraSource src{cam, width, height, tmin, tmax, prim_out, t_out};
-    traverse_queue<kQueryBlock, kQueryRefill>(s.nodes, s.tris, s.num_nodes,
-                                              (uint64_t)width * height, fetch, st, ts, src);
+    traverse_scene<kQueryBlock, kQueryRefill>(s, (uint64_t)width * height, fetch, st, ts, src, true);
 }
 
 hipError_t launch_intersect(const SceneDev& s, const void* rays, uint64_t n, void* hits,

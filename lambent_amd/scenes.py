@@ -45,6 +45,7 @@ class SceneSpec:
     view: np.ndarray = None
     width: int = 256
     height: int = 256
+    opaque: bool = True                   # False: any-hit alpha test
 
     @property
     def num_triangles(self) -> int:
@@ -60,7 +61,8 @@ class SceneSpec:
     def build(self) -> api.Scene:
         """Uploads the scene to the GPU (BVH build included)."""
         tex = self.texture_set()
-        scene = api.Scene(self.vertices, self.indices, self.material_idx, self.materials, tex)
+        scene = api.Scene(self.vertices, self.indices, self.material_idx, self.materials, tex,
+                          opaque=self.opaque)
         scene._textures = tex
         return scene
 

@@ -55,6 +55,7 @@ _sig("orc_scene_create", [_f32p, ctypes.c_uint64, _u32p, ctypes.c_uint64, _u32p,
                           ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                           ctypes.c_uint32], _vp)
 _sig("orc_scene_destroy", [_vp])
+_sig("orc_scene_set_alpha_test", [_vp, ctypes.c_int], None)
 _sig("orc_trace_debug", [_vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                          ctypes.c_float, ctypes.c_float, _f32p], None)
 _sig("orc_trace_plain", [_vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
@@ -186,6 +187,7 @@ class Scene:
                                         _u(mi), _u(mats), len(mats), _u(texels), len(texels),
                                         _u(d), len(images), diffuse_base, emission_base,
                                         attr_base, recolour, bg_base)
+        _lib.orc_scene_set_alpha_test(self._h, 0 if getattr(spec, "opaque", True) else 1)
         self.spec = spec
 
     def trace_plain(self, camera_bytes: bytes, width: int, height: int, quality: int = 1,

@@ -167,11 +167,16 @@ static inline void tri_verts(const float* verts, const uint32_t* idx, uint64_t t
     *c = verts + 4 * i2;
 }
 
+// Any-hit filter of non-opaque geometry: accept(ctx, prim, u, v) decides
+// whether a candidate counts (null: every candidate counts).
+typedef bool (*OrcAccept)(const void* ctx, uint32_t prim, float u, float v);
+
 static inline void consider(const float* r, const float* a, const float* b, const float* c,
-                            uint32_t prim, OrcHit* h) {
+                            uint32_t prim, OrcHit* h, OrcAccept accept = nullptr,
+                            const void* ctx = nullptr) {
     float t, u, v;
     if (orc_watertight(r, r + 4, a, b, c, r[3], h->t, &t, &u, &v)) {
-        if (t < h->t || prim < h->prim) {
+        if ((t < h->t || prim < h->prim) && (!accept || accept(ctx, prim, u, v))) {
             h->t = t;
             h->u = u;
             h->v = v;
@@ -309,7 +314,9 @@ static bool orc_box(const OrcNode& n, const double* o, const double* inv, double
     return tn <= tf * (1.0 + 1e-9) + 1e-12;
 }
 
-void orc_bvh_intersect1(void* hb, const float* r, OrcHit* out) {
+// Closest hit among the candidates that `accept` takes (all when null).
+void orc_bvh_intersect1_filtered(void* hb, const float* r, OrcHit* out, OrcAccept accept,
+                                 const void* ctx) {
     OrcBvh* b = (OrcBvh*)hb;
     {
         OrcHit h{0xFFFFFFFFu, r[7], 0.f, 0.f};
@@ -328,7 +335,7 @@ void orc_bvh_intersect1(void* hb, const float* r, OrcHit* out) {
                         uint32_t p = b->prims[k];
                         const float *a, *bb, *c;
                         tri_verts(b->verts, b->idx, p, &a, &bb, &c);
-                        consider(r, a, bb, c, p, &h);
+                        consider(r, a, bb, c, p, &h, accept, ctx);
                     }
                 } else {
                     stack[sp++] = nd.left;
@@ -338,6 +345,10 @@ void orc_bvh_intersect1(void* hb, const float* r, OrcHit* out) {
         }
         *out = h;
     }
+}
+
+void orc_bvh_intersect1(void* hb, const float* r, OrcHit* out) {
+    orc_bvh_intersect1_filtered(hb, r, out, nullptr, nullptr);
 }
 
 void orc_bvh_intersect(void* hb, const float* rays, uint64_t n, OrcHit* hits) {

@@ -39,6 +39,9 @@ struct OrcHit {
     float t, u, v;
 };
 void orc_bvh_intersect1(void* hb, const float* ray, OrcHit* hit);
+typedef bool (*OrcAccept)(const void* ctx, uint32_t prim, float u, float v);
+void orc_bvh_intersect1_filtered(void* hb, const float* ray, OrcHit* hit, OrcAccept accept,
+                                 const void* ctx);
 }
 
 namespace {
@@ -109,6 +112,7 @@ struct Scene {
     std::vector<uint32_t> texels;
     std::vector<uint32_t> descs;  // offset, width, height per texture
     uint32_t diffuse_base, emission_base, attr_base, recolour, bg_base;
+    bool alpha_test = false;  // non-opaque geometry: any-hit alpha test
 };
 
 struct Tex4 {
@@ -199,10 +203,43 @@ F3 correct_tangent(F3 dir) {
     return rev ? neg(dir) : dir;
 }
 
-Hit ray_hit(const Scene& s, F3 org, F3 dir, float tmin, float tmax) {
+// Texture coordinates of a hit: tex_pos_1 (1 - u - v) + tex_pos_2 u +
+// tex_pos_3 v (general.wesl:1060-1061, 1118-1119).
+void hit_tex_coords(const uint32_t* m, float bx, float by, float* tu, float* tv) {
+    float bz = (1.0f - bx) - by;
+    *tu = std::fma(half_to_float(m[0] & 0xFFFFu), bz,
+                   std::fma(half_to_float(m[1] & 0xFFFFu), bx, half_to_float(m[2] & 0xFFFFu) * by));
+    *tv = std::fma(half_to_float(m[0] >> 16), bz,
+                   std::fma(half_to_float(m[1] >> 16), bx, half_to_float(m[2] >> 16) * by));
+}
+
+// Any-hit rule of non-opaque geometry (general.wesl:1057-1066): the diffuse
+// texel's alpha is not near zero, or (path-tracing shaders) the material is
+// TRANSPARENT; the debugging shaders test the alpha alone.
+struct AcceptCtx {
+    const Scene* s;
+    bool transparent_passes;
+};
+bool alpha_accept(const void* c, uint32_t prim, float u, float v) {
+    const AcceptCtx& a = *(const AcceptCtx*)c;
+    const uint32_t* m = &a.s->mats[8 * a.s->matidx[prim]];
+    if (a.transparent_passes && (m[5] >> 16) == 2u) return true;
+    float tu, tv;
+    hit_tex_coords(m, u, v, &tu, &tv);
+    float alpha = sample_tex(*a.s, a.s->diffuse_base + (m[4] & 0xFFFFu), tu, tv).a;
+    return !(alpha < 0.0001f && alpha > -0.0001f);
+}
+
+Hit ray_hit(const Scene& s, F3 org, F3 dir, float tmin, float tmax,
+            bool transparent_passes = true) {
     float ray[8] = {org.x, org.y, org.z, tmin, dir.x, dir.y, dir.z, tmax};
     OrcHit oh;
-    orc_bvh_intersect1(s.bvh, ray, &oh);
+    if (s.alpha_test) {
+        AcceptCtx ctx{&s, transparent_passes};
+        orc_bvh_intersect1_filtered(s.bvh, ray, &oh, alpha_accept, &ctx);
+    } else {
+        orc_bvh_intersect1(s.bvh, ray, &oh);
+    }
     Hit h;
     std::memset(&h, 0, sizeof h);
     if (oh.prim == 0xFFFFFFFFu) return h;
@@ -226,11 +263,8 @@ Hit ray_hit(const Scene& s, F3 org, F3 dir, float tmin, float tmax) {
     F3 normal = normalize(cross(tangent, sub(v0, v2)));
     h.normal = dot(dir, normal) > 0.0f ? neg(normal) : normal;
     h.tangent = tangent;
-    float bx = oh.u, by = oh.v, bz = (1.0f - oh.u) - oh.v;
-    float tu = std::fma(half_to_float(m[0] & 0xFFFFu), bz,
-                        std::fma(half_to_float(m[1] & 0xFFFFu), bx, half_to_float(m[2] & 0xFFFFu) * by));
-    float tv = std::fma(half_to_float(m[0] >> 16), bz,
-                        std::fma(half_to_float(m[1] >> 16), bx, half_to_float(m[2] >> 16) * by));
+    float tu, tv;
+    hit_tex_coords(m, oh.u, oh.v, &tu, &tv);
     Tex4 dt = sample_tex(s, s.diffuse_base + idx_diffuse, tu, tv);
     Tex4 rc = texel_value(s, s.recolour, m[3] & 0xFFFFu, m[3] >> 16);
     h.color = mk(dt.r * rc.r, dt.g * rc.g, dt.b * rc.b);
@@ -1160,6 +1194,8 @@ void* orc_scene_create(const float* verts, uint64_t nverts, const uint32_t* idx,
     return s;
 }
 
+void orc_scene_set_alpha_test(void* h, int on) { ((Scene*)h)->alpha_test = on != 0; }
+
 void orc_scene_destroy(void* h) {
     Scene* s = (Scene*)h;
     orc_bvh_destroy(s->bvh);
@@ -1244,7 +1280,7 @@ void orc_trace_debug(void* h, const float* cam, uint32_t width, uint32_t height,
         uint32_t x = (uint32_t)(k % width), y = (uint32_t)(k / width);
         float o3[3], d3[3];
         orc_create_ray(cam, x, y, width, height, o3, d3);
-        Hit hit = ray_hit(s, mk(o3[0], o3[1], o3[2]), mk(d3[0], d3[1], d3[2]), tmin, tmax);
+        Hit hit = ray_hit(s, mk(o3[0], o3[1], o3[2]), mk(d3[0], d3[1], d3[2]), tmin, tmax, false);
         F3 c = splat(0.0f);
         if (kind == 0 && hit.hit) {
             float shade = 1.0f / (hit.t + 1.0f);

@@ -426,3 +426,34 @@ def test_debug_tangent_closed_form(d, expect):
     miss = np.all(c[..., :3] == 0.5, axis=-1)
     assert hit.sum() >= 4 and miss.any()
     assert (hit | miss).all()
+
+
+@pytest.mark.parametrize("front_type", [api.MaterialType.Diffuse, api.MaterialType.Transparent])
+@pytest.mark.parametrize("opaque", [False, True])
+def test_alpha_tested_any_hit_closed_form(front_type, opaque):
+    """general.wesl:1057-1066 on non-opaque geometry: a camera ray passes the
+    cut-out quad where its texel's alpha is zero and hits the quad behind
+    (t = 4 / d_z instead of 2 / d_z); a TRANSPARENT material always counts
+    for the path tracer but not for the debugging shaders, which test the
+    alpha alone; opaque geometry ignores the alpha."""
+    import cutout
+    spec = cutout.cutout_scene(front_type, opaque)
+    cam = spec.camera().to_bytes()
+    sc = oracle.Scene(spec)
+    rays = oracle.create_rays(cam, 48, 48)
+    inside, opaque_texel, margin = cutout.expected_front(rays)
+    clear = margin > 1e-3
+    dz = rays[:, 6].astype(np.float64)
+    ff = sc.trace_debug(cam, 48, 48, 0).reshape(-1, 4)
+    shade = np.maximum(ff[:, 0], ff[:, 1]).astype(np.float64)
+    front_debug = inside & (opaque | opaque_texel)
+    t_expect = np.where(front_debug, 2.0 / dz, 4.0 / dz)
+    np.testing.assert_allclose(shade[clear], 1.0 / (t_expect[clear] + 1.0), rtol=1e-5)
+    assert (inside & ~opaque_texel & clear).sum() > 50 and (inside & opaque_texel & clear).sum() > 50
+    # the path tracer's rule, seen in the albedo output: front texel colour or white
+    _, _, alb, _ = sc.trace_plain(cam, 48, 48, quality=1, samples=1, bounces=1)
+    alb = alb.reshape(-1, 4)
+    front_pt = inside & (opaque | opaque_texel | (front_type == api.MaterialType.Transparent))
+    white = np.all(alb[:, :3] == 1.0, axis=1)
+    assert (white[clear & ~front_pt]).all()
+    assert (~white[clear & front_pt]).all()
