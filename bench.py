@@ -85,7 +85,8 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
     """Time-to-relMSE of the cached tracer against the plain tracer, both with
     8-bounce paths (RR from bounce 3), judged against a plain reference of
     ref_spp samples accumulated in fp64.  The cache starts untrained: `warm`
-    1-spp frames (timed, not accumulated), then `frames` frames of `spp`
+    training frames (lmb_trace_nrc_train: about 65536 paths at random pixels
+    train it, no image; timed), then `frames` frames of `spp`
     samples (the camera rays are traced once per frame, training paths come
     from the first sample), `steps` Adam steps per frame over the frame's
     65536 training records.  Times are wall times of the dispatches (host
@@ -126,7 +127,7 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
     for f in range(warm):
         api.sync()
         t0 = time.perf_counter()
-        warm_pipe.dispatch_cached(scene, cam, b, cache, f)
+        warm_pipe.train_cached(scene, cam, b, cache, f)
         api.sync()
         t += time.perf_counter() - t0
     t_warm = t
@@ -169,8 +170,8 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
     return {
         "workload": f"procedural {W}x{H}, 8 bounces, RR from 3; cache: paths end at the second "
                     f"vertex (a specular scatter moves it one vertex on), 65536 training paths "
-                    f"and {steps} Adam steps (lr {lr:g}) per frame; {warm} warm-up frames of "
-                    f"1 spp (timed, not accumulated), then frames of {spp} spp",
+                    f"and {steps} Adam steps (lr {lr:g}) per frame; {warm} training frames "
+                    f"(no image, timed), then frames of {spp} spp",
         "reference": f"plain tracer {ref_spp} spp, fp64 accumulation of {chunk}-spp dispatches",
         "plain": plain_pts,
         "cache_points": pts,

@@ -430,6 +430,15 @@ lmb_status lmb_nrc_stats_get(lmb_nrc nrc, lmb_nrc_stats *stats);
 lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera *cam,
                          lmb_data_buffers bufs, uint32_t seed_offset, lmb_nrc nrc,
                          void *stream);
+/* No reference counterpart.  A training frame of the cached tracer: about
+ * train_paths camera paths at random pixels (seeded as in lmb_trace_nrc's
+ * first sample) become training records at their cache vertices and are
+ * traced to full depth, then train_steps Adam steps.  Writes no image (the
+ * buffers' outputs keep their contents); one sample whatever the tracer's
+ * sample count.  For warming up a cache faster than with full frames. */
+lmb_status lmb_trace_nrc_train(lmb_tracer tracer, lmb_scene scene, const lmb_camera *cam,
+                               lmb_data_buffers bufs, uint32_t seed_offset, lmb_nrc nrc,
+                               void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Batched tracer                                                      */

@@ -221,6 +221,8 @@ _OPTIONAL = {
     "lmb_nrc_stats_get": ([_vp, ctypes.POINTER(_NrcStatsC)], ctypes.c_int32),
     "lmb_trace_nrc": ([_vp, _vp, ctypes.POINTER(_CameraC), _vp, ctypes.c_uint32, _vp, _vp],
                       ctypes.c_int32),
+    "lmb_trace_nrc_train": ([_vp, _vp, ctypes.POINTER(_CameraC), _vp, ctypes.c_uint32, _vp,
+                             _vp], ctypes.c_int32),
     "lmb_batch_config_default": ([ctypes.POINTER(_BatchConfigC)], None),
     "lmb_batch_default_sweep": ([ctypes.POINTER(_BatchConfigC), ctypes.c_uint32], ctypes.c_uint32),
     "lmb_batch_create": ([_vp, ctypes.c_uint32, ctypes.POINTER(_TracerOptsC),
@@ -730,6 +732,14 @@ class Pipeline:
         them train it).  No counterpart in the reference."""
         _check(_lib.lmb_trace_nrc(self._h, scene.handle, ctypes.byref(camera._c), buffers.handle,
                                   int(seed_offset) & 0xFFFFFFFF, cache.handle, stream))
+
+    def train_cached(self, scene: Scene, camera: Camera, buffers: "DataBuffers",
+                     cache: "NeuralCache", seed_offset: int = 0, stream=None) -> None:
+        """A training frame: about cache.opts.train_paths paths at random
+        pixels train the cache; no image.  No counterpart in the reference."""
+        _check(_lib.lmb_trace_nrc_train(self._h, scene.handle, ctypes.byref(camera._c),
+                                        buffers.handle, int(seed_offset) & 0xFFFFFFFF,
+                                        cache.handle, stream))
 
     def close(self):
         if self._h:

@@ -476,8 +476,9 @@ lmb_status lmb_trace(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
     return LMB_OK;
 }
 
-lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
-                         lmb_data_buffers bufs, uint32_t seed_offset, lmb_nrc nrc, void* stream) {
+static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
+                            lmb_data_buffers bufs, uint32_t seed_offset, lmb_nrc nrc, void* stream,
+                            bool train_only) {
     if (!tracer || !scene || !cam || !bufs || !nrc)
         return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
     const lmb_tracer_opts& o = tracer->opts;
@@ -485,11 +486,13 @@ lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera* c
         return fail(LMB_ERR_INVALID_ARGUMENT, "the cached tracer needs guiding off");
     if (o.quality > LMB_QUALITY_SPECTRAL)
         return fail(LMB_ERR_INVALID_ARGUMENT, "the cached tracer has no debugging shaders");
-    lmb_status pst = ensure_primary_hits(bufs, o.samples);
-    if (pst) return pst;
+    if (!train_only) {
+        lmb_status pst = ensure_primary_hits(bufs, o.samples);
+        if (pst) return pst;
+    }
     TraceParams p;
     p.quality = o.quality;
-    p.samples = o.samples;
+    p.samples = train_only ? 1u : o.samples;
     p.tmin = o.t_min;
     p.tmax = o.t_max;
     p.restir = o.restir;
@@ -513,6 +516,7 @@ lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera* c
     }
     np.train_expect = (float)nrc->opts.train_paths;  // all in the trace's first sample
     np.defer_specular = nrc->opts.defer_specular;
+    np.train_only = train_only ? 1u : 0u;
     const hipStream_t hs = (hipStream_t)stream;
     // The first shading launch zeroes the record count (after counting the
     // previous trace's last sample for the selection probability).
@@ -521,6 +525,19 @@ lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera* c
     if (nrc->opts.train_paths == 0) return LMB_OK;
     return nrc_run_steps(nrc, nrc->dev.rec_in, nrc->dev.rec_target, nrc->dev.ctl + kNrcCtlRecords,
                          nrc->dev.rec_cap, steps, true, nullptr, hs, bufs->timer.get());
+}
+
+lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
+                         lmb_data_buffers bufs, uint32_t seed_offset, lmb_nrc nrc, void* stream) {
+    return trace_nrc(tracer, scene, cam, bufs, seed_offset, nrc, stream, false);
+}
+
+lmb_status lmb_trace_nrc_train(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
+                               lmb_data_buffers bufs, uint32_t seed_offset, lmb_nrc nrc,
+                               void* stream) {
+    if (nrc && nrc->opts.train_paths == 0)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "a training frame needs train_paths > 0");
+    return trace_nrc(tracer, scene, cam, bufs, seed_offset, nrc, stream, true);
 }
 
 static lmb_status need_spatial_buffers(lmb_data_buffers b) {

@@ -216,9 +216,10 @@ struct Traversal {
         // The |O| part is folded into the per-axis offsets (near planes use
         // O - 6u|O|, far planes O + 6u|O|, covering the rounding of that fma
         // too), so it never couples axes: a near-parallel axis with a huge
-        // |O| widens only its own planes.  The 2u |t| part is the per-child
-        // `slack` = 3u (|tn| + |tf|).  Every box the exact slab test hits is
-        // therefore hit here.
+        // |O| widens only its own planes.  The 2u |t| part is covered per
+        // child by tn - 4u|tn| and tf + 4u|tf| (one fma each, whose own
+        // rounding leaves at least 3u|t| of margin).  Every box the exact slab
+        // test hits is therefore hit here.
         const float Sx = fminf(fmaxf(sx * ix, -1e38f), 1e38f);
         const float Sy = fminf(fmaxf(sy * iy, -1e38f), 1e38f);
         const float Sz = fminf(fmaxf(sz * iz, -1e38f), 1e38f);
@@ -242,9 +243,10 @@ struct Traversal {
             const float tfz = fmaf((float)byte_of(fz0, fz1, s), Sz, Ofz);
             const float tn = fmaxf(fmaxf(tnx, tny), tnz);
             const float tf = fminf(fminf(tfx, tfy), tfz);
-            const float slack = (fabsf(tn) + fabsf(tf)) * (3.0f * 5.9604645e-8f);
-            const bool hit = meta != kMetaEmpty && tn <= fminf(tf, best) + slack &&
-                             r.tmin <= tf + slack;
+            constexpr float kChildSlack = 4.0f * 5.9604645e-8f;
+            const float tn_lo = fmaf(-fabsf(tn), kChildSlack, tn);
+            const float tf_hi = fmaf(fabsf(tf), kChildSlack, tf);
+            const bool hit = meta != kMetaEmpty && tn_lo <= fminf(tf_hi, best) && r.tmin <= tf_hi;
             if (hit) {
                 if (meta == kMetaInternal) {
                     node_hits |= 1u << (s ^ oct);

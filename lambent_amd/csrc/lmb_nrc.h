@@ -120,7 +120,8 @@ constexpr uint32_t kNrcCtlRecordsBase = 2 * kNrcCtlStride;  // min(records, rec_
 constexpr uint32_t kNrcCtlHitsPrev = 3 * kNrcCtlStride;     // cache vertices of the last sample
                                                             // traced (kept across traces)
 constexpr uint32_t kNrcCtlHostCount = 4 * kNrcCtlStride;    // record count of the host-array calls
-constexpr uint32_t kNrcCtlWords = 5 * kNrcCtlStride;
+constexpr uint32_t kNrcCtlFrac = 5 * kNrcCtlStride;         // training frame: its pixel fraction (f32)
+constexpr uint32_t kNrcCtlWords = 6 * kNrcCtlStride;
 
 struct NrcDev {
     float* master;  // kNrcNumWeights fp32
@@ -175,6 +176,7 @@ struct NrcTraceParams {
     float train_expect;     // expected training records per sample
     uint32_t sample;
     uint32_t defer_specular;  // most vertices a specular chain moves the cache vertex
+    uint32_t train_only;      // a training frame: random pixels, no image
 };
 hipError_t launch_trace_nrc(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
                             const TraceParams& p, const NrcDev& d, NrcTraceParams np,

@@ -32,6 +32,7 @@
 #include "dev_shade.h"
 #include "lmb_frame_dev.h"
 #include "lmb_nrc.h"
+#include "lmb_restir.h"
 
 namespace lmb {
 
@@ -76,12 +77,8 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
     __shared__ ReserveLds<kNrcTraceBlock, 1> rl_rec;
     __shared__ ReserveLds<kNrcTraceBlock, 2> rl_out;
     if (bounce == 0 && gtid == 0) {
-        // The last sample's cache vertices: its queries plus the records it
-        // stored (a record that found no room became a query).  No lane
-        // reserves queries or records at bounce 0.
+        // No lane reserves queries or records at bounce 0.
         const uint32_t rec = min(d.ctl[kNrcCtlRecords], d.rec_cap);
-        const uint32_t hits = d.ctl[kNrcCtlQueries] + (rec - min(d.ctl[kNrcCtlRecordsBase], rec));
-        if (hits) d.ctl[kNrcCtlHitsPrev] = hits;
         d.ctl[kNrcCtlQueries] = 0;
         if (np.sample == 0) d.ctl[kNrcCtlRecords] = 0;  // a new trace
         d.ctl[kNrcCtlRecordsBase] = np.sample == 0 ? 0u : rec;
@@ -92,7 +89,7 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
         // before any sample has been counted, the paths reaching bounce 1.
         const uint32_t prev = d.ctl[kNrcCtlHitsPrev];
         const uint32_t ref = prev ? prev : max(f.counters[kCacheBounce], 1u);
-        prob = fminf(np.train_expect / (float)ref, 1.0f);
+        prob = np.train_only ? 1.0f : fminf(np.train_expect / (float)ref, 1.0f);
     }
     for (uint64_t slot = gtid; slot < count_r; slot += stride) {
         // ---- phase A: hit record and the cache-vertex inputs
@@ -129,7 +126,7 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
             const float t = __uint_as_float(h.y);
             it = make_isect(s, h.x, t, __uint_as_float(h.z), __uint_as_float(h.w), dir);
             new_pos = fma3(dir, v3s(t), org);
-            if (bounce == 0) {
+            if (bounce == 0 && !np.train_only) {
                 f.out_normal[pid] = make_float4(it.normal.x, it.normal.y, it.normal.z, 1.f);
                 f.out_albedo[pid] = make_float4(it.color.x, it.color.y, it.color.z, 1.f);
             }
@@ -173,8 +170,10 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
                     scol = v3s(0.0f);
                 }
                 if (is_query) {
-                    const float4 ps = f.pix_sum[pid];
-                    f.pix_sum[pid] = make_float4(ps.x + scol.x, ps.y + scol.y, ps.z + scol.z, 0.f);
+                    if (!np.train_only) {
+                        const float4 ps = f.pix_sum[pid];
+                        f.pix_sum[pid] = make_float4(ps.x + scol.x, ps.y + scol.y, ps.z + scol.z, 0.f);
+                    }
                 } else {
                     const V3 ndir = surface_scatter<Q>(it, dir, seed, color, picked);
                     bool alive = bounce + 1 < p.bounces;
@@ -202,7 +201,7 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
                     }
                 }
             } else {
-                if (bounce == 0) {
+                if (bounce == 0 && !np.train_only) {
                     f.out_normal[pid] = make_float4(0.f, 0.f, 0.f, 1.f);
                     f.out_albedo[pid] = make_float4(0.f, 0.f, 0.f, 1.f);
                 }
@@ -218,8 +217,10 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
                     pix = v3(fmaf(a0.x, scol.x, a0.w), fmaf(a0.y, scol.y, a1.x),
                              fmaf(a0.z, scol.z, a1.y));
                 }
-                const float4 ps = f.pix_sum[pid];
-                f.pix_sum[pid] = make_float4(ps.x + pix.x, ps.y + pix.y, ps.z + pix.z, 0.f);
+                if (!np.train_only) {
+                    const float4 ps = f.pix_sum[pid];
+                    f.pix_sum[pid] = make_float4(ps.x + pix.x, ps.y + pix.y, ps.z + pix.z, 0.f);
+                }
             }
         }
         // ---- phase D (whole block): queries and the continuation queue
@@ -236,6 +237,80 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
             qout[2 * (uint64_t)ci + 1] = out_d;
         }
     }
+}
+
+// Camera rays of a training frame: each pixel with probability `frac`
+// (about train_expect cache vertices: the fraction a full sample would
+// select there), compacted into the bounce-0 queue.  Seeds as in k_generate
+// for a first sample.
+__global__ __launch_bounds__(kNrcTraceBlock) void k_nrc_generate_train(CameraDev cam, FrameDev f,
+                                                                       TraceParams p, NrcDev d,
+                                                                       NrcTraceParams np) {
+    __shared__ ReserveLds<kNrcTraceBlock, 1> rl;
+    const uint64_t stride = (uint64_t)gridDim.x * kNrcTraceBlock;
+    // before any count: as if every pixel reached a cache vertex
+    const uint32_t prev = d.ctl[kNrcCtlHitsPrev];
+    const float frac = fminf(np.train_expect / (float)(prev ? prev : max(f.npix, (uint64_t)1)), 1.0f);
+    if (blockIdx.x == 0 && threadIdx.x == 0) d.ctl[kNrcCtlFrac] = __float_as_uint(frac);
+    const uint64_t n_r = block_uniform_count<kNrcTraceBlock>((uint32_t)f.npix);
+    for (uint64_t pix = (uint64_t)blockIdx.x * kNrcTraceBlock + threadIdx.x; pix < n_r;
+         pix += stride) {
+        const bool sel = pix < f.npix && train_draw((uint32_t)pix, p.seed_offset, 0xFFFFu) < frac;
+        uint32_t idx[1];
+        block_reserve<kNrcTraceBlock, 1>({sel}, {f.counters}, rl, idx);
+        if (!sel) continue;
+        const uint32_t x = (uint32_t)(pix % f.width), y = (uint32_t)(pix / f.width);
+        const uint32_t seed = rand_u32(x * y) + p.seed_offset + y;
+        float o[3], dd[3];
+        create_ray(cam, x, y, f.width, f.height, o, dd);
+        f.queue[0][2 * (uint64_t)idx[0]] = make_float4(o[0], o[1], o[2], __uint_as_float((uint32_t)pix));
+        f.queue[0][2 * (uint64_t)idx[0] + 1] = make_float4(dd[0], dd[1], dd[2], 0.f);
+        f.ps_seed[pix] = seed;
+        f.ps_through[pix] = make_float4(1.f, 1.f, 1.f, -1.f);
+        f.ps_accum[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// End of a sample: its cache vertices are its queries plus the records it
+// stored (a record that found no room became a query); a training frame's
+// count is scaled to the whole image by its pixel fraction.  The count sets
+// the selection probability of the next samples.
+__global__ void k_nrc_end_sample(NrcDev d, uint32_t train_only) {
+    const uint32_t rec = min(d.ctl[kNrcCtlRecords], d.rec_cap);
+    const uint32_t hits = d.ctl[kNrcCtlQueries] + (rec - min(d.ctl[kNrcCtlRecordsBase], rec));
+    float full = (float)hits;
+    if (train_only) {
+        const float frac = __uint_as_float(d.ctl[kNrcCtlFrac]);
+        full = frac > 0.0f ? fminf((float)hits / frac, 4.0e9f) : 0.0f;
+    }
+    if (full >= 1.0f) d.ctl[kNrcCtlHitsPrev] = (uint32_t)full;
+}
+
+template <uint32_t Q>
+static hipError_t launch_nrc_train(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
+                                   const TraceParams& p, const NrcDev& d, NrcTraceParams np,
+                                   hipStream_t stream, KernelTimer* tm) {
+    const dim3 grid((uint32_t)(frame_threads(f.npix) / kNrcTraceBlock)), block(kNrcTraceBlock);
+    // queue lengths, this trace's vertex count and both traversal fetch sets
+    hipError_t e = launch_fill_zero(f.counters, (kMaxBounces + 1) * 4, stream);
+    if (e == hipSuccess) e = launch_fill_zero(f.counters + kCounterVertices, 8, stream);
+    if (e == hipSuccess) e = launch_fill_zero(f.counters + kCounterFetch, 2 * kFetchWords * 4, stream);
+    if (e != hipSuccess) return e;
+    np.sample = 0;
+    hipLaunchKernelGGL(k_nrc_generate_train, grid, block, 0, stream, cam, f, p, d, np);
+    for (uint32_t b = 0; b < p.bounces; ++b) {
+        if (tm) tm->begin(kKernelExtend, stream);
+        e = launch_extend(s, f, p, b, stream);
+        if (tm) tm->end(kKernelExtend, stream);
+        if (e != hipSuccess) return e;
+        if (tm) tm->begin(kKernelShade, stream);
+        hipLaunchKernelGGL(k_shade_nrc<Q>, grid, block, 0, stream, s, f, p, d, np, b);
+        if (tm) tm->end(kKernelShade, stream);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_nrc_end_sample, dim3(1), dim3(1), 0, stream, d, 1u);
+    return hipGetLastError();
 }
 
 template <uint32_t Q>
@@ -272,6 +347,7 @@ static hipError_t launch_nrc_q(const SceneDev& s, const CameraDev& cam, const Fr
             e = hipGetLastError();
             if (e != hipSuccess) return e;
         }
+        hipLaunchKernelGGL(k_nrc_end_sample, dim3(1), dim3(1), 0, stream, d, 0u);
         if (p.bounces > kCacheBounce) {
             if (tm) tm->begin(kKernelCacheInfer, stream);
             e = nrc_launch_infer_accumulate(d, f.pix_sum, stream);
@@ -286,6 +362,18 @@ hipError_t launch_trace_nrc(const SceneDev& s, const CameraDev& cam, const Frame
                             const TraceParams& p, const NrcDev& d, NrcTraceParams np,
                             hipStream_t stream, KernelTimer* timer) {
     if (f.npix == 0) return hipSuccess;
+    if (np.train_only) {
+        switch (p.quality) {
+            case kQualityLow:
+                return launch_nrc_train<kQualityLow>(s, cam, f, p, d, np, stream, timer);
+            case kQualityHigh:
+                return launch_nrc_train<kQualityHigh>(s, cam, f, p, d, np, stream, timer);
+            case kQualitySpectral:
+                return launch_nrc_train<kQualitySpectral>(s, cam, f, p, d, np, stream, timer);
+            default:
+                return launch_nrc_train<kQualityMedium>(s, cam, f, p, d, np, stream, timer);
+        }
+    }
     switch (p.quality) {
         case kQualityLow:
             return launch_nrc_q<kQualityLow>(s, cam, f, p, d, np, stream, timer);

@@ -237,3 +237,49 @@ def test_cached_tracer_several_samples_per_trace():
     assert st["queries"] > 48 * 48 * 0.5, st  # the last sample: no training paths
     img = bufs.output(api.Output.COLOR)[..., :3]
     assert np.isfinite(img).all() and img.mean() > 0
+
+
+def test_training_frames_train_without_an_image():
+    """lmb_trace_nrc_train: about train_paths records from random pixels, the
+    optimiser steps taken, the output buffers untouched; a cache warmed up by
+    training frames alone renders the Cornell box much closer to a 4096-spp
+    reference than the truncated paths alone (measured: relMSE 0.093 vs 0.242;
+    300 full cached frames give 0.090)."""
+    spec = scenes.cornell_box(64, 64)
+    scene = spec.build()
+    cam = spec.camera()
+    opts = api.PathTracerOpts(samples=1, bounces=8, rr_start=3, guiding=api.Guiding.OFF)
+    pipe = api.RayTracer(api.Medium).create_pipeline(opts)
+    ref_pipe = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=4096, bounces=8, rr_start=3, guiding=api.Guiding.OFF))
+    rb = api.DataBuffers(64, 64, 1)
+    ref_pipe.dispatch(scene, cam, rb, 999)
+    ref = rb.output(api.Output.COLOR)[..., :3].astype(np.float64)
+    bufs = api.DataBuffers(64, 64, 1)
+    pipe.dispatch(scene, cam, bufs, 1)
+    before = bufs.output(api.Output.COLOR)
+    cache = api.NeuralCache(api.NeuralCacheOpts(train_paths=1024, train_steps=4, seed=5))
+    for f in range(300):
+        pipe.train_cached(scene, cam, bufs, cache, 10 + f)
+    st = cache.stats()
+    assert st["steps"] == 1200
+    assert 0.6 * 1024 < st["records"] < 1.4 * 1024, st
+    np.testing.assert_array_equal(bufs.output(api.Output.COLOR), before)
+    frozen = api.NeuralCacheOpts(train_paths=0, seed=5)
+
+    def render(c):
+        c.set_opts(frozen)
+        acc = np.zeros((64, 64, 3))
+        for f in range(200):
+            pipe.dispatch_cached(scene, cam, bufs, c, 5000 + f)
+            acc += bufs.output(api.Output.COLOR)[..., :3]
+        return acc / 200
+
+    untrained = api.NeuralCache(api.NeuralCacheOpts(seed=5))
+    untrained.set_weights(np.zeros(NW, np.float32))  # the truncated paths alone
+    img = render(cache)
+    e_trained, e_untrained = _rel_mse(img, ref), _rel_mse(render(untrained), ref)
+    assert e_trained < 0.5 * e_untrained, (e_trained, e_untrained)
+    assert abs(img.mean() - ref.mean()) / ref.mean() < 0.06, (img.mean(), ref.mean())
+    with pytest.raises(api.LambentError):
+        pipe.train_cached(scene, cam, bufs, api.NeuralCache(frozen), 0)
