@@ -279,6 +279,29 @@ lmb_status lmb_data_buffers_output(lmb_data_buffers bufs, uint32_t which,
 /* Copies an output image (width*height*4 floats) to host memory (blocks). */
 lmb_status lmb_data_buffers_read_output(lmb_data_buffers bufs, uint32_t which,
                                         float *host);
+
+/* The example application's frame ring and average (no counterpart in the
+ * library API: examples/cornell_box/src/main.rs:377-413, 699-748 and
+ * average_to_buf.wgsl:13-22).  A ring of `frames` device images; push
+ * copies the buffers' colour output into the next slot (cycling), average
+ * writes, per pixel, the sum over the filled slots in slot order of
+ * colour / n (n = filled slots) as packed RGB f32 (3 floats per pixel, the
+ * layout of the example's denoiser staging buffers). */
+typedef struct lmb_frame_ring_s *lmb_frame_ring;
+lmb_status lmb_frame_ring_create(uint32_t width, uint32_t height, uint32_t frames,
+                                 lmb_frame_ring *out);
+lmb_status lmb_frame_ring_destroy(lmb_frame_ring ring);
+lmb_status lmb_frame_ring_push(lmb_frame_ring ring, lmb_data_buffers bufs, void *stream);
+lmb_status lmb_frame_ring_count(lmb_frame_ring ring, uint32_t *count);
+lmb_status lmb_frame_ring_average(lmb_frame_ring ring, void *stream);
+/* The average's device buffer (width * height * 3 floats) and a host copy. */
+lmb_status lmb_frame_ring_output(lmb_frame_ring ring, void **rgb_dev, uint64_t *bytes);
+lmb_status lmb_frame_ring_read_average(lmb_frame_ring ring, float *host);
+/* One output (LMB_OUTPUT_*) as packed RGB f32 into a device buffer of
+ * width * height * 3 floats: the denoiser's colour / normal / albedo inputs
+ * of main.rs:797-963. */
+lmb_status lmb_data_buffers_pack_rgb(lmb_data_buffers bufs, uint32_t which, void *rgb_dev,
+                                     void *stream);
 /* Byte-level access to the persistent state for tests and checkpoints.
  * which: 0 gi_reservoirs, 1 old_gi_reservoirs, 2 info, 3 markov_chains,
  *        4 old_markov_chains, 5 world_markov_chains, 6 old_world_markov_chains */

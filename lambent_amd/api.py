@@ -193,6 +193,16 @@ _OPTIONAL = {
     "lmb_data_buffers_output": ([_vp, ctypes.c_uint32, ctypes.POINTER(_vp),
                                  ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int32),
     "lmb_data_buffers_read_output": ([_vp, ctypes.c_uint32, _f32p], ctypes.c_int32),
+    "lmb_data_buffers_pack_rgb": ([_vp, ctypes.c_uint32, _vp, _vp], ctypes.c_int32),
+    "lmb_frame_ring_create": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                               ctypes.POINTER(_vp)], ctypes.c_int32),
+    "lmb_frame_ring_destroy": ([_vp], ctypes.c_int32),
+    "lmb_frame_ring_push": ([_vp, _vp, _vp], ctypes.c_int32),
+    "lmb_frame_ring_count": ([_vp, _u32p], ctypes.c_int32),
+    "lmb_frame_ring_average": ([_vp, _vp], ctypes.c_int32),
+    "lmb_frame_ring_output": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_uint64)],
+                              ctypes.c_int32),
+    "lmb_frame_ring_read_average": ([_vp, _f32p], ctypes.c_int32),
     "lmb_data_buffers_state_size": ([_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)],
                                     ctypes.c_int32),
     "lmb_data_buffers_read_state": ([_vp, ctypes.c_uint32, _vp, ctypes.c_uint64],
@@ -799,6 +809,48 @@ class Output(enum.IntEnum):
     COLOR = 0
     NORMAL = 1
     ALBEDO = 2
+
+
+class FrameRing:
+    """The example application's frame ring and average (main.rs:377-413,
+    699-748; average_to_buf.wgsl): push() keeps a trace's colour output in the
+    next of `frames` slots, average() returns the mean of the filled slots as
+    packed RGB (h, w, 3) float32.  No counterpart in the library API."""
+
+    def __init__(self, width: int, height: int, frames: int):
+        h = _vp()
+        _check(_lib.lmb_frame_ring_create(width, height, frames, ctypes.byref(h)))
+        self._h, self.width, self.height = h, width, height
+
+    def push(self, buffers: "DataBuffers", stream=None) -> None:
+        _check(_lib.lmb_frame_ring_push(self._h, buffers.handle, stream))
+
+    def __len__(self) -> int:
+        n = ctypes.c_uint32()
+        _check(_lib.lmb_frame_ring_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def average(self, stream=None) -> np.ndarray:
+        _check(_lib.lmb_frame_ring_average(self._h, stream))
+        out = np.empty((self.height, self.width, 3), np.float32)
+        _check(_lib.lmb_frame_ring_read_average(self._h, _fptr(out)))
+        return out
+
+    def output_device_ptr(self) -> tuple[int, int]:
+        p, n = _vp(), ctypes.c_uint64()
+        _check(_lib.lmb_frame_ring_output(self._h, ctypes.byref(p), ctypes.byref(n)))
+        return p.value, n.value
+
+    def close(self):
+        if self._h:
+            _lib.lmb_frame_ring_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DataBuffers:

@@ -108,6 +108,11 @@ hipError_t launch_extend(const SceneDev& s, const FrameDev& f, const TraceParams
 hipError_t launch_resolve(const FrameDev& f, const TraceParams& p, hipStream_t stream);
 
 // One full trace (all samples, all bounces) on `stream`.
+// The example's frame averaging and RGB packing (frame_ring.hip).
+hipError_t launch_ring_average(const float4* frames, uint32_t num, uint64_t npix, float* rgb,
+                               hipStream_t stream);
+hipError_t launch_pack_rgb(const float4* src, uint64_t npix, float* rgb, hipStream_t stream);
+
 // The debugging shaders (debug.hip; p.quality = LMB_QUALITY_DEBUG_*).
 hipError_t launch_debug(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
                         const TraceParams& p, hipStream_t stream, KernelTimer* timer);

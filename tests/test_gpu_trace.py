@@ -281,3 +281,31 @@ def test_batch_rejects_non_opaque_scene():
     with pytest.raises(api.LambentError):
         api.Batch([(spec.build(), spec.camera(), 16, 16)],
                   api.PathTracerOpts(samples=1, guiding=api.Guiding.OFF), api.Medium)
+
+
+def test_frame_ring_average_bit_exact():
+    """The example's frame ring (average_to_buf.wgsl:13-22): after 5 pushes
+    into 3 slots, slots hold frames 3, 4, 2 and the average is
+    ((0 + s0 / 3) + s1 / 3) + s2 / 3 per channel, in float32."""
+    spec = scenes.cornell_box(40, 30)
+    scene = spec.build()
+    pipe = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=2, bounces=5, guiding=api.Guiding.OFF))
+    bufs = api.DataBuffers(40, 30, 1)
+    ring = api.FrameRing(40, 30, 3)
+    frames = []
+    for f in range(5):
+        pipe.dispatch(scene, spec.camera(), bufs, f)
+        frames.append(bufs.output(api.Output.COLOR))
+        ring.push(bufs)
+        assert len(ring) == min(f + 1, 3)
+        if f == 0:
+            np.testing.assert_array_equal(ring.average(), frames[0][..., :3])
+    slots = [frames[3], frames[4], frames[2]]
+    n = np.float32(3)
+    expect = np.zeros((30, 40, 3), np.float32)
+    for s_ in slots:
+        expect = expect + s_[..., :3] / n
+    np.testing.assert_array_equal(ring.average(), expect)
+    with pytest.raises(api.LambentError):
+        ring.push(api.DataBuffers(20, 20, 1))
