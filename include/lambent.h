@@ -166,6 +166,42 @@ typedef struct lmb_scene_desc {
 #define LMB_SCENE_ALPHA_TEST 1u
 
 lmb_status lmb_scene_create(const lmb_scene_desc *desc, lmb_scene *out);
+
+/* TLAS instances (wgpu TlasInstance::new(blas, transform, custom_index,
+ * mask), examples/cornell_box/src/main.rs:560-563): instance k draws
+ * geometry `geometry` through `transform` (3x4 row-major object to world)
+ * with materials materials[material_idx[prim]] (general.wesl:1059:
+ * material_idx[instance_index].materials[primitive_index]).  The instances
+ * are baked into one world-space BVH at creation (the ray query's closest
+ * hit over instances, with single-level traversal); triangles are numbered
+ * instance by instance in hits; an instance with (mask & 0xFF) == 0 is not
+ * seen (the cull mask of the reference's ray queries is 0xFF).  Shading
+ * uses the world-space vertices (the reference's get_vertices reads object
+ * space, bindings.wesl:102-110, which is the same for identity
+ * transforms). */
+typedef struct lmb_geometry_desc {
+    const float *vertices;      /* vec4 per vertex (xyz used) */
+    uint64_t num_vertices;
+    const uint32_t *indices;    /* 3 per triangle; NULL = non-indexed */
+    uint64_t num_indices;
+} lmb_geometry_desc;
+typedef struct lmb_instance {
+    uint32_t geometry;
+    float transform[12];
+    const uint32_t *material_idx; /* 1 per triangle of the geometry */
+    uint32_t mask;
+} lmb_instance;
+typedef struct lmb_instanced_desc {
+    const lmb_geometry_desc *geometries;
+    uint32_t num_geometries;
+    const lmb_instance *instances;
+    uint32_t num_instances;
+    const lmb_material *materials;
+    uint64_t num_materials;
+    lmb_textures textures;
+    uint32_t flags;             /* LMB_SCENE_* */
+} lmb_instanced_desc;
+lmb_status lmb_scene_create_instanced(const lmb_instanced_desc *desc, lmb_scene *out);
 lmb_status lmb_scene_destroy(lmb_scene scene);
 /* BVH statistics: node count, depth, leaf triangle count, SAH cost, build ms */
 typedef struct lmb_bvh_stats {

@@ -110,6 +110,23 @@ class _SceneDescC(ctypes.Structure):
                 ("num_materials", ctypes.c_uint64), ("textures", _vp), ("flags", ctypes.c_uint32)]
 
 
+class _GeometryDescC(ctypes.Structure):
+    _fields_ = [("vertices", _f32p), ("num_vertices", ctypes.c_uint64),
+                ("indices", _u32p), ("num_indices", ctypes.c_uint64)]
+
+
+class _InstanceC(ctypes.Structure):
+    _fields_ = [("geometry", ctypes.c_uint32), ("transform", ctypes.c_float * 12),
+                ("material_idx", _u32p), ("mask", ctypes.c_uint32)]
+
+
+class _InstancedDescC(ctypes.Structure):
+    _fields_ = [("geometries", ctypes.POINTER(_GeometryDescC)), ("num_geometries", ctypes.c_uint32),
+                ("instances", ctypes.POINTER(_InstanceC)), ("num_instances", ctypes.c_uint32),
+                ("materials", ctypes.POINTER(_MaterialC)), ("num_materials", ctypes.c_uint64),
+                ("textures", _vp), ("flags", ctypes.c_uint32)]
+
+
 class _BvhStatsC(ctypes.Structure):
     _fields_ = [("num_triangles", ctypes.c_uint64), ("num_nodes", ctypes.c_uint64),
                 ("max_depth", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
@@ -173,6 +190,7 @@ _sig("lmb_textures_set_background", [_vp, ctypes.c_uint32, _u8p])
 _sig("lmb_textures_destroy", [_vp])
 _sig("lmb_scene_create", [ctypes.POINTER(_SceneDescC), ctypes.POINTER(_vp)])
 _sig("lmb_scene_destroy", [_vp])
+_sig("lmb_scene_create_instanced", [ctypes.POINTER(_InstancedDescC), ctypes.POINTER(_vp)])
 _sig("lmb_scene_bvh_stats", [_vp, ctypes.POINTER(_BvhStatsC)])
 _sig("lmb_scene_intersect", [_vp, _f32p, ctypes.c_uint64, _vp])
 _sig("lmb_scene_intersect_device", [_vp, _vp, ctypes.c_uint64, _vp, _vp])
@@ -264,7 +282,8 @@ HEADER_FUNCTIONS = [
     "lmb_textures_create", "lmb_textures_add", "lmb_textures_set_recolour",
     "lmb_textures_set_background", "lmb_textures_destroy", "lmb_scene_create",
     "lmb_scene_destroy", "lmb_scene_bvh_stats", "lmb_scene_intersect",
-    "lmb_scene_intersect_device", "lmb_sync", "lmb_primary_hits", *list(_OPTIONAL),
+    "lmb_scene_intersect_device", "lmb_sync", "lmb_primary_hits", "lmb_scene_create_instanced",
+    *list(_OPTIONAL),
 ]
 
 
@@ -608,6 +627,49 @@ class Scene:
         _check(_lib.lmb_scene_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
         self.num_triangles = (len(idx) // 3) if idx is not None else len(v) // 3
+
+    @classmethod
+    def from_instances(cls, geometries, instances, materials: Sequence[Material] | np.ndarray,
+                       textures: Optional[TextureSet] = None, opaque: bool = True) -> "Scene":
+        """TLAS instances (main.rs:560-563): geometries = [(vertices, indices
+        or None)], instances = [(geometry, 3x4 row-major transform,
+        material_idx per triangle, mask)], baked into one BVH."""
+        keep = []
+        geo = (_GeometryDescC * max(len(geometries), 1))()
+        for k, (v, idx) in enumerate(geometries):
+            v = np.ascontiguousarray(v, np.float32).reshape(-1, 4)
+            idx = None if idx is None else np.ascontiguousarray(idx, np.uint32).reshape(-1)
+            keep += [v, idx]
+            geo[k].vertices, geo[k].num_vertices = _fptr(v), len(v)
+            geo[k].indices = None if idx is None else _uptr(idx)
+            geo[k].num_indices = 0 if idx is None else len(idx)
+        ins = (_InstanceC * max(len(instances), 1))()
+        for k, (g, tf, mi, mask) in enumerate(instances):
+            mi = np.ascontiguousarray(mi, np.uint32).reshape(-1)
+            keep.append(mi)
+            ins[k].geometry = int(g)
+            ins[k].transform[:] = [float(x) for x in np.asarray(tf, np.float32).reshape(12)]
+            ins[k].material_idx = _uptr(mi)
+            ins[k].mask = int(mask)
+        mats = (np.ascontiguousarray(materials, np.uint32).reshape(-1, 8)
+                if isinstance(materials, np.ndarray) else materials_array(materials))
+        d = _InstancedDescC()
+        d.geometries, d.num_geometries = geo, len(geometries)
+        d.instances, d.num_instances = ins, len(instances)
+        d.materials = mats.ctypes.data_as(ctypes.POINTER(_MaterialC))
+        d.num_materials = len(mats)
+        d.textures = None if textures is None else textures.handle
+        d.flags = 0 if opaque else SCENE_ALPHA_TEST
+        h = _vp()
+        _check(_lib.lmb_scene_create_instanced(ctypes.byref(d), ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self._keep = (keep, mats, textures)
+        self._h = h
+        self.num_triangles = sum(
+            (len(np.asarray(geometries[g][1]).reshape(-1)) // 3 if geometries[g][1] is not None
+             else len(np.asarray(geometries[g][0]).reshape(-1, 4)) // 3)
+            for g, _, _, mask in instances if int(mask) & 0xFF)
+        return self
 
     @property
     def handle(self):

@@ -334,6 +334,64 @@ lmb_status lmb_scene_create(const lmb_scene_desc* desc, lmb_scene* out) {
     return LMB_OK;
 }
 
+// TLAS instances (main.rs:560-563, general.wesl:1052-1059): each instance's
+// geometry is transformed to world space here and all instances share one
+// BVH.  The closest hit over instances is the same as the ray query's
+// (rounding aside: the reference transforms the ray into object space
+// instead), and traversal stays single-level.  Triangles are numbered
+// instance by instance; instances whose mask has no bit in 0xFF (the ray
+// query's cull mask) are left out.
+lmb_status lmb_scene_create_instanced(const lmb_instanced_desc* desc, lmb_scene* out) {
+    if (!desc || !out) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    if (desc->num_geometries > 0 && !desc->geometries)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "geometries is NULL");
+    if (desc->num_instances > 0 && !desc->instances)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "instances is NULL");
+    std::vector<float> verts;
+    std::vector<uint32_t> mat_idx;
+    for (uint32_t k = 0; k < desc->num_instances; ++k) {
+        const lmb_instance& in = desc->instances[k];
+        if (in.geometry >= desc->num_geometries)
+            return fail(LMB_ERR_INVALID_ARGUMENT,
+                        "instance " + std::to_string(k) + " names a missing geometry");
+        if ((in.mask & 0xFFu) == 0) continue;
+        const lmb_geometry_desc& g = desc->geometries[in.geometry];
+        if (g.num_vertices > 0 && !g.vertices)
+            return fail(LMB_ERR_INVALID_ARGUMENT, "geometry vertices are NULL");
+        if (g.indices && g.num_indices % 3 != 0)
+            return fail(LMB_ERR_INVALID_ARGUMENT, "index count is not a multiple of 3");
+        const uint64_t ntri = g.indices ? g.num_indices / 3 : g.num_vertices / 3;
+        if (ntri > 0 && !in.material_idx)
+            return fail(LMB_ERR_INVALID_ARGUMENT,
+                        "instance " + std::to_string(k) + " has no material_idx");
+        const float* m = in.transform;
+        for (uint64_t t = 0; t < 3 * ntri; ++t) {
+            const uint64_t vi = g.indices ? g.indices[t] : t;
+            if (vi >= g.num_vertices)
+                return fail(LMB_ERR_INVALID_ARGUMENT, "index out of range");
+            const float x = g.vertices[4 * vi], y = g.vertices[4 * vi + 1], z = g.vertices[4 * vi + 2];
+            // 3x4 row-major object-to-world, summed left to right
+            verts.push_back(((m[0] * x + m[1] * y) + m[2] * z) + m[3]);
+            verts.push_back(((m[4] * x + m[5] * y) + m[6] * z) + m[7]);
+            verts.push_back(((m[8] * x + m[9] * y) + m[10] * z) + m[11]);
+            verts.push_back(0.0f);
+        }
+        mat_idx.insert(mat_idx.end(), in.material_idx, in.material_idx + ntri);
+    }
+    lmb_scene_desc flat{};
+    flat.vertices = verts.data();
+    flat.num_vertices = verts.size() / 4;
+    flat.indices = nullptr;
+    flat.num_indices = 0;
+    flat.material_idx = mat_idx.data();
+    flat.materials = desc->materials;
+    flat.num_materials = desc->num_materials;
+    flat.textures = desc->textures;
+    flat.flags = desc->flags;
+    return lmb_scene_create(&flat, out);
+}
+
 lmb_status lmb_scene_destroy(lmb_scene scene) {
     if (!scene) return LMB_OK;
     scene_free(scene);

@@ -178,3 +178,13 @@ def test_gpu_objects_fail_loudly_without_device(gpu_available):
     with pytest.raises(api.LambentError) as e:
         api.DataBuffers(8, 8, 1)
     assert e.value.code == api.ERR_DEVICE
+
+
+def test_instanced_scene_rejects_a_missing_geometry():
+    """Instance validation happens before any device work."""
+    v = np.zeros((3, 4), np.float32)
+    with pytest.raises(api.LambentError, match="missing geometry"):
+        api.Scene.from_instances([(v, None)], [(1, np.eye(3, 4), [0], 0xFF)],
+                                 [api.Material.new([0, 0], [1, 0], [1, 1], [0, 0], 0, None, None,
+                                                   api.EmissionScale(0.0, 0.0), None,
+                                                   api.MaterialType.Diffuse)])

@@ -309,3 +309,61 @@ def test_frame_ring_average_bit_exact():
     np.testing.assert_array_equal(ring.average(), expect)
     with pytest.raises(api.LambentError):
         ring.push(api.DataBuffers(20, 20, 1))
+
+
+def _rot_y(deg, t):
+    a = np.radians(deg)
+    c, s = np.cos(a), np.sin(a)
+    return np.array([[c, 0, s, t[0]], [0, 1, 0, t[1]], [-s, 0, c, t[2]]], np.float32)
+
+
+def test_instanced_scene_equals_baked_geometry():
+    """TLAS instances (main.rs:560-563, general.wesl:1059): a box instanced
+    twice with its own transforms and material tables, a quad once, and a
+    fourth instance whose mask hides it, render bit-exactly like one scene
+    holding the same triangles transformed ((m0 x + m1 y) + m2 z) + m3 in
+    float32."""
+    base = scenes.cornell_box(64, 64)
+    box_v = np.array([[x, y, z, 0] for z in (-0.1, 0.1) for y in (0, 0.3) for x in (-0.1, 0.1)],
+                     np.float32)
+    quads = [(0, 2, 6, 4), (1, 5, 7, 3), (0, 4, 5, 1), (2, 3, 7, 6), (0, 1, 3, 2), (4, 6, 7, 5)]
+    box_i = np.array([k for q in quads for k in (q[0], q[1], q[2], q[0], q[2], q[3])], np.uint32)
+    quad_v = np.array([[0.1, 0.6, 0.6, 0], [0.3, 0.6, 0.6, 0], [0.3, 0.8, 0.6, 0],
+                       [0.1, 0.6, 0.6, 0], [0.3, 0.8, 0.6, 0], [0.1, 0.8, 0.6, 0]], np.float32)
+    nmat = len(base.materials)
+    t1, t2 = _rot_y(15, (0.3, 0.0, 0.35)), _rot_y(-18, (0.7, 0.0, 0.6))
+    ident = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0]], np.float32)
+    m_box1 = np.zeros(12, np.uint32)
+    m_box2 = (np.arange(12) % nmat).astype(np.uint32)
+    m_quad = np.full(2, nmat - 1, np.uint32)
+    instances = [(0, t1, m_box1, 0xFF), (0, t2, m_box2, 0x01), (1, ident, m_quad, 0xFF),
+                 (0, ident, m_box1, 0x100)]
+    flat_v, flat_m = [], []
+    for g, tf, mi, mask in instances:
+        if mask & 0xFF == 0:
+            continue
+        v, idx = (box_v, box_i) if g == 0 else (quad_v, None)
+        p = v[idx] if idx is not None else v
+        x, y, z = p[:, 0], p[:, 1], p[:, 2]
+        w = [((tf[r, 0] * x + tf[r, 1] * y) + tf[r, 2] * z) + tf[r, 3] for r in range(3)]
+        flat_v.append(np.stack(w + [np.zeros_like(x)], 1).astype(np.float32))
+        flat_m.append(mi)
+    # the room without its boxes, as one more (identity) instance and in the flat scene
+    room_v = base.vertices[base.indices.reshape(-1)] if base.indices is not None else base.vertices
+    room_m = base.material_idx
+    tex = base.texture_set()
+    inst = api.Scene.from_instances(
+        [(box_v, box_i), (quad_v, None), (room_v, None)],
+        instances + [(2, ident, room_m, 0xFF)], base.materials, tex)
+    flat = api.Scene(np.concatenate(flat_v + [room_v]), None,
+                     np.concatenate(flat_m + [room_m]), base.materials, tex)
+    assert inst.bvh_stats().num_triangles == flat.bvh_stats().num_triangles == 12 + 12 + 2 + len(room_m)
+    imgs = []
+    for scene in (inst, flat):
+        pipe = api.RayTracer(api.Medium).create_pipeline(
+            api.PathTracerOpts(samples=4, bounces=5, guiding=api.Guiding.OFF))
+        bufs = api.DataBuffers(64, 64, 1)
+        pipe.dispatch(scene, base.camera(), bufs, 3)
+        imgs.append((bufs.output(api.Output.COLOR), bufs.output(api.Output.ALBEDO)))
+    np.testing.assert_array_equal(imgs[0][0], imgs[1][0])
+    np.testing.assert_array_equal(imgs[0][1], imgs[1][1])
