@@ -29,6 +29,7 @@ struct lmb_batch_s {
     DeviceBuffer scene_table, item_table, seed_word;
     DeviceBuffer out_color, out_normal, out_albedo, own_seed, pix_sum, ps_seed, ps_through,
         ps_accum, queue0, queue1, hits, counters;
+    DeviceBuffer hits_primary;  // per pixel: camera-ray hits reused by later samples
     uint64_t queue_cap = 0;
     SpillBuffer spill;
     FrameDev dev{};
@@ -258,6 +259,7 @@ lmb_status lmb_batch_create(const lmb_batch_item* items, uint32_t n, const lmb_t
     if (!st) st = b->ps_through.alloc(npix * 16);
     if (!st) st = b->ps_accum.alloc(npix * 16);
     if (!st) st = b->counters.alloc(kNumCounters * 4);
+    if (!st && opts->samples > 1) st = b->hits_primary.alloc(npix * 16);
     if (st) {
         delete b;
         return st;
@@ -282,6 +284,7 @@ lmb_status lmb_batch_create(const lmb_batch_item* items, uint32_t n, const lmb_t
     f.ps_through = b->ps_through.as<float4>();
     f.ps_accum = b->ps_accum.as<float4>();
     f.counters = b->counters.as<uint32_t>();
+    f.hits_primary = b->hits_primary.as<uint4>();  // indexed by pixel here
     BatchDev& bd = b->bdev;
     bd.scenes = b->scene_table.as<SceneDev>();
     bd.num_items = n;

@@ -349,9 +349,9 @@ __global__ __launch_bounds__(kPtBlock) void k_batch_generate(BatchDev b, FrameDe
     const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
     const uint32_t tile_px = b.tile_w * b.tile_h;
     const uint64_t n = (t1 - t0) * tile_px;
-    const uint64_t n_r = (n + 63) & ~63ull;
+    const uint64_t n_r = block_uniform_count<kPtBlock>((uint32_t)n);
     const uint32_t seed_offset = *b.seed;
-    const uint32_t lane = __lane_id();
+    __shared__ ReserveLds<kPtBlock, 1> rl;
     for (uint64_t g = gtid; g < n_r; g += stride) {
         bool valid = g < n;
         uint32_t item = 0, x = 0, y = 0;
@@ -386,18 +386,28 @@ __global__ __launch_bounds__(kPtBlock) void k_batch_generate(BatchDev b, FrameDe
             f.ps_through[pid] = make_float4(1.f, 1.f, 1.f, -1.f);
             f.ps_accum[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        const uint64_t mask = __ballot(valid);
-        if (mask) {
-            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&f.counters[0], (uint32_t)__popcll(mask));
-            base = __shfl(base, (int)leader);
-            if (valid) {
-                const uint64_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-                f.queue[0][2 * slot] = o4;
-                f.queue[0][2 * slot + 1] = d4;
-            }
+        uint32_t idx[1];
+        block_reserve<kPtBlock, 1>({valid}, {f.counters}, rl, idx);
+        if (valid) {
+            f.queue[0][2 * (uint64_t)idx[0]] = o4;
+            f.queue[0][2 * (uint64_t)idx[0] + 1] = d4;
         }
+    }
+}
+
+// Camera-ray hits of a batch kept per pixel (the bounce-0 queue order
+// differs between samples): scattered after the first sample's traversal,
+// gathered into the queue's hit slots by later samples.
+__global__ __launch_bounds__(kPtBlock) void k_hits_keep(FrameDev f, uint32_t gather) {
+    const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
+    const uint32_t count = f.counters[0];
+    for (uint64_t slot = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x; slot < count;
+         slot += stride) {
+        const uint32_t pid = __float_as_uint(f.queue[0][2 * slot].w);
+        if (gather)
+            f.hits[slot] = f.hits_primary[pid];
+        else
+            f.hits_primary[pid] = f.hits[slot];
     }
 }
 
@@ -461,9 +471,16 @@ static hipError_t launch_batch_q(const BatchDev& b, const FrameDev& f, const Tra
             hipLaunchKernelGGL(k_batch_begin, dim3(1), block, 0, stream, f,
                                (uint32_t)(t0 == 0 && sample == 0));
             hipLaunchKernelGGL(k_batch_generate, grid, block, 0, stream, b, f, p, sample, t0, t1);
+            // camera rays carry no jitter: later samples reuse the first's hits
+            const bool reuse = p.samples > 1 && f.hits_primary != nullptr;
+            if (reuse && sample > 0)
+                hipLaunchKernelGGL(k_hits_keep, grid, block, 0, stream, f, 1u);
             for (uint32_t bn = 0; bn < p.bounces; ++bn) {
-                hipLaunchKernelGGL(k_batch_extend, grid, block, 0, stream, b.scenes, f, p, bn,
-                                   l.refill);
+                if (!(reuse && sample > 0 && bn == 0))
+                    hipLaunchKernelGGL(k_batch_extend, grid, block, 0, stream, b.scenes, f, p, bn,
+                                       l.refill);
+                if (reuse && sample == 0 && bn == 0)
+                    hipLaunchKernelGGL(k_hits_keep, grid, block, 0, stream, f, 0u);
                 hipLaunchKernelGGL((k_shade_plain_t<Q, true>), grid, block, 0, stream, none,
                                    b.scenes, f, p, bn);
             }

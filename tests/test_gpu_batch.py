@@ -32,9 +32,7 @@ def single(items, seed, quality=api.Medium):
         b = api.DataBuffers(w, h, 1, api.BufferType(0))
         pipe.dispatch(scene, cam, b, seed)
         outs.append(b.output(api.Output.COLOR))
-        # the single-scene tracer reuses the first sample's camera-ray hits,
-        # the batched tracer traces every sample's
-        verts += b.path_vertices() + (OPTS.samples - 1) * w * h
+        verts += b.path_vertices()  # both tracers reuse the first sample's camera-ray hits
     return outs, verts
 
 
