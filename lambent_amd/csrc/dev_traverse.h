@@ -178,14 +178,11 @@ struct Traversal {
         active = num_nodes != 0;
     }
 
-    // kTriangles: test the hit leaves' triangles here (one lane alone);
-    // otherwise hand them out as (*tri_base_out, *tri_mask_out) for the
-    // wave's shared triangle phase (shared_triangles).
-    template <int BLOCK, bool kTriangles = true>
-    __device__ __forceinline__ void step(const Bvh8Node* __restrict__ nodes,
-                                         const LeafTri* __restrict__ tris, ShortStack<BLOCK>& st,
-                                         uint32_t* tri_base_out = nullptr,
-                                         uint32_t* tri_mask_out = nullptr) {
+    // The hit leaves' triangles are handed out as (*tri_base_out,
+    // *tri_mask_out) for the wave's shared triangle phase (shared_triangles).
+    template <int BLOCK>
+    __device__ __forceinline__ void step(const Bvh8Node* __restrict__ nodes, ShortStack<BLOCK>& st,
+                                         uint32_t* tri_base_out, uint32_t* tri_mask_out) {
         const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
         const uint4 w0 = np[0];
         const uint4 w1 = np[1];
@@ -255,28 +252,8 @@ struct Traversal {
                 }
             }
         }
-        if (!kTriangles) {
-            *tri_base_out = tri_base;
-            *tri_mask_out = tri_hits;
-            tri_hits = 0;
-        }
-        while (tri_hits) {
-            const uint32_t k = __builtin_ctz(tri_hits);
-            tri_hits &= tri_hits - 1u;
-            const float4* tp = reinterpret_cast<const float4*>(tris + tri_base + k);
-            const float4 a = tp[0], b = tp[1], c = tp[2];
-            float t, u, v;
-            if (intersect_triangle(sh, r.ox, r.oy, r.oz, a, b, c, r.tmin, h.t, &t, &u, &v)) {
-                const uint32_t prim = __float_as_uint(a.w);
-                if (t < h.t || prim < h.prim) {
-                    h.t = t;
-                    h.u = u;
-                    h.v = v;
-                    h.prim = prim;
-                    h.leaf = tri_base + k;
-                }
-            }
-        }
+        *tri_base_out = tri_base;
+        *tri_mask_out = tri_hits;
         uint32_t gbase, ghits, gimask;
         if (node_hits) {
             gbase = child_base;
@@ -299,19 +276,6 @@ struct Traversal {
         node = gbase + __builtin_popcount(gimask & ((1u << slot) - 1u));
     }
 };
-
-// Whole traversal of one ray by one lane.
-template <int BLOCK>
-__device__ __forceinline__ Hit trace_closest(const Bvh8Node* __restrict__ nodes,
-                                             const LeafTri* __restrict__ tris,
-                                             uint32_t num_nodes, const Ray& r,
-                                             ShortStack<BLOCK>& st) {
-    Traversal tv;
-    tv.init(r, num_nodes);
-    st.sp = 0;
-    while (tv.active) tv.step<BLOCK>(nodes, tris, st);
-    return tv.h;
-}
 
 // Scene binding of the queue traversal: every ray of a OneScene queue uses
 // the same BVH (uniform pointers, kept in scalar registers); a SceneTable
@@ -552,7 +516,7 @@ __device__ __forceinline__ void traverse_queue_t(Scenes& sc, uint64_t count, uin
         }
         const bool was_active = tv.active;
         uint32_t tbase = 0, tmask = 0;
-        if (was_active) tv.step<BLOCK, false>(sc.nodes(), sc.tris(), st, &tbase, &tmask);
+        if (was_active) tv.step<BLOCK>(sc.nodes(), st, &tbase, &tmask);
         shared_triangles(tv, tbase, tmask, sc, ts, lane);
         if (was_active && !tv.active) src.store(slot, tv.h);
         if (exhausted && !__any(tv.active)) break;

@@ -3,8 +3,10 @@
 // Per sample: generate -> extend / shade for every bounce -> inference.
 // Shading is the plain tracer's (surface_scatter, Russian roulette) except at
 // the second vertex (bounce 1):
-//   * about train_expect of the paths reaching it (a hash of path id, frame
-//     and sample picks them) become training records: the record keeps the
+//   * in a trace's first sample, about train_expect of the paths reaching it
+//     (a hash of path id, frame and sample picks them, with the probability
+//     calibrated on the previous sample's count of cache vertices) become
+//     training records: the record keeps the
 //     network inputs there, the throughput T and the radiance S gathered so
 //     far; the path restarts its throughput at 1 and its sum at 0 and is
 //     traced on to full depth; when it ends, its sum (the radiance scattered
@@ -22,7 +24,11 @@
 // there is blurred by a lobe instead of shown sharply, and specular chains
 // stop n vertices late at most (their short queues are latency-bound).
 // Every path still meets at most one cache vertex; training continuations
-// (rslot != 0) never meet another.
+// (rslot != 0) never meet another.  Later samples of a trace reuse the
+// camera-ray hits, carry no training paths and stop after the last possible
+// cache vertex.  A training frame (train_only) traces the camera rays of
+// random pixels (the fraction that gives about train_expect cache
+// vertices), makes every cache vertex a training record and writes no image.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
