@@ -81,7 +81,7 @@ def rel_mse(img, ref) -> float:
 
 
 def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
-              lr: float = 1e-3, spp: int = 8, steps: int = 32) -> dict:
+              lr: float = 1e-3, spp: int = 8, steps: int = 32, overlap: bool = True) -> dict:
     """Time-to-relMSE of the cached tracer against the plain tracer, both with
     8-bounce paths (RR from bounce 3), judged against a plain reference of
     ref_spp samples accumulated in fp64.  The cache starts untrained: `warm`
@@ -89,8 +89,10 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
     train it, no image; timed), then `frames` frames of `spp`
     samples (the camera rays are traced once per frame, training paths come
     from the first sample), `steps` Adam steps per frame over the frame's
-    65536 training records.  Times are wall times of the dispatches (host
-    read-back of the images for scoring is not counted)."""
+    65536 training records (overlap: the training pass and the steps run on
+    the cache's own stream beside the frame's samples).  Times are wall times
+    of the dispatches (host read-back of the images for scoring is not
+    counted)."""
     import numpy as np
     W, H = spec.width, spec.height
     popts = dict(bounces=8, rr_start=3, guiding=api.Guiding.OFF,
@@ -119,7 +121,8 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
         plain_pts.append({"spp": n_spp, "seconds": dt, "relmse": rel_mse(img, ref)})
         log(f"cache leg: plain {n_spp} spp {dt * 1e3:.1f} ms relMSE {plain_pts[-1]['relmse']:.5f}")
 
-    cache = api.NeuralCache(api.NeuralCacheOpts(learning_rate=lr, train_steps=steps))
+    cache = api.NeuralCache(api.NeuralCacheOpts(learning_rate=lr, train_steps=steps,
+                                                overlap_training=overlap))
     warm_pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=1, **popts))
     pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=spp, **popts))
     b = api.DataBuffers(W, H, 1, api.BufferType(0))
@@ -170,8 +173,9 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
     return {
         "workload": f"procedural {W}x{H}, 8 bounces, RR from 3; cache: paths end at the second "
                     f"vertex (a specular scatter moves it one vertex on), 65536 training paths "
-                    f"and {steps} Adam steps (lr {lr:g}) per frame; {warm} training frames "
-                    f"(no image, timed), then frames of {spp} spp",
+                    f"and {steps} Adam steps (lr {lr:g}) per frame"
+                    f"{' on a second stream beside the samples' if overlap else ''}; {warm} "
+                    f"training frames (no image, timed), then frames of {spp} spp",
         "reference": f"plain tracer {ref_spp} spp, fp64 accumulation of {chunk}-spp dispatches",
         "plain": plain_pts,
         "cache_points": pts,
@@ -351,6 +355,8 @@ def main() -> int:
     ap.add_argument("--cache-lr", type=float, default=1e-3)
     ap.add_argument("--cache-spp", type=int, default=8)
     ap.add_argument("--cache-steps", type=int, default=32)
+    ap.add_argument("--cache-no-overlap", action="store_true",
+                    help="cache training on the frame's stream after its samples")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (rocprofv3 counter calibration)")
     args = ap.parse_args()
@@ -453,7 +459,7 @@ def main() -> int:
     if not args.no_cache_leg:
         out["time_to_relmse"] = cache_leg(api, scene, spec, cam, args.ref_spp, args.cache_frames,
                                           args.cache_warmup, args.cache_lr, args.cache_spp,
-                                          args.cache_steps)
+                                          args.cache_steps, not args.cache_no_overlap)
     if not args.no_spectral_leg:
         del scene
         out["spectral_restir"] = spectral_leg(api, scenes, args.triangles, args.width,

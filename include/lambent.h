@@ -442,6 +442,16 @@ typedef struct lmb_nrc_opts {
                               at its second vertex but at the vertex after
                               its first rough scatter, at most n vertices
                               later; 0: always the second vertex (<= 64) (1) */
+    uint32_t overlap_training; /* 1: lmb_trace_nrc runs its training (a
+                              training pass as lmb_trace_nrc_train's, then the
+                              Adam steps) on a stream of the cache, beside
+                              the frame's samples, which all end at their
+                              cache vertices; 0: the training paths come from
+                              the frame's first sample and the steps follow
+                              the samples on the caller's stream.  Either way
+                              the samples of frame k use the weights trained
+                              up to frame k - 1, and the call's work is
+                              ordered on the caller's stream              (1) */
 } lmb_nrc_opts;
 
 void lmb_nrc_opts_default(lmb_nrc_opts *opts);
@@ -481,10 +491,12 @@ lmb_status lmb_nrc_stats_get(lmb_nrc nrc, lmb_nrc_stats *stats);
 /* One frame of the cached tracer: for each of the tracer's samples, every
  * path is traced to its second vertex (a later one past specular scatters,
  * see defer_specular), where it ends with
- * throughput * cache(vertex); about train_paths of them (chosen at random
- * there) continue to full depth instead and become training records (their
- * full-path estimate also goes to the image).  Then train_steps Adam steps
- * on the records.  Uses the tracer's quality, samples, bounces, rr_start,
+ * throughput * cache(vertex).  Training (overlap_training = 0): in the first
+ * sample about train_paths of them (chosen at random there) continue to full
+ * depth instead and become training records (their full-path estimate also
+ * goes to the image), then train_steps Adam steps on the records.  With
+ * overlap_training = 1 the records come from a training pass of their own
+ * traced beside the samples.  Uses the tracer's quality, samples, bounces, rr_start,
  * t_min / t_max; guiding must be off. */
 lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera *cam,
                          lmb_data_buffers bufs, uint32_t seed_offset, lmb_nrc nrc,

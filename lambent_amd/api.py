@@ -153,7 +153,7 @@ class _NrcOptsC(ctypes.Structure):
                 ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float),
                 ("train_paths", ctypes.c_uint32), ("train_steps", ctypes.c_uint32),
                 ("seed", ctypes.c_uint32), ("ema_decay", ctypes.c_float),
-                ("defer_specular", ctypes.c_uint32)]
+                ("defer_specular", ctypes.c_uint32), ("overlap_training", ctypes.c_uint32)]
 
 
 class _NrcStatsC(ctypes.Structure):
@@ -1014,6 +1014,7 @@ class NeuralCacheOpts:
     seed: int = 1
     ema_decay: float = 0.0
     defer_specular: int = 1
+    overlap_training: bool = True
 
 
 def _nrc_opts_c(o: NeuralCacheOpts) -> _NrcOptsC:
@@ -1022,6 +1023,7 @@ def _nrc_opts_c(o: NeuralCacheOpts) -> _NrcOptsC:
     c.train_paths, c.train_steps, c.seed = o.train_paths, o.train_steps, o.seed
     c.ema_decay = o.ema_decay
     c.defer_specular = int(o.defer_specular)
+    c.overlap_training = 1 if o.overlap_training else 0
     return c
 
 

@@ -476,6 +476,10 @@ lmb_status lmb_trace(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
     return LMB_OK;
 }
 
+// Samples of an overlapped cached frame traced together (their state is
+// 132 bytes per pixel and sample).
+constexpr uint32_t kWideSamples = 8;
+
 static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
                             lmb_data_buffers bufs, uint32_t seed_offset, lmb_nrc nrc, void* stream,
                             bool train_only) {
@@ -518,13 +522,112 @@ static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera
     np.defer_specular = nrc->opts.defer_specular;
     np.train_only = train_only ? 1u : 0u;
     const hipStream_t hs = (hipStream_t)stream;
-    // The first shading launch zeroes the record count (after counting the
-    // previous trace's last sample for the selection probability).
-    hipError_t e = launch_trace_nrc(scene->dev, cd, bufs->dev, p, nrc->dev, np, hs, bufs->timer.get());
+    KernelTimer* tm = bufs->timer.get();
+    if (train_only || !nrc->opts.overlap_training || nrc->opts.train_paths == 0) {
+        st = nrc_sync_frags(nrc, hs);
+        if (st) return st;
+        nrc->rec_in_train_ctl = false;
+        // The first shading launch zeroes the record count (after counting
+        // the previous trace's last sample for the selection probability).
+        hipError_t e = launch_trace_nrc(scene->dev, cd, bufs->dev, p, nrc->dev, np, hs, tm);
+        if (e != hipSuccess) return fail_hip(e, "cached trace launch");
+        if (nrc->opts.train_paths == 0) return LMB_OK;
+        return nrc_run_steps(nrc, nrc->dev, nullptr, nrc->dev.rec_in, nrc->dev.rec_target,
+                             nrc->dev.ctl + kNrcCtlRecords, nrc->dev.rec_cap, steps, true, nullptr,
+                             hs, tm);
+    }
+    // Overlapped training.  On the caller's stream: every sample ends at its
+    // cache vertices.  On the training stream: a training pass (the camera
+    // paths of about train_paths random pixels, as in a training frame, with
+    // its own control words and path state) and the Adam steps, whose last
+    // step publishes the new inference fragments into the set the samples do
+    // not read.  The samples are enqueued first, so the GPU starts on them
+    // while the host enqueues the training; at the end the caller's stream
+    // waits for the training, so the call's work is complete when the
+    // caller's stream is.
+    st = nrc_ensure_overlap(nrc, npix);
+    if (st) return st;
+    const hipStream_t ts = nrc->train_stream;
+    uint16_t* sets = nrc->inf_sets.as<uint16_t>();
+    uint16_t* cur = sets + (size_t)nrc->inf_cur * kNrcFwdFrags * kFragElems;
+    uint16_t* nxt = sets + (size_t)(nrc->inf_cur ^ 1u) * kNrcFwdFrags * kFragElems;
+    hipError_t e = hipSuccess;
+    if (!nrc->inf_valid) {  // weights changed since the last overlapped frame
+        NrcDev dp = nrc->dev;
+        dp.fwd_pub = cur;
+        e = nrc_launch_pack(dp, hs);
+        if (e != hipSuccess) return fail_hip(e, "cache weight pack");
+    }
+    NrcDev da = nrc->dev;
+    da.fwd_inf = cur;
+    da.fwd_pub = nullptr;
+    NrcDev db = nrc->dev;
+    db.ctl = nrc->ctl_train.as<uint32_t>();
+    db.fwd_pub = nullptr;
+    // The training pass's pixel fraction from the cache vertices per sample
+    // counted so far on the caller's stream.
+    e = hipMemcpyAsync(db.ctl + kNrcCtlHitsPrev, da.ctl + kNrcCtlHitsPrev, 4,
+                       hipMemcpyDeviceToDevice, hs);
+    if (e == hipSuccess) e = hipEventRecord(nrc->ev_start, hs);
+    if (e != hipSuccess) return fail_hip(e, "training stream order");
+    np.train_expect = 0.0f;  // no training paths among the samples
+    // Samples traced up to kWideSamples at a time (path id j * npix + pixel
+    // must fit 31 bits).
+    const uint64_t w_max = std::min<uint64_t>(kWideSamples, 0x7FFFFFFFull / std::max<uint64_t>(npix, 1));
+    const uint32_t w = (uint32_t)std::min<uint64_t>(p.samples, w_max);
+    if (w > 1 && bufs->dev.hits_primary) {
+        st = nrc_ensure_wide(nrc, (uint64_t)w * npix);
+        if (!st) st = nrc_ensure_queries(nrc, (uint64_t)w * npix);
+        if (st) return st;
+        da.queries = nrc->dev.queries;
+        da.query_weight = nrc->dev.query_weight;
+        da.query_cap = nrc->dev.query_cap;
+        FrameDev fw = bufs->dev;
+        fw.ps_seed = nrc->w_seed.as<uint32_t>();
+        fw.ps_through = nrc->w_through.as<float4>();
+        fw.ps_accum = nrc->w_accum.as<float4>();
+        fw.pix_sum = nrc->w_sum.as<float4>();
+        fw.queue[0] = nrc->w_queue0.as<float4>();
+        fw.queue[1] = nrc->w_queue1.as<float4>();
+        fw.hits = nrc->w_hits.as<uint4>();
+        e = launch_trace_nrc_wide(scene->dev, cd, bufs->dev, fw, p, da, np, w, hs, tm);
+    } else {
+        e = launch_trace_nrc(scene->dev, cd, bufs->dev, p, da, np, hs, tm);
+    }
     if (e != hipSuccess) return fail_hip(e, "cached trace launch");
-    if (nrc->opts.train_paths == 0) return LMB_OK;
-    return nrc_run_steps(nrc, nrc->dev.rec_in, nrc->dev.rec_target, nrc->dev.ctl + kNrcCtlRecords,
-                         nrc->dev.rec_cap, steps, true, nullptr, hs, bufs->timer.get());
+    e = hipStreamWaitEvent(ts, nrc->ev_start, 0);
+    if (e != hipSuccess) return fail_hip(e, "training stream order");
+    FrameDev ft = bufs->dev;
+    ft.out_color = ft.out_normal = ft.out_albedo = nullptr;  // a training pass writes no image
+    ft.pix_sum = nullptr;
+    ft.ps_seed = nrc->tf_seed.as<uint32_t>();
+    ft.ps_through = nrc->tf_through.as<float4>();
+    ft.ps_accum = nrc->tf_accum.as<float4>();
+    ft.queue[0] = nrc->tf_queue0.as<float4>();
+    ft.queue[1] = nrc->tf_queue1.as<float4>();
+    ft.hits = nrc->tf_hits.as<uint4>();
+    ft.hits_primary = nullptr;
+    ft.counters = nrc->tf_counters.as<uint32_t>();
+    ft.spill = nrc->tf_spill.ptr;
+    ft.grid_paths = std::max<uint64_t>(nrc->opts.train_paths, 256);
+    TraceParams pt = p;
+    pt.samples = 1;
+    NrcTraceParams npt = np;
+    npt.train_expect = (float)nrc->opts.train_paths;
+    npt.train_only = 1u;
+    e = launch_trace_nrc(scene->dev, cd, ft, pt, db, npt, ts, tm);
+    if (e != hipSuccess) return fail_hip(e, "cached trace training pass");
+    st = nrc_run_steps(nrc, db, nxt, db.rec_in, db.rec_target, db.ctl + kNrcCtlRecords, db.rec_cap,
+                       steps, true, nullptr, ts, tm);
+    if (st) return st;
+    e = hipEventRecord(nrc->ev_done, ts);
+    if (e == hipSuccess) e = hipStreamWaitEvent(hs, nrc->ev_done, 0);
+    if (e != hipSuccess) return fail_hip(e, "training stream order");
+    nrc->inf_cur ^= 1u;  // the next frame reads what the steps published
+    nrc->inf_valid = true;
+    nrc->frags_stale = true;
+    nrc->rec_in_train_ctl = true;
+    return LMB_OK;
 }
 
 lmb_status lmb_trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,

@@ -96,20 +96,92 @@ static AdamParams next_adam(lmb_nrc n) {
     return a;
 }
 
-lmb_status nrc_run_steps(lmb_nrc n, const NrcQuery* in, const float4* target,
-                         const uint32_t* count, uint32_t count_limit, uint32_t steps, bool apply,
-                         float* grad_out, hipStream_t stream, KernelTimer* timer) {
-    hipError_t e = launch_fill_zero(n->dev.loss, 64 * sizeof(float), stream);
+lmb_status nrc_run_steps(lmb_nrc n, const NrcDev& d, uint16_t* publish, const NrcQuery* in,
+                         const float4* target, const uint32_t* count, uint32_t count_limit,
+                         uint32_t steps, bool apply, float* grad_out, hipStream_t stream,
+                         KernelTimer* timer) {
+    hipError_t e = launch_fill_zero(d.loss, 64 * sizeof(float), stream);
     if (e != hipSuccess) return fail_hip(e, "loss clear");
     for (uint32_t s = 0; s < steps; ++s) {
         const AdamParams a = apply ? next_adam(n) : AdamParams{};
+        NrcDev ds = d;
+        if (publish && s + 1 == steps) ds.fwd_pub = publish;
         if (timer) timer->begin(kKernelCacheTrain, stream);
-        e = nrc_launch_step(n->dev, in, target, count, count_limit, steps, s, a, apply, grad_out,
+        e = nrc_launch_step(ds, in, target, count, count_limit, steps, s, a, apply, grad_out,
                             nullptr, stream);
         if (timer) timer->end(kKernelCacheTrain, stream);
         if (e != hipSuccess) return fail_hip(e, "cache training step");
     }
     n->last_steps = steps;
+    if (apply) n->inf_valid = false;
+    return LMB_OK;
+}
+
+lmb_status nrc_sync_frags(lmb_nrc n, hipStream_t stream) {
+    if (!n->frags_stale) return LMB_OK;
+    hipError_t e = nrc_launch_pack(n->dev, stream);
+    if (e != hipSuccess) return fail_hip(e, "cache weight pack");
+    n->frags_stale = false;
+    return LMB_OK;
+}
+
+lmb_status nrc_ensure_overlap(lmb_nrc n, uint64_t npix) {
+    hipError_t e = hipSuccess;
+    if (!n->train_stream) {
+        // non-blocking: no implicit ordering against a caller on the null stream
+        e = hipStreamCreateWithFlags(&n->train_stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&n->ev_start, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&n->ev_done, hipEventDisableTiming);
+        if (e != hipSuccess) return fail_hip(e, "training stream");
+    }
+    if (!n->ctl_train.ptr) {
+        lmb_status st = n->ctl_train.alloc(kNrcCtlWords * 4);
+        if (st) return st;
+        e = hipMemset(n->ctl_train.ptr, 0, kNrcCtlWords * 4);
+        if (e != hipSuccess) return fail_hip(e, "hipMemset");
+    }
+    if (!n->inf_sets.ptr) {
+        lmb_status st = n->inf_sets.alloc(2 * (size_t)kNrcFwdFrags * kFragElems * 2);
+        if (st) return st;
+        n->inf_valid = false;
+    }
+    const uint64_t grid = std::max<uint64_t>(n->opts.train_paths, 256);
+    if (npix <= n->tf_npix && grid <= n->tf_grid) return LMB_OK;
+    // the buffers may still be in use by the previous frame's training pass
+    e = hipStreamSynchronize(n->train_stream);
+    if (e != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
+    const uint64_t np = std::max(npix, n->tf_npix);
+    lmb_status st = LMB_OK;
+    if (np > n->tf_npix) {
+        n->tf_npix = 0;
+        if (!st) st = n->tf_seed.alloc(np * 4);
+        if (!st) st = n->tf_through.alloc(np * 16);
+        if (!st) st = n->tf_accum.alloc(np * 16);
+        if (!st) st = n->tf_queue0.alloc(np * 32);
+        if (!st) st = n->tf_queue1.alloc(np * 32);
+        if (!st) st = n->tf_hits.alloc(np * 16);
+        if (!st && !n->tf_counters.ptr) st = n->tf_counters.alloc(kNumCounters * 4);
+        if (st) return st;
+        n->tf_npix = np;
+    }
+    st = ensure_spill(&n->tf_spill, extend_threads(std::max(grid, n->tf_grid)));
+    if (st) return st;
+    n->tf_grid = std::max(grid, n->tf_grid);
+    return LMB_OK;
+}
+
+lmb_status nrc_ensure_wide(lmb_nrc n, uint64_t paths) {
+    if (paths <= n->wide_paths) return LMB_OK;
+    n->wide_paths = 0;
+    lmb_status st = n->w_seed.alloc(paths * 4);
+    if (!st) st = n->w_through.alloc(paths * 16);
+    if (!st) st = n->w_accum.alloc(paths * 16);
+    if (!st) st = n->w_sum.alloc(paths * 16);
+    if (!st) st = n->w_queue0.alloc(paths * 32);
+    if (!st) st = n->w_queue1.alloc(paths * 32);
+    if (!st) st = n->w_hits.alloc(paths * 16);
+    if (st) return st;
+    n->wide_paths = paths;
     return LMB_OK;
 }
 
@@ -191,8 +263,8 @@ lmb_status train_or_grad(lmb_nrc n, const float* inputs, const float* targets, u
         }
         gdev = n->io_grad.as<float>();
     }
-    st = nrc_run_steps(n, n->io_in.as<NrcQuery>(), n->io_target.as<float4>(), cnt, (uint32_t)count,
-                       1, apply, gdev, nullptr, nullptr);
+    st = nrc_run_steps(n, n->dev, nullptr, n->io_in.as<NrcQuery>(), n->io_target.as<float4>(), cnt,
+                       (uint32_t)count, 1, apply, gdev, nullptr, nullptr);
     if (st) return st;
     hipError_t e = hipDeviceSynchronize();
     if (e != hipSuccess) return fail_hip(e, "cache step");
@@ -220,6 +292,7 @@ void lmb_nrc_opts_default(lmb_nrc_opts* o) {
     o->seed = 1;
     o->ema_decay = 0.0f;
     o->defer_specular = 1u;
+    o->overlap_training = 1u;
 }
 
 static lmb_status validate(const lmb_nrc_opts& o) {
@@ -234,6 +307,8 @@ static lmb_status validate(const lmb_nrc_opts& o) {
         return fail(LMB_ERR_INVALID_ARGUMENT, "ema_decay must be in [0, 1)");
     if (o.defer_specular > kMaxBounces)
         return fail(LMB_ERR_INVALID_ARGUMENT, "defer_specular must be at most 64");
+    if (o.overlap_training > 1u)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "overlap_training must be 0 or 1");
     return LMB_OK;
 }
 
@@ -260,6 +335,8 @@ static lmb_status ensure_records(lmb_nrc n, uint32_t train_paths) {
 static lmb_status ensure_ema(lmb_nrc n) {
     NrcDev& d = n->dev;
     d.ema_decay = n->opts.ema_decay;
+    n->inf_valid = false;
+    n->frags_stale = false;
     if (n->opts.ema_decay > 0.0f) {
         if (!n->ema.ptr) {
             lmb_status st = n->ema.alloc(kNrcNumWeights * sizeof(float));
@@ -270,10 +347,11 @@ static lmb_status ensure_ema(lmb_nrc n) {
             if (e != hipSuccess) return fail_hip(e, "hipMemcpy weights");
         }
         d.ema = n->ema.as<float>();
-        d.fwd_inf = n->fwd_inf.as<uint16_t>();
+        d.fwd_inf = d.fwd_pub = n->fwd_inf.as<uint16_t>();
     } else {
         d.ema = nullptr;
         d.fwd_inf = d.fwd;
+        d.fwd_pub = nullptr;
     }
     hipError_t e = nrc_launch_pack(d, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -361,6 +439,9 @@ lmb_status lmb_nrc_destroy(lmb_nrc n) {
     if (!n) return LMB_OK;
     (void)hipSetDevice(n->device);
     (void)hipDeviceSynchronize();
+    if (n->ev_start) (void)hipEventDestroy(n->ev_start);
+    if (n->ev_done) (void)hipEventDestroy(n->ev_done);
+    if (n->train_stream) (void)hipStreamDestroy(n->train_stream);
     delete n;
     return LMB_OK;
 }
@@ -393,6 +474,8 @@ lmb_status lmb_nrc_set_weights(lmb_nrc n, const float* w) {
     if (e != hipSuccess) return fail_hip(e, "cache weight upload");
     n->steps = 0;
     n->pow1 = n->pow2 = 1.0;
+    n->inf_valid = false;
+    n->frags_stale = false;
     return LMB_OK;
 }
 
@@ -403,6 +486,7 @@ lmb_status lmb_nrc_infer(lmb_nrc n, const float* inputs, uint64_t count, float* 
     if (!inputs || !outputs) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
     if (count > 0x7FFFFFFFull) return fail(LMB_ERR_INVALID_ARGUMENT, "too many rows");
     st = stage_inputs(n, inputs, count);
+    if (!st) st = nrc_sync_frags(n, nullptr);
     if (st) return st;
     if (n->io_out.bytes < count * 16) {
         st = n->io_out.alloc(count * 16);
@@ -445,6 +529,8 @@ lmb_status lmb_nrc_apply_gradient(lmb_nrc n, const float* grad) {
     const AdamParams a = next_adam(n);
     e = nrc_launch_step(n->dev, nullptr, nullptr, nullptr, 0, 1, 0, a, true, nullptr,
                         n->io_grad.as<float>(), nullptr);
+    n->inf_valid = false;
+    n->frags_stale = false;  // the step re-packed every fragment set of n->dev
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) return fail_hip(e, "cache Adam step");
     return LMB_OK;
@@ -460,7 +546,11 @@ lmb_status lmb_nrc_stats_get(lmb_nrc n, lmb_nrc_stats* s) {
     if (e != hipSuccess) return fail_hip(e, "hipMemcpy cache counters");
     std::memset(s, 0, sizeof *s);
     s->steps = n->steps;
-    s->records = std::min<uint32_t>(ctl[kNrcCtlRecords], n->dev.rec_cap);
+    uint32_t records = ctl[kNrcCtlRecords];
+    if (n->rec_in_train_ctl && e == hipSuccess)
+        e = hipMemcpy(&records, (uint32_t*)n->ctl_train.ptr + kNrcCtlRecords, 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy cache counters");
+    s->records = std::min<uint32_t>(records, n->dev.rec_cap);
     s->queries = ctl[kNrcCtlQueries];
     s->loss = read_loss(n, s->records, n->last_steps);
     return LMB_OK;

@@ -62,7 +62,12 @@ struct FrameDev {
     uint64_t mc_count;   // entries of mc / mc_old
     uint32_t world_size;
     uint32_t flags;
+    // Paths that size the persistent grids of this frame's launches (0: npix);
+    // a pass over a small subset of the pixels runs on a smaller grid.
+    uint64_t grid_paths;
 };
+
+inline uint64_t grid_paths(const FrameDev& f) { return f.grid_paths ? f.grid_paths : f.npix; }
 
 struct TraceParams {
     uint32_t quality;

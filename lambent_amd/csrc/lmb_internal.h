@@ -84,6 +84,25 @@ struct lmb_nrc_s {
     uint64_t steps = 0;                  // Adam steps taken (t)
     double pow1 = 1.0, pow2 = 1.0;       // beta1^t, beta2^t
     uint32_t last_steps = 0;             // optimiser steps of the last call
+    // Overlapped training (opts.overlap_training): the training pass and the
+    // optimiser steps of a cached frame run on train_stream beside the
+    // frame's samples, with their own control words and tracing state; the
+    // samples read inference fragment set inf_cur while the steps publish
+    // the next set into the other one.
+    hipStream_t train_stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_done = nullptr;
+    lmb::DeviceBuffer ctl_train, inf_sets;  // inf_sets: 2 x kNrcFwdFrags fragments
+    uint32_t inf_cur = 0;
+    bool inf_valid = false;    // set inf_cur holds the current inference weights
+    bool frags_stale = false;  // dev.fwd_pub (EMA fragments) behind the weights
+    bool rec_in_train_ctl = false;  // the last trace's records are counted in ctl_train
+    uint64_t tf_npix = 0, tf_grid = 0;
+    lmb::DeviceBuffer tf_seed, tf_through, tf_accum, tf_queue0, tf_queue1, tf_hits, tf_counters;
+    lmb::SpillBuffer tf_spill;
+    // Wide tracing of an overlapped frame's samples: state for wide_paths
+    // path ids (samples traced together x pixels).
+    uint64_t wide_paths = 0;
+    lmb::DeviceBuffer w_seed, w_through, w_accum, w_sum, w_queue0, w_queue1, w_hits;
 };
 
 namespace lmb {
@@ -97,7 +116,19 @@ lmb_status nrc_ensure_queries(lmb_nrc n, uint64_t queries);
 // `steps` optimiser steps over the min(*count, count_limit) records of
 // in / target (*count is a device word), record i * steps + s in step s.
 // Zeroes the per-step loss sums first.
-lmb_status nrc_run_steps(lmb_nrc n, const NrcQuery* in, const float4* target,
-                         const uint32_t* count, uint32_t count_limit, uint32_t steps, bool apply,
-                         float* grad_out, hipStream_t stream, KernelTimer* timer);
+// d is the cache's device view to launch with (n->dev, or a copy with other
+// control words); publish (optional) receives the inference fragments of the
+// weights after the last step.
+lmb_status nrc_run_steps(lmb_nrc n, const NrcDev& d, uint16_t* publish, const NrcQuery* in,
+                         const float4* target, const uint32_t* count, uint32_t count_limit,
+                         uint32_t steps, bool apply, float* grad_out, hipStream_t stream,
+                         KernelTimer* timer);
+// Brings the synchronous inference fragments (dev.fwd_inf) up to date after
+// overlapped training, on `stream`.
+lmb_status nrc_sync_frags(lmb_nrc n, hipStream_t stream);
+// The overlapped form of a cached frame's training: buffers for a frame of
+// npix pixels, the stream and events (created on first use).
+lmb_status nrc_ensure_overlap(lmb_nrc n, uint64_t npix);
+// State of the wide tracing for `paths` path ids.
+lmb_status nrc_ensure_wide(lmb_nrc n, uint64_t paths);
 }  // namespace lmb

@@ -132,7 +132,8 @@ struct NrcDev {
     // Inference weights: an exponential moving average of the masters and its
     // forward fragments, or (ema == nullptr) the masters and `fwd` themselves.
     float* ema;
-    uint16_t* fwd_inf;
+    uint16_t* fwd_inf;  // read by the inference kernels
+    uint16_t* fwd_pub;  // written by k_nrc_pack from the inference weights (null: not written)
     float ema_decay;
     // Training scratch, feature-major bf16 with row stride `batch`:
     // x0 [320][batch], h [5][64][batch], dz [5][64][batch], g [3][batch].
@@ -177,10 +178,21 @@ struct NrcTraceParams {
     uint32_t sample;
     uint32_t defer_specular;  // most vertices a specular chain moves the cache vertex
     uint32_t train_only;      // a training frame: random pixels, no image
+    uint32_t wide;            // > 1: samples traced up to this many at a time (path id
+                              // pixel * wide + j for sample `sample` + j)
+    uint32_t wide_count;      // samples of the current wide group (<= wide)
 };
 hipError_t launch_trace_nrc(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
                             const TraceParams& p, const NrcDev& d, NrcTraceParams np,
                             hipStream_t stream, KernelTimer* timer);
+// The samples of a trace w at a time, without training paths: f_img is the
+// image's frame (outputs, camera rays and their hits), fw the wide state
+// (ps_*, pix_sum, queues and hits for w * npix path ids; counters and spill
+// may be f_img's).  d.queries must hold w * npix queries.
+hipError_t launch_trace_nrc_wide(const SceneDev& s, const CameraDev& cam, const FrameDev& f_img,
+                                 const FrameDev& fw, const TraceParams& p, const NrcDev& d,
+                                 NrcTraceParams np, uint32_t w, hipStream_t stream,
+                                 KernelTimer* timer);
 
 // One optimiser step on batch `step` of `steps`: records i * steps + step,
 // i < d.batch, of the min(*count, count_limit) records `in` / `target`

@@ -172,7 +172,8 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_pack(NrcDev d) {
     if (frag < kNrcFwdFrags) {
         const int w = nrc_fwd_index(frag, lane, e);
         d.fwd[i] = w >= 0 ? bits((__bf16)d.master[w]) : (uint16_t)0;
-        if (d.ema) d.fwd_inf[i] = w >= 0 ? bits((__bf16)d.ema[w]) : (uint16_t)0;
+        if (d.fwd_pub)
+            d.fwd_pub[i] = w >= 0 ? bits((__bf16)(d.ema ? d.ema[w] : d.master[w])) : (uint16_t)0;
     } else {
         const int w = nrc_bwd_index(frag - kNrcFwdFrags, lane, e);
         d.bwd[i - kNrcFwdFrags * kFragElems] = w >= 0 ? bits((__bf16)d.master[w]) : (uint16_t)0;

@@ -106,7 +106,22 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
         bool armed = false;
         V3 color = v3s(0.f), scol = v3s(0.f);
         float picked = -1.0f;
-        if (active) {
+        if (active && np.wide > 1u && bounce == 0) {
+            // Wide group: bounce-0 slot pixel * wide_count + j is sample
+            // sample + j of the pixel, path id pixel * wide + j; the camera
+            // ray and its hit are the pixel's (qin and hits in pixel order),
+            // the path starts fresh with k_generate's seed for that sample.
+            const uint32_t pixel = (uint32_t)slot / np.wide_count;
+            const uint32_t j = (uint32_t)slot - pixel * np.wide_count;
+            pid = pixel * np.wide + j;
+            o4 = qin[2 * (uint64_t)pixel];
+            d4 = qin[2 * (uint64_t)pixel + 1];
+            h = f.hits[pixel];
+            const uint32_t x = pixel % f.width, y = pixel / f.width;
+            seed = rand_u32(x * y) + p.seed_offset + y;  // general.wesl:516, 544
+            for (uint32_t k = 0; k < np.sample + j; ++k) seed = rand_u32(seed);
+            color = v3s(1.0f);
+        } else if (active) {
             o4 = qin[2 * slot];
             d4 = qin[2 * slot + 1];
             pid = __float_as_uint(o4.w);
@@ -121,6 +136,9 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
             armed = (rslot & kArmedBit) != 0u;
             rslot &= ~kArmedBit;
         }
+        // the pixel of an image output (a wide group writes them from j = 0)
+        const bool img_px = np.wide > 1u ? pid % np.wide == 0u : true;
+        const uint32_t opix = np.wide > 1u ? pid / np.wide : pid;
         const bool hit = active && h.x != kMissPrim;
         const V3 dir = v3(d4.x, d4.y, d4.z);
         const V3 org = v3(o4.x, o4.y, o4.z);
@@ -132,9 +150,9 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
             const float t = __uint_as_float(h.y);
             it = make_isect(s, h.x, t, __uint_as_float(h.z), __uint_as_float(h.w), dir);
             new_pos = fma3(dir, v3s(t), org);
-            if (bounce == 0 && !np.train_only) {
-                f.out_normal[pid] = make_float4(it.normal.x, it.normal.y, it.normal.z, 1.f);
-                f.out_albedo[pid] = make_float4(it.color.x, it.color.y, it.color.z, 1.f);
+            if (bounce == 0 && !np.train_only && img_px) {
+                f.out_normal[opix] = make_float4(it.normal.x, it.normal.y, it.normal.z, 1.f);
+                f.out_albedo[opix] = make_float4(it.color.x, it.color.y, it.color.z, 1.f);
             }
             scol = fma3(color, it.emission, scol);  // general.wesl:714
             if (bounce >= kCacheBounce && armed && rslot == 0u) {
@@ -207,9 +225,9 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
                     }
                 }
             } else {
-                if (bounce == 0 && !np.train_only) {
-                    f.out_normal[pid] = make_float4(0.f, 0.f, 0.f, 1.f);
-                    f.out_albedo[pid] = make_float4(0.f, 0.f, 0.f, 1.f);
+                if (bounce == 0 && !np.train_only && img_px) {
+                    f.out_normal[opix] = make_float4(0.f, 0.f, 0.f, 1.f);
+                    f.out_albedo[opix] = make_float4(0.f, 0.f, 0.f, 1.f);
                 }
                 scol = fma3(color, background(s, dir), scol);  // general.wesl:915-938
                 finish = true;
@@ -234,7 +252,10 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
         block_reserve<kNrcTraceBlock, 2>({is_query, cont}, {d.ctl + kNrcCtlQueries, f.counters + bounce + 1},
                                          rl_out, o_idx);
         const uint32_t qi = o_idx[0], ci = o_idx[1];
-        if (is_query) {
+        // A training frame counts the cache vertices that found no record
+        // slot but stores no query: it runs no inference, and its pass may
+        // run beside a frame whose queries fill the same buffer.
+        if (is_query && !np.train_only) {
             store_query(d.queries + qi, in, pid);
             d.query_weight[qi] = make_float4(color.x, color.y, color.z, 0.f);
         }
@@ -277,11 +298,49 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_nrc_generate_train(CameraDev
     }
 }
 
-// End of a sample: its cache vertices are its queries plus the records it
-// stored (a record that found no room became a query); a training frame's
-// count is scaled to the whole image by its pixel fraction.  The count sets
-// the selection probability of the next samples.
-__global__ void k_nrc_end_sample(NrcDev d, uint32_t train_only) {
+// Start of a wide group of samples s0 .. s0 + w - 1: the bounce-0 queue
+// length (the bounce-0 slots are implicit, pixel * w + j) and the other
+// queues' counters, and with the trace's first group the sums of every path
+// id (pixel * stride + j, j < stride).  The path-vertex count is left to the
+// camera pass.
+__global__ __launch_bounds__(kNrcTraceBlock) void k_nrc_begin_wide(FrameDev f, uint32_t s0,
+                                                                   uint32_t w, uint32_t stride_w) {
+    const uint64_t gtid = (uint64_t)blockIdx.x * kNrcTraceBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kNrcTraceBlock;
+    if (gtid == 0) {
+        f.counters[0] = (uint32_t)((uint64_t)w * f.npix);
+        for (uint32_t i = 1; i <= kMaxBounces; ++i) f.counters[i] = 0;
+        for (uint32_t k = 0; k < 2 * kFetchSegments; ++k) f.counters[kCounterFetch + k * kFetchStride] = 0;
+    }
+    if (s0 != 0) return;
+    const uint64_t total = (uint64_t)stride_w * f.npix;
+    for (uint64_t pid = gtid; pid < total; pid += stride) f.pix_sum[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// The image of a wide trace: each pixel's w sums (path ids pixel * w + j)
+// over the trace's samples, clamped at zero as k_resolve does.
+__global__ __launch_bounds__(kNrcTraceBlock) void k_nrc_resolve_wide(FrameDev f, uint32_t samples,
+                                                                     uint32_t w) {
+    const uint64_t gtid = (uint64_t)blockIdx.x * kNrcTraceBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kNrcTraceBlock;
+    const float ns = (float)samples;
+    for (uint64_t pix = gtid; pix < f.npix; pix += stride) {
+        float4 t = f.pix_sum[pix * w];
+        for (uint32_t j = 1; j < w; ++j) {
+            const float4 u = f.pix_sum[pix * w + j];
+            t = make_float4(t.x + u.x, t.y + u.y, t.z + u.z, 0.f);
+        }
+        f.out_color[pix] = make_float4(fmaxf(t.x / ns, 0.f), fmaxf(t.y / ns, 0.f),
+                                       fmaxf(t.z / ns, 0.f), 1.f);
+    }
+}
+
+// End of a sample (of `per` samples traced together): its cache vertices are
+// its queries plus the records it stored (a record that found no room became
+// a query); a training frame's count is scaled to the whole image by its
+// pixel fraction.  The count per sample sets the selection probability of
+// the next training paths.
+__global__ void k_nrc_end_sample(NrcDev d, uint32_t train_only, uint32_t per) {
     const uint32_t rec = min(d.ctl[kNrcCtlRecords], d.rec_cap);
     const uint32_t hits = d.ctl[kNrcCtlQueries] + (rec - min(d.ctl[kNrcCtlRecordsBase], rec));
     float full = (float)hits;
@@ -289,6 +348,7 @@ __global__ void k_nrc_end_sample(NrcDev d, uint32_t train_only) {
         const float frac = __uint_as_float(d.ctl[kNrcCtlFrac]);
         full = frac > 0.0f ? fminf((float)hits / frac, 4.0e9f) : 0.0f;
     }
+    full /= (float)per;
     if (full >= 1.0f) d.ctl[kNrcCtlHitsPrev] = (uint32_t)full;
 }
 
@@ -296,7 +356,7 @@ template <uint32_t Q>
 static hipError_t launch_nrc_train(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
                                    const TraceParams& p, const NrcDev& d, NrcTraceParams np,
                                    hipStream_t stream, KernelTimer* tm) {
-    const dim3 grid((uint32_t)(frame_threads(f.npix) / kNrcTraceBlock)), block(kNrcTraceBlock);
+    const dim3 grid((uint32_t)(frame_threads(grid_paths(f)) / kNrcTraceBlock)), block(kNrcTraceBlock);
     // queue lengths, this trace's vertex count and both traversal fetch sets
     hipError_t e = launch_fill_zero(f.counters, (kMaxBounces + 1) * 4, stream);
     if (e == hipSuccess) e = launch_fill_zero(f.counters + kCounterVertices, 8, stream);
@@ -315,7 +375,7 @@ static hipError_t launch_nrc_train(const SceneDev& s, const CameraDev& cam, cons
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_nrc_end_sample, dim3(1), dim3(1), 0, stream, d, 1u);
+    hipLaunchKernelGGL(k_nrc_end_sample, dim3(1), dim3(1), 0, stream, d, 1u, 1u);
     return hipGetLastError();
 }
 
@@ -353,7 +413,7 @@ static hipError_t launch_nrc_q(const SceneDev& s, const CameraDev& cam, const Fr
             e = hipGetLastError();
             if (e != hipSuccess) return e;
         }
-        hipLaunchKernelGGL(k_nrc_end_sample, dim3(1), dim3(1), 0, stream, d, 0u);
+        hipLaunchKernelGGL(k_nrc_end_sample, dim3(1), dim3(1), 0, stream, d, 0u, 1u);
         if (p.bounces > kCacheBounce) {
             if (tm) tm->begin(kKernelCacheInfer, stream);
             e = nrc_launch_infer_accumulate(d, f.pix_sum, stream);
@@ -362,6 +422,82 @@ static hipError_t launch_nrc_q(const SceneDev& s, const CameraDev& cam, const Fr
         }
     }
     return launch_resolve(f, p, stream);
+}
+
+// A trace whose samples are traced w at a time (no training paths): the
+// camera rays once into the camera-ray hits (f_img, bounce 0), then per group
+// of w samples one set of bounces and one inference over all of them in fw
+// (state, queues, hits and sums of w * npix path ids), so the latency-bound
+// tail of each bounce (the deferred specular paths) is paid once per group
+// instead of once per sample.  The samples of a pixel have adjacent slots and
+// path ids: their camera ray, hit and surface data are read together, and
+// their secondary rays leave one point together.
+template <uint32_t Q>
+static hipError_t launch_nrc_wide_q(const SceneDev& s, const CameraDev& cam, const FrameDev& f_img,
+                                    const FrameDev& fw, const TraceParams& p, const NrcDev& d,
+                                    NrcTraceParams np, uint32_t w, hipStream_t stream,
+                                    KernelTimer* tm) {
+    const dim3 grid((uint32_t)(frame_threads(fw.npix) / kNrcTraceBlock)), block(kNrcTraceBlock);
+    const FrameDev f0 = primary_view(f_img, p);
+    hipError_t e = launch_generate(cam, f_img, p, 0, stream);
+    if (e != hipSuccess) return e;
+    if (tm) tm->begin(kKernelExtend, stream);
+    e = launch_extend(s, f0, p, 0, stream);
+    if (tm) tm->end(kKernelExtend, stream);
+    if (e != hipSuccess) return e;
+    FrameDev fw0 = fw;  // bounce 0: the camera rays and their hits, in pixel order
+    fw0.queue[0] = f0.queue[0];
+    fw0.hits = f0.hits;
+    const uint32_t bounces = std::min(p.bounces, kCacheBounce + np.defer_specular + 1u);
+    np.train_expect = 0.0f;
+    np.wide = w;
+    for (uint32_t s0 = 0; s0 < p.samples; s0 += w) {
+        const uint32_t wc = std::min(w, p.samples - s0);
+        hipLaunchKernelGGL(k_nrc_begin_wide, grid, block, 0, stream, fw, s0, wc, w);
+        np.sample = s0;
+        np.wide_count = wc;
+        for (uint32_t b = 0; b < bounces; ++b) {
+            const FrameDev& fb = b == 0 ? fw0 : fw;
+            if (b > 0) {
+                if (tm) tm->begin(kKernelExtend, stream);
+                e = launch_extend(s, fb, p, b, stream);
+                if (tm) tm->end(kKernelExtend, stream);
+                if (e != hipSuccess) return e;
+            }
+            if (tm) tm->begin(kKernelShade, stream);
+            hipLaunchKernelGGL(k_shade_nrc<Q>, grid, block, 0, stream, s, fb, p, d, np, b);
+            if (tm) tm->end(kKernelShade, stream);
+            e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_nrc_end_sample, dim3(1), dim3(1), 0, stream, d, 0u, wc);
+        if (p.bounces > kCacheBounce) {
+            if (tm) tm->begin(kKernelCacheInfer, stream);
+            e = nrc_launch_infer_accumulate(d, fw.pix_sum, stream);
+            if (tm) tm->end(kKernelCacheInfer, stream);
+            if (e != hipSuccess) return e;
+        }
+    }
+    hipLaunchKernelGGL(k_nrc_resolve_wide, grid, block, 0, stream, fw, p.samples,
+                       std::min(w, p.samples));
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_nrc_wide(const SceneDev& s, const CameraDev& cam, const FrameDev& f_img,
+                                 const FrameDev& fw, const TraceParams& p, const NrcDev& d,
+                                 NrcTraceParams np, uint32_t w, hipStream_t stream,
+                                 KernelTimer* timer) {
+    if (f_img.npix == 0) return hipSuccess;
+    switch (p.quality) {
+        case kQualityLow:
+            return launch_nrc_wide_q<kQualityLow>(s, cam, f_img, fw, p, d, np, w, stream, timer);
+        case kQualityHigh:
+            return launch_nrc_wide_q<kQualityHigh>(s, cam, f_img, fw, p, d, np, w, stream, timer);
+        case kQualitySpectral:
+            return launch_nrc_wide_q<kQualitySpectral>(s, cam, f_img, fw, p, d, np, w, stream, timer);
+        default:
+            return launch_nrc_wide_q<kQualityMedium>(s, cam, f_img, fw, p, d, np, w, stream, timer);
+    }
 }
 
 hipError_t launch_trace_nrc(const SceneDev& s, const CameraDev& cam, const FrameDev& f,

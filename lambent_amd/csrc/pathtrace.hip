@@ -272,7 +272,7 @@ FrameDev primary_view(const FrameDev& f, const TraceParams& p) {
 
 hipError_t launch_extend(const SceneDev& s, const FrameDev& f, const TraceParams& p,
                          uint32_t bounce, hipStream_t stream) {
-    const dim3 grid((uint32_t)(extend_threads(f.npix) / kPtBlock));
+    const dim3 grid((uint32_t)(extend_threads(grid_paths(f)) / kPtBlock));
     launch_extend_kernel(s, f, p, bounce, grid, stream);
     return hipGetLastError();
 }

@@ -138,7 +138,8 @@ def _rel_mse(img, ref):
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
 
 
-def test_cached_tracer_learns_the_indirect_light():
+@pytest.mark.parametrize("overlap", [True, False])
+def test_cached_tracer_learns_the_indirect_light(overlap):
     spec = scenes.cornell_box(64, 64)
     scene = spec.build()
     cam = spec.camera()
@@ -159,7 +160,8 @@ def test_cached_tracer_learns_the_indirect_light():
                 acc += bufs.output(api.Output.COLOR)[..., :3]
         return acc / (frames - skip)
 
-    cache = api.NeuralCache(api.NeuralCacheOpts(train_paths=1024, train_steps=4, seed=3))
+    cache = api.NeuralCache(api.NeuralCacheOpts(train_paths=1024, train_steps=4, seed=3,
+                                                overlap_training=overlap))
     img = run(cache, 300, 100)
     st = cache.stats()
     assert st["steps"] == 300 * 4
@@ -220,15 +222,18 @@ def test_set_opts_freezes_training_in_the_tracer():
         cache.set_opts(api.NeuralCacheOpts(train_steps=0))
 
 
-def test_cached_tracer_several_samples_per_trace():
-    """Training paths come from the first sample only; later samples reuse
-    its camera-ray hits and end at their cache vertices."""
+@pytest.mark.parametrize("overlap", [True, False])
+def test_cached_tracer_several_samples_per_trace(overlap):
+    """Training paths come from the first sample only (or from the training
+    pass beside the samples); later samples reuse its camera-ray hits and end
+    at their cache vertices."""
     spec = scenes.cornell_box(48, 48)
     scene = spec.build()
     pipe = api.RayTracer(api.Medium).create_pipeline(
         api.PathTracerOpts(samples=4, bounces=8, rr_start=3, guiding=api.Guiding.OFF))
     bufs = api.DataBuffers(48, 48, 1)
-    cache = api.NeuralCache(api.NeuralCacheOpts(train_paths=512, train_steps=2, seed=4))
+    cache = api.NeuralCache(api.NeuralCacheOpts(train_paths=512, train_steps=2, seed=4,
+                                                overlap_training=overlap))
     for f in range(40):
         pipe.dispatch_cached(scene, spec.camera(), bufs, cache, f)
     st = cache.stats()
@@ -283,3 +288,79 @@ def test_training_frames_train_without_an_image():
     assert abs(img.mean() - ref.mean()) / ref.mean() < 0.06, (img.mean(), ref.mean())
     with pytest.raises(api.LambentError):
         pipe.train_cached(scene, cam, bufs, api.NeuralCache(frozen), 0)
+
+
+def _frozen_render(pipe, scene, cam, w, h, weights, seed):
+    frozen = api.NeuralCache(api.NeuralCacheOpts(train_paths=0))
+    frozen.set_weights(weights)
+    fb = api.DataBuffers(w, h, 1)
+    pipe.dispatch_cached(scene, cam, fb, frozen, seed)
+    return fb.output(api.Output.COLOR), fb
+
+
+def _same_render(a, b):
+    """Images of the same paths traced one sample at a time and several
+    samples at a time: equal but for the order of the per-pixel float sums."""
+    np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("samples", [3, 11])
+def test_overlapped_training_renders_with_the_previous_frames_weights(samples):
+    """With overlap_training the samples of frame k read the inference weights
+    published by frame k - 1's last optimiser step while frame k's own
+    training pass and steps run on the cache's stream: frame k's image equals
+    a frozen cache's render with the weights after frame k - 1.  The frozen
+    cache traces one sample at a time; the overlapped frame traces up to 8
+    together (11 samples: a group of 8 and one of 3), so the images agree to
+    the rounding of the per-pixel sums."""
+    spec = scenes.cornell_box(48, 40)
+    scene = spec.build()
+    cam = spec.camera()
+    pipe = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=samples, bounces=8, rr_start=3, guiding=api.Guiding.OFF))
+    bufs = api.DataBuffers(48, 40, 1)
+    cache = api.NeuralCache(api.NeuralCacheOpts(train_paths=512, train_steps=3, seed=6,
+                                                learning_rate=3e-3))
+    for f in range(4):
+        pipe.dispatch_cached(scene, cam, bufs, cache, f)
+    w = cache.weights()
+    pipe.dispatch_cached(scene, cam, bufs, cache, 4)
+    img = bufs.output(api.Output.COLOR)
+    st = cache.stats()
+    assert st["steps"] == 15
+    assert 0.6 * 512 < st["records"] < 1.4 * 512, st
+    assert np.abs(cache.weights() - w).max() > 0  # frame 4 trained on
+    assert np.isfinite(img).all() and img[..., :3].mean() > 0
+    frozen, fb = _frozen_render(pipe, scene, cam, 48, 40, w, 4)
+    _same_render(img, frozen)
+    np.testing.assert_array_equal(bufs.output(api.Output.NORMAL), fb.output(api.Output.NORMAL))
+    np.testing.assert_array_equal(bufs.output(api.Output.ALBEDO), fb.output(api.Output.ALBEDO))
+    assert bufs.path_vertices() == fb.path_vertices()
+
+
+def test_switching_training_modes_keeps_inference_current():
+    """Host inference after overlapped frames, synchronous frames between
+    overlapped ones, and new weights set between overlapped frames all see
+    the current weights."""
+    spec = scenes.cornell_box(40, 32)
+    scene = spec.build()
+    cam = spec.camera()
+    pipe = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=2, bounces=8, rr_start=3, guiding=api.Guiding.OFF))
+    bufs = api.DataBuffers(40, 32, 1)
+    o = api.NeuralCacheOpts(train_paths=256, train_steps=2, seed=7, learning_rate=3e-3)
+    cache = api.NeuralCache(o)
+    for f in range(3):
+        pipe.dispatch_cached(scene, cam, bufs, cache, f)
+    x = np.random.default_rng(21).random((256, 13)).astype(np.float32)
+    ref = oracle.nrc_forward(cache.weights().astype(np.float64), x, oracle.NRC_BF16)
+    assert np.abs(cache.infer(x) - ref).max() < 2e-2 * max(1.0, np.abs(ref).max())
+    cache.set_opts(api.NeuralCacheOpts(**{**o.__dict__, "overlap_training": False}))
+    pipe.dispatch_cached(scene, cam, bufs, cache, 3)
+    cache.set_opts(o)
+    pipe.dispatch_cached(scene, cam, bufs, cache, 4)
+    assert cache.stats()["steps"] == 10
+    w = he_weights(31)
+    cache.set_weights(w)
+    pipe.dispatch_cached(scene, cam, bufs, cache, 5)
+    _same_render(bufs.output(api.Output.COLOR), _frozen_render(pipe, scene, cam, 40, 32, w, 5)[0])
