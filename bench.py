@@ -81,18 +81,20 @@ def rel_mse(img, ref) -> float:
 
 
 def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
-              lr: float = 1e-3, spp: int = 8, steps: int = 32, overlap: bool = True) -> dict:
+              lr: float = 1e-3, spp: int = 8, steps: int = 32, overlap: bool = True,
+              warm_paths: int = 524288, warm_steps: int = 256) -> dict:
     """Time-to-relMSE of the cached tracer against the plain tracer, both with
     8-bounce paths (RR from bounce 3), judged against a plain reference of
     ref_spp samples accumulated in fp64.  The cache starts untrained: `warm`
-    training frames (lmb_trace_nrc_train: about 65536 paths at random pixels
-    train it, no image; timed), then `frames` frames of `spp`
-    samples (the camera rays are traced once per frame, training paths come
-    from the first sample), `steps` Adam steps per frame over the frame's
-    65536 training records (overlap: the training pass and the steps run on
-    the cache's own stream beside the frame's samples).  Times are wall times
-    of the dispatches (host read-back of the images for scoring is not
-    counted)."""
+    training frames (lmb_trace_nrc_train: about warm_paths paths at random
+    pixels train it with warm_steps Adam steps, no image; timed), then
+    `frames` frames of `spp` samples (the camera rays are traced once per
+    frame), `steps` Adam steps per frame over the frame's 65536 training
+    records (overlap: a training pass of its own and the steps run on the
+    cache's own stream beside the frame's samples, which are traced eight at
+    a time; otherwise the training paths come from the first sample).  Times
+    are wall times of the dispatches (host read-back of the images for
+    scoring is not counted)."""
     import numpy as np
     W, H = spec.width, spec.height
     popts = dict(bounces=8, rr_start=3, guiding=api.Guiding.OFF,
@@ -121,8 +123,9 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
         plain_pts.append({"spp": n_spp, "seconds": dt, "relmse": rel_mse(img, ref)})
         log(f"cache leg: plain {n_spp} spp {dt * 1e3:.1f} ms relMSE {plain_pts[-1]['relmse']:.5f}")
 
-    cache = api.NeuralCache(api.NeuralCacheOpts(learning_rate=lr, train_steps=steps,
-                                                overlap_training=overlap))
+    frame_opts = api.NeuralCacheOpts(learning_rate=lr, train_steps=steps, overlap_training=overlap)
+    cache = api.NeuralCache(api.NeuralCacheOpts(**{**frame_opts.__dict__, "train_paths": warm_paths,
+                                                   "train_steps": warm_steps}))
     warm_pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=1, **popts))
     pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=spp, **popts))
     b = api.DataBuffers(W, H, 1, api.BufferType(0))
@@ -134,6 +137,7 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
         api.sync()
         t += time.perf_counter() - t0
     t_warm = t
+    cache.set_opts(frame_opts)
     acc = np.zeros((H, W, 3))
     checks = set(range(1, 65, 2)) | set(range(64, frames + 1, 8)) | {frames}
     pts = []
@@ -175,7 +179,8 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
                     f"vertex (a specular scatter moves it one vertex on), 65536 training paths "
                     f"and {steps} Adam steps (lr {lr:g}) per frame"
                     f"{' on a second stream beside the samples' if overlap else ''}; {warm} "
-                    f"training frames (no image, timed), then frames of {spp} spp",
+                    f"training frames of {warm_paths} paths and {warm_steps} steps (no image, "
+                    f"timed), then frames of {spp} spp",
         "reference": f"plain tracer {ref_spp} spp, fp64 accumulation of {chunk}-spp dispatches",
         "plain": plain_pts,
         "cache_points": pts,
@@ -351,10 +356,12 @@ def main() -> int:
     ap.add_argument("--batch-frames", type=int, default=3)
     ap.add_argument("--ref-spp", type=int, default=4096)
     ap.add_argument("--cache-frames", type=int, default=96)
-    ap.add_argument("--cache-warmup", type=int, default=32)
+    ap.add_argument("--cache-warmup", type=int, default=4)
     ap.add_argument("--cache-lr", type=float, default=1e-3)
     ap.add_argument("--cache-spp", type=int, default=8)
     ap.add_argument("--cache-steps", type=int, default=32)
+    ap.add_argument("--cache-warm-paths", type=int, default=524288)
+    ap.add_argument("--cache-warm-steps", type=int, default=256)
     ap.add_argument("--cache-no-overlap", action="store_true",
                     help="cache training on the frame's stream after its samples")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -459,7 +466,8 @@ def main() -> int:
     if not args.no_cache_leg:
         out["time_to_relmse"] = cache_leg(api, scene, spec, cam, args.ref_spp, args.cache_frames,
                                           args.cache_warmup, args.cache_lr, args.cache_spp,
-                                          args.cache_steps, not args.cache_no_overlap)
+                                          args.cache_steps, not args.cache_no_overlap,
+                                          args.cache_warm_paths, args.cache_warm_steps)
     if not args.no_spectral_leg:
         del scene
         out["spectral_restir"] = spectral_leg(api, scenes, args.triangles, args.width,

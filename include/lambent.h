@@ -431,7 +431,7 @@ typedef struct lmb_nrc_opts {
     float beta1, beta2;    /* Adam moment decays                    (0.9, 0.99) */
     float epsilon;         /* Adam denominator offset               (1e-8)  */
     uint32_t train_paths;  /* training paths per lmb_trace_nrc      (65536) */
-    uint32_t train_steps;  /* optimiser steps per lmb_trace_nrc     (4)     */
+    uint32_t train_steps;  /* optimiser steps per lmb_trace_nrc, 1..256 (4) */
     uint32_t seed;         /* weight initialisation (He-uniform)            */
     float ema_decay;       /* inference uses an exponential moving average
                               of the weights with this decay per step; 0 = the

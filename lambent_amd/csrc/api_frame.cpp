@@ -3,6 +3,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -509,8 +510,12 @@ static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera
     const uint64_t npix = (uint64_t)bufs->width * bufs->height;
     lmb_status st = nrc_ensure_queries(nrc, npix);
     if (st) return st;
+    // Steps of ceil(records / steps) samples, records as the current options
+    // expect them (the record buffers may be larger from earlier options).
     const uint32_t steps = nrc->opts.train_steps;
-    st = nrc_ensure_batch(nrc, (nrc->dev.rec_cap + steps - 1) / steps);
+    const uint64_t expect = std::max<uint64_t>(kNrcChunk, (uint64_t)nrc->opts.train_paths * 5 / 4);
+    const uint64_t rec = std::min<uint64_t>(nrc->dev.rec_cap, expect);
+    st = nrc_ensure_batch(nrc, (uint32_t)((rec + steps - 1) / steps));
     if (st) return st;
     NrcTraceParams np{};
     for (int k = 0; k < 3; ++k) {
@@ -564,6 +569,9 @@ static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera
     NrcDev db = nrc->dev;
     db.ctl = nrc->ctl_train.as<uint32_t>();
     db.fwd_pub = nullptr;
+    const char* prio_env = std::getenv("LMB_TRAIN_PRIO");
+    const uint32_t prio = prio_env ? (uint32_t)std::atoi(prio_env) : 0u;
+    db.wave_prio = prio;
     // The training pass's pixel fraction from the cache vertices per sample
     // counted so far on the caller's stream.
     e = hipMemcpyAsync(db.ctl + kNrcCtlHitsPrev, da.ctl + kNrcCtlHitsPrev, 4,
@@ -610,6 +618,7 @@ static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera
     ft.counters = nrc->tf_counters.as<uint32_t>();
     ft.spill = nrc->tf_spill.ptr;
     ft.grid_paths = std::max<uint64_t>(nrc->opts.train_paths, 256);
+    ft.wave_prio = prio;
     TraceParams pt = p;
     pt.samples = 1;
     NrcTraceParams npt = np;

@@ -100,7 +100,7 @@ lmb_status nrc_run_steps(lmb_nrc n, const NrcDev& d, uint16_t* publish, const Nr
                          const float4* target, const uint32_t* count, uint32_t count_limit,
                          uint32_t steps, bool apply, float* grad_out, hipStream_t stream,
                          KernelTimer* timer) {
-    hipError_t e = launch_fill_zero(d.loss, 64 * sizeof(float), stream);
+    hipError_t e = launch_fill_zero(d.loss, kNrcMaxSteps * sizeof(float), stream);
     if (e != hipSuccess) return fail_hip(e, "loss clear");
     for (uint32_t s = 0; s < steps; ++s) {
         const AdamParams a = apply ? next_adam(n) : AdamParams{};
@@ -234,12 +234,12 @@ uint32_t* host_count(lmb_nrc n, uint32_t count) {
 }
 
 float read_loss(lmb_nrc n, uint64_t count, uint32_t steps) {
-    float sums[64] = {0};
-    if (hipMemcpy(sums, n->dev.loss, sizeof(float) * std::min<uint32_t>(steps, 64),
-                  hipMemcpyDeviceToHost) != hipSuccess)
+    float sums[kNrcMaxSteps] = {0};
+    const uint32_t k = std::min<uint32_t>(steps, kNrcMaxSteps);
+    if (hipMemcpy(sums, n->dev.loss, sizeof(float) * k, hipMemcpyDeviceToHost) != hipSuccess)
         return NAN;
     double tot = 0.0;
-    for (uint32_t s = 0; s < std::min<uint32_t>(steps, 64); ++s) tot += sums[s];
+    for (uint32_t s = 0; s < k; ++s) tot += sums[s];
     return count ? (float)(tot / (3.0 * (double)count)) : 0.0f;
 }
 
@@ -299,8 +299,8 @@ static lmb_status validate(const lmb_nrc_opts& o) {
     if (!(o.learning_rate > 0.0f) || !(o.beta1 >= 0.0f && o.beta1 < 1.0f) ||
         !(o.beta2 >= 0.0f && o.beta2 < 1.0f) || !(o.epsilon >= 0.0f))
         return fail(LMB_ERR_INVALID_ARGUMENT, "invalid Adam parameters");
-    if (o.train_steps == 0 || o.train_steps > 64)
-        return fail(LMB_ERR_INVALID_ARGUMENT, "train_steps must be in 1..64");
+    if (o.train_steps == 0 || o.train_steps > kNrcMaxSteps)
+        return fail(LMB_ERR_INVALID_ARGUMENT, "train_steps must be in 1..256");
     if (o.train_paths > (1u << 24))
         return fail(LMB_ERR_INVALID_ARGUMENT, "train_paths must be <= 2^24");
     if (!(o.ema_decay >= 0.0f && o.ema_decay < 1.0f))
@@ -382,7 +382,7 @@ lmb_status lmb_nrc_create(const lmb_nrc_opts* opts, lmb_nrc* out) {
     if (!st) st = n->fwd.alloc(kNrcFwdFrags * kFragElems * 2);
     if (!st) st = n->bwd.alloc(kNrcBwdFrags * kFragElems * 2);
     if (!st) st = n->ctl.alloc(kNrcCtlWords * 4);
-    if (!st) st = n->loss.alloc(64 * sizeof(float));
+    if (!st) st = n->loss.alloc(kNrcMaxSteps * sizeof(float));
     if (!st) st = ensure_records(n, o.train_paths);
     if (st) {
         delete n;

@@ -51,6 +51,9 @@ constexpr int kNrcFwdFragHidden = 40;
 constexpr int kNrcFwdFragOut = 72;
 constexpr int kNrcBwdFragOut = 32;
 
+// Most optimiser steps of one call (one loss sum per step).
+constexpr uint32_t kNrcMaxSteps = 256;
+
 // Samples of one training column chunk of the weight-gradient kernel.
 constexpr int kNrcChunk = 256;
 // Output tiles (32 x 32) of the weight-gradient kernel: layer 0 10 x 2,
@@ -143,8 +146,9 @@ struct NrcDev {
     uint16_t* g;
     float* partial;  // [batch / kNrcChunk][kNrcNumWeights]
     uint32_t batch;  // samples per optimiser step (multiple of kNrcChunk)
+    uint32_t wave_prio;  // raised instruction-issue priority of the training kernels' waves
     uint32_t* ctl;
-    float* loss;  // per optimiser step: sum of (y - t)^2 / (y^2 + 0.01)
+    float* loss;  // per optimiser step (kNrcMaxSteps): sum of (y - t)^2 / (y^2 + 0.01)
     // queries of the tracer (one per path that reaches the cache vertex)
     NrcQuery* queries;
     float4* query_weight;  // path throughput at the cache vertex
