@@ -3,7 +3,6 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
-#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -569,9 +568,6 @@ static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera
     NrcDev db = nrc->dev;
     db.ctl = nrc->ctl_train.as<uint32_t>();
     db.fwd_pub = nullptr;
-    const char* prio_env = std::getenv("LMB_TRAIN_PRIO");
-    const uint32_t prio = prio_env ? (uint32_t)std::atoi(prio_env) : 0u;
-    db.wave_prio = prio;
     // The training pass's pixel fraction from the cache vertices per sample
     // counted so far on the caller's stream.
     e = hipMemcpyAsync(db.ctl + kNrcCtlHitsPrev, da.ctl + kNrcCtlHitsPrev, 4,
@@ -618,7 +614,6 @@ static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera
     ft.counters = nrc->tf_counters.as<uint32_t>();
     ft.spill = nrc->tf_spill.ptr;
     ft.grid_paths = std::max<uint64_t>(nrc->opts.train_paths, 256);
-    ft.wave_prio = prio;
     TraceParams pt = p;
     pt.samples = 1;
     NrcTraceParams npt = np;

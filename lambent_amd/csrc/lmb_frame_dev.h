@@ -65,15 +65,7 @@ struct FrameDev {
     // Paths that size the persistent grids of this frame's launches (0: npix);
     // a pass over a small subset of the pixels runs on a smaller grid.
     uint64_t grid_paths;
-    // Raised instruction-issue priority of this frame's waves (s_setprio):
-    // a small pass beside a large one on the same CUs.
-    uint32_t wave_prio;
 };
-
-// The waves of a kernel launched for a raised-priority pass.
-__device__ __forceinline__ void raise_wave_prio(uint32_t prio) {
-    if (prio) __builtin_amdgcn_s_setprio(2);
-}
 
 inline uint64_t grid_paths(const FrameDev& f) { return f.grid_paths ? f.grid_paths : f.npix; }
 

@@ -146,7 +146,6 @@ struct NrcDev {
     uint16_t* g;
     float* partial;  // [batch / kNrcChunk][kNrcNumWeights]
     uint32_t batch;  // samples per optimiser step (multiple of kNrcChunk)
-    uint32_t wave_prio;  // raised instruction-issue priority of the training kernels' waves
     uint32_t* ctl;
     float* loss;  // per optimiser step (kNrcMaxSteps): sum of (y - t)^2 / (y^2 + 0.01)
     // queries of the tracer (one per path that reaches the cache vertex)

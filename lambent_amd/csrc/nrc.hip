@@ -28,10 +28,6 @@ typedef float f32x16 __attribute__((__vector_size__(16 * sizeof(float))));
 
 constexpr int kNrcBlock = 256;  // 4 waves
 
-#ifndef LMB_NRC_TRAIN_LDS
-#define LMB_NRC_TRAIN_LDS 1
-#endif
-
 template <class Fn, int... I>
 __device__ __forceinline__ void static_for_impl(Fn&& fn, std::integer_sequence<int, I...>) {
     (fn(std::integral_constant<int, I>{}), ...);
@@ -170,7 +166,6 @@ __device__ __forceinline__ void load_inputs(const NrcQuery* __restrict__ q, uint
 }
 
 __global__ __launch_bounds__(kNrcBlock) void k_nrc_pack(NrcDev d) {
-    raise_wave_prio(d.wave_prio);
     const int i = blockIdx.x * kNrcBlock + threadIdx.x;
     if (i >= (kNrcFwdFrags + kNrcBwdFrags) * kFragElems) return;
     const int frag = i / kFragElems, lane = (i % kFragElems) / 8, e = i % 8;
@@ -271,8 +266,6 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_fwd_bwd(NrcDev d, const NrcQu
                                                            const uint32_t* __restrict__ count_ptr,
                                                            uint32_t count_limit, uint32_t steps,
                                                            uint32_t step) {
-    raise_wave_prio(d.wave_prio);
-#if LMB_NRC_TRAIN_LDS
     __shared__ bf16x8 lds[(kNrcFwdFrags + kNrcBwdFrags) * 64];
     {
         const bf16x8* sf = reinterpret_cast<const bf16x8*>(d.fwd);
@@ -283,9 +276,6 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_fwd_bwd(NrcDev d, const NrcQu
     }
     __syncthreads();
     const FragPtr fr{lds}, br{lds + kNrcFwdFrags * 64};
-#else
-    const FragPtr fr{reinterpret_cast<const bf16x8*>(d.fwd)}, br{reinterpret_cast<const bf16x8*>(d.bwd)};
-#endif
     const uint32_t S = d.batch;
     const uint32_t count = min(*count_ptr, count_limit);
     const uint32_t nvalid = count > step ? min(S, (count - step + steps - 1u) / steps) : 0u;
@@ -365,7 +355,6 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_fwd_bwd(NrcDev d, const NrcQu
 // kNrcChunk samples: K = samples, both operands loaded straight from the
 // feature-major scratch (16 contiguous bytes per lane).
 __global__ __launch_bounds__(kNrcBlock) void k_nrc_dw(NrcDev d) {
-    raise_wave_prio(d.wave_prio);
     const uint32_t wave = (blockIdx.x * kNrcBlock + threadIdx.x) >> 6;
     const uint32_t chunks = d.batch / kNrcChunk;
     if (wave >= chunks * kNrcGradTiles) return;
@@ -422,7 +411,6 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_adam(NrcDev d, AdamParams p,
                                                         uint32_t step, int apply,
                                                         float* __restrict__ grad_out,
                                                         const float* __restrict__ grad_in) {
-    raise_wave_prio(d.wave_prio);
     const int w = blockIdx.x * kNrcBlock + threadIdx.x;
     if (w >= kNrcNumWeights) return;
     float g;

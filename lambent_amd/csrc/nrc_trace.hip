@@ -74,7 +74,6 @@ template <uint32_t Q>
 __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameDev f, TraceParams p,
                                                               NrcDev d, NrcTraceParams np,
                                                               uint32_t bounce) {
-    raise_wave_prio(f.wave_prio);
     const uint64_t gtid = (uint64_t)blockIdx.x * kNrcTraceBlock + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * kNrcTraceBlock;
     const uint32_t count = f.counters[bounce];
@@ -274,7 +273,6 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
 __global__ __launch_bounds__(kNrcTraceBlock) void k_nrc_generate_train(CameraDev cam, FrameDev f,
                                                                        TraceParams p, NrcDev d,
                                                                        NrcTraceParams np) {
-    raise_wave_prio(f.wave_prio);
     __shared__ ReserveLds<kNrcTraceBlock, 1> rl;
     const uint64_t stride = (uint64_t)gridDim.x * kNrcTraceBlock;
     // before any count: as if every pixel reached a cache vertex

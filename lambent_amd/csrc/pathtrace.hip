@@ -132,7 +132,6 @@ constexpr int kRefill = LMB_REFILL;  // idle lanes that make a wave fetch more r
 template <bool kAlpha>
 __global__ __launch_bounds__(kPtBlock, LMB_EXTEND_WAVES) void k_extend(SceneDev s, FrameDev f, TraceParams p,
                                                      uint32_t bounce) {
-    raise_wave_prio(f.wave_prio);
     __shared__ uint2 lds_stack[kStackLds * kPtBlock];
     const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
     const uint32_t count = f.counters[bounce];
