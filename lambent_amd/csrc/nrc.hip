@@ -182,21 +182,31 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_pack(NrcDev d) {
 
 // Inference.  kAccumulate: pix_sum[pid] += weight * max(y, 0) for the
 // queries counted in *count; otherwise out[i] = y for i < count.
+// Inference blocks: 8 waves share one LDS copy of the forward fragments,
+// one block per CU (2 waves per SIMD at 164 VGPRs).
+#ifndef LMB_NRC_INFER_WAVES
+#define LMB_NRC_INFER_WAVES 8
+#endif
+constexpr int kNrcInferWaves = LMB_NRC_INFER_WAVES;
+constexpr int kNrcInferBlock = 64 * kNrcInferWaves;
+constexpr int kNrcInferBlocksPerCu = kNrcInferWaves > 4 ? 1 : 2;
+
 template <bool kAccumulate>
-__global__ __launch_bounds__(kNrcBlock) void k_nrc_infer(NrcDev d, const NrcQuery* __restrict__ q,
+__global__ __launch_bounds__(kNrcInferBlock) void k_nrc_infer(NrcDev d, const NrcQuery* __restrict__ q,
                                                          const float4* __restrict__ weight,
                                                          const uint32_t* __restrict__ count_ptr,
                                                          uint32_t count_fixed,
                                                          float4* __restrict__ dst) {
     __shared__ bf16x8 lds[kNrcFwdFrags * 64];
     const bf16x8* src = reinterpret_cast<const bf16x8*>(d.fwd_inf);
-    for (int i = threadIdx.x; i < kNrcFwdFrags * 64; i += kNrcBlock) lds[i] = src[i];
+    for (int i = threadIdx.x; i < kNrcFwdFrags * 64; i += kNrcInferBlock) lds[i] = src[i];
     __syncthreads();
     const uint32_t count = count_ptr ? *count_ptr : count_fixed;
     const int lane = threadIdx.x & 63, n = lane & 31, h = lane >> 5;
     const uint32_t tiles = (count + 63u) / 64u;
     const FragPtr fr{lds};
-    for (uint32_t tile = blockIdx.x * 4u + (threadIdx.x >> 6); tile < tiles; tile += gridDim.x * 4u) {
+    for (uint32_t tile = blockIdx.x * (uint32_t)kNrcInferWaves + (threadIdx.x >> 6); tile < tiles;
+         tile += gridDim.x * (uint32_t)kNrcInferWaves) {
         float x[2][kNrcInputs];
         uint32_t idx[2];
 #pragma unroll
@@ -443,21 +453,21 @@ hipError_t nrc_launch_pack(const NrcDev& d, hipStream_t stream) {
 }
 
 static uint32_t infer_blocks(uint64_t n) {
-    const uint64_t want = (n + 4 * 64 - 1) / (4 * 64);
-    const uint64_t cap = (uint64_t)device_cu_count() * 2;
+    const uint64_t want = (n + kNrcInferWaves * 64 - 1) / (kNrcInferWaves * 64);
+    const uint64_t cap = (uint64_t)device_cu_count() * kNrcInferBlocksPerCu;
     return (uint32_t)std::max<uint64_t>(1, std::min(want, cap));
 }
 
 hipError_t nrc_launch_infer(const NrcDev& d, const NrcQuery* q, uint32_t n, float4* out,
                             hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_nrc_infer<false>, dim3(infer_blocks(n)), dim3(kNrcBlock), 0, stream, d, q,
+    hipLaunchKernelGGL(k_nrc_infer<false>, dim3(infer_blocks(n)), dim3(kNrcInferBlock), 0, stream, d, q,
                        (const float4*)nullptr, (const uint32_t*)nullptr, n, out);
     return hipGetLastError();
 }
 
 hipError_t nrc_launch_infer_accumulate(const NrcDev& d, float4* pix_sum, hipStream_t stream) {
-    hipLaunchKernelGGL(k_nrc_infer<true>, dim3(infer_blocks(d.query_cap)), dim3(kNrcBlock), 0,
+    hipLaunchKernelGGL(k_nrc_infer<true>, dim3(infer_blocks(d.query_cap)), dim3(kNrcInferBlock), 0,
                        stream, d, (const NrcQuery*)d.queries, (const float4*)d.query_weight,
                        (const uint32_t*)(d.ctl + kNrcCtlQueries), 0u, pix_sum);
     return hipGetLastError();
