@@ -69,6 +69,19 @@ struct FrameDev {
 
 inline uint64_t grid_paths(const FrameDev& f) { return f.grid_paths ? f.grid_paths : f.npix; }
 
+// Pixel of camera-ray queue slot `slot`: bands of 8 rows (the last band the
+// rows left), each band column by column, so a wave's 64 rays cover an 8 x 8
+// block of pixels (more coherent traversal than a 64 x 1 strip).
+__host__ __device__ inline void camera_slot_pixel(uint64_t slot, uint32_t width, uint32_t height,
+                                                  uint32_t& x, uint32_t& y) {
+    const uint64_t band_px = 8ull * width;
+    const uint32_t band = (uint32_t)(slot / band_px);
+    const uint32_t r = (uint32_t)(slot - (uint64_t)band * band_px);
+    const uint32_t rows = height - 8u * band < 8u ? height - 8u * band : 8u;
+    x = r / rows;
+    y = 8u * band + r % rows;
+}
+
 struct TraceParams {
     uint32_t quality;
     uint32_t samples;

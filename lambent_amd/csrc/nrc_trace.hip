@@ -107,16 +107,17 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
         V3 color = v3s(0.f), scol = v3s(0.f);
         float picked = -1.0f;
         if (active && np.wide > 1u && bounce == 0) {
-            // Wide group: bounce-0 slot pixel * wide_count + j is sample
-            // sample + j of the pixel, path id pixel * wide + j; the camera
-            // ray and its hit are the pixel's (qin and hits in pixel order),
-            // the path starts fresh with k_generate's seed for that sample.
-            const uint32_t pixel = (uint32_t)slot / np.wide_count;
-            const uint32_t j = (uint32_t)slot - pixel * np.wide_count;
+            // Wide group: bounce-0 slot c * wide_count + j is sample
+            // sample + j of the pixel of camera-ray slot c (qin and hits are
+            // the camera pass's), path id pixel * wide + j; the path starts
+            // fresh with k_generate's seed for that sample.
+            const uint32_t c = (uint32_t)slot / np.wide_count;
+            const uint32_t j = (uint32_t)slot - c * np.wide_count;
+            o4 = qin[2 * (uint64_t)c];
+            d4 = qin[2 * (uint64_t)c + 1];
+            h = f.hits[c];
+            const uint32_t pixel = __float_as_uint(o4.w);
             pid = pixel * np.wide + j;
-            o4 = qin[2 * (uint64_t)pixel];
-            d4 = qin[2 * (uint64_t)pixel + 1];
-            h = f.hits[pixel];
             const uint32_t x = pixel % f.width, y = pixel / f.width;
             seed = rand_u32(x * y) + p.seed_offset + y;  // general.wesl:516, 544
             for (uint32_t k = 0; k < np.sample + j; ++k) seed = rand_u32(seed);
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_nrc_generate_train(CameraDev
 }
 
 // Start of a wide group of samples s0 .. s0 + w - 1: the bounce-0 queue
-// length (the bounce-0 slots are implicit, pixel * w + j) and the other
+// length (the bounce-0 slots are implicit, camera slot * w + j) and the other
 // queues' counters, and with the trace's first group the sums of every path
 // id (pixel * stride + j, j < stride).  The path-vertex count is left to the
 // camera pass.
@@ -445,7 +446,7 @@ static hipError_t launch_nrc_wide_q(const SceneDev& s, const CameraDev& cam, con
     e = launch_extend(s, f0, p, 0, stream);
     if (tm) tm->end(kKernelExtend, stream);
     if (e != hipSuccess) return e;
-    FrameDev fw0 = fw;  // bounce 0: the camera rays and their hits, in pixel order
+    FrameDev fw0 = fw;  // bounce 0: the camera rays and their hits, by camera slot
     fw0.queue[0] = f0.queue[0];
     fw0.hits = f0.hits;
     const uint32_t bounces = std::min(p.bounces, kCacheBounce + np.defer_specular + 1u);

@@ -62,7 +62,9 @@ __device__ __forceinline__ uint64_t* vertex_counter(const FrameDev& f) {
 }
 
 // general.wesl:516: own_seed = rand_u32(x * y) + seed_offset + y; later
-// samples advance it with rand_u32 (general.wesl:544).
+// samples advance it with rand_u32 (general.wesl:544).  The camera rays go
+// into the bounce-0 queue in camera_slot_pixel order (8 x 8 pixel blocks);
+// every later sample of a trace uses the same slots.
 __global__ __launch_bounds__(kPtBlock) void k_generate(CameraDev cam, FrameDev f, TraceParams p,
                                                        uint32_t sample) {
     const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
@@ -73,8 +75,10 @@ __global__ __launch_bounds__(kPtBlock) void k_generate(CameraDev cam, FrameDev f
         for (uint32_t k = 0; k < 2 * kFetchSegments; ++k) f.counters[kCounterFetch + k * kFetchStride] = 0;
         if (sample == 0) *vertex_counter(f) = 0;
     }
-    for (uint64_t pix = gtid; pix < f.npix; pix += stride) {
-        const uint32_t x = (uint32_t)(pix % f.width), y = (uint32_t)(pix / f.width);
+    for (uint64_t slot = gtid; slot < f.npix; slot += stride) {
+        uint32_t x, y;
+        camera_slot_pixel(slot, f.width, f.height, x, y);
+        const uint64_t pix = (uint64_t)y * f.width + x;
         uint32_t seed;
         if (sample == 0) {
             seed = rand_u32(x * y) + p.seed_offset + y;
@@ -85,8 +89,8 @@ __global__ __launch_bounds__(kPtBlock) void k_generate(CameraDev cam, FrameDev f
         f.own_seed[pix] = seed;
         float o[3], d[3];
         create_ray(cam, x, y, f.width, f.height, o, d);
-        f.queue[0][2 * pix] = make_float4(o[0], o[1], o[2], __uint_as_float((uint32_t)pix));
-        f.queue[0][2 * pix + 1] = make_float4(d[0], d[1], d[2], 0.f);
+        f.queue[0][2 * slot] = make_float4(o[0], o[1], o[2], __uint_as_float((uint32_t)pix));
+        f.queue[0][2 * slot + 1] = make_float4(d[0], d[1], d[2], 0.f);
         f.ps_seed[pix] = seed;
         f.ps_through[pix] = make_float4(1.f, 1.f, 1.f, -1.f);
         f.ps_accum[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
