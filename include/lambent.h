@@ -481,7 +481,8 @@ lmb_status lmb_nrc_set_opts(lmb_nrc nrc, const lmb_nrc_opts *opts);
 typedef struct lmb_nrc_stats {
     uint64_t steps;          /* optimiser steps taken                     */
     uint64_t records;        /* training records of the last lmb_trace_nrc */
-    uint64_t queries;        /* cache queries of its last sample          */
+    uint64_t queries;        /* cache queries of its last sample (of its
+                                last group of samples traced together)    */
     float loss;              /* mean loss of its training steps           */
     uint32_t reserved;
 } lmb_nrc_stats;
