@@ -82,7 +82,7 @@ def rel_mse(img, ref) -> float:
 
 def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
               lr: float = 1e-3, spp: int = 8, steps: int = 32, overlap: bool = True,
-              warm_paths: int = 524288, warm_steps: int = 256) -> dict:
+              warm_paths: int = 524288, warm_steps: int = 256, ema: float = 0.0) -> dict:
     """Time-to-relMSE of the cached tracer against the plain tracer, both with
     8-bounce paths (RR from bounce 3), judged against a plain reference of
     ref_spp samples accumulated in fp64.  The cache starts untrained: `warm`
@@ -123,7 +123,8 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
         plain_pts.append({"spp": n_spp, "seconds": dt, "relmse": rel_mse(img, ref)})
         log(f"cache leg: plain {n_spp} spp {dt * 1e3:.1f} ms relMSE {plain_pts[-1]['relmse']:.5f}")
 
-    frame_opts = api.NeuralCacheOpts(learning_rate=lr, train_steps=steps, overlap_training=overlap)
+    frame_opts = api.NeuralCacheOpts(learning_rate=lr, train_steps=steps, overlap_training=overlap,
+                                     ema_decay=ema)
     cache = api.NeuralCache(api.NeuralCacheOpts(**{**frame_opts.__dict__, "train_paths": warm_paths,
                                                    "train_steps": warm_steps}))
     warm_pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=1, **popts))
@@ -360,6 +361,8 @@ def main() -> int:
     ap.add_argument("--cache-lr", type=float, default=1e-3)
     ap.add_argument("--cache-spp", type=int, default=8)
     ap.add_argument("--cache-steps", type=int, default=32)
+    ap.add_argument("--cache-ema", type=float, default=0.0,
+                    help="inference from an exponential moving average of the weights")
     ap.add_argument("--cache-warm-paths", type=int, default=524288)
     ap.add_argument("--cache-warm-steps", type=int, default=256)
     ap.add_argument("--cache-no-overlap", action="store_true",
@@ -467,7 +470,8 @@ def main() -> int:
         out["time_to_relmse"] = cache_leg(api, scene, spec, cam, args.ref_spp, args.cache_frames,
                                           args.cache_warmup, args.cache_lr, args.cache_spp,
                                           args.cache_steps, not args.cache_no_overlap,
-                                          args.cache_warm_paths, args.cache_warm_steps)
+                                          args.cache_warm_paths, args.cache_warm_steps,
+                                          args.cache_ema)
     if not args.no_spectral_leg:
         del scene
         out["spectral_restir"] = spectral_leg(api, scenes, args.triangles, args.width,

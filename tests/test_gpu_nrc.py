@@ -364,3 +364,26 @@ def test_switching_training_modes_keeps_inference_current():
     cache.set_weights(w)
     pipe.dispatch_cached(scene, cam, bufs, cache, 5)
     _same_render(bufs.output(api.Output.COLOR), _frozen_render(pipe, scene, cam, 40, 32, w, 5)[0])
+
+
+@pytest.mark.parametrize("quality", [api.Low, api.High, api.Spectral])
+def test_overlapped_training_other_qualities(quality):
+    """The grouped samples of an overlapped frame against a frozen cache's
+    one-sample-at-a-time render with the same weights, for the qualities
+    other than Medium (Low: no roughness, every metal and dielectric scatter
+    specular; High: one wavelength per path; Spectral: four)."""
+    spec = scenes.cornell_box(40, 40)
+    scene = spec.build()
+    cam = spec.camera()
+    pipe = api.RayTracer(quality).create_pipeline(
+        api.PathTracerOpts(samples=5, bounces=8, rr_start=3, guiding=api.Guiding.OFF))
+    bufs = api.DataBuffers(40, 40, 1)
+    cache = api.NeuralCache(api.NeuralCacheOpts(train_paths=512, train_steps=2, seed=8,
+                                                learning_rate=3e-3))
+    for f in range(3):
+        pipe.dispatch_cached(scene, cam, bufs, cache, f)
+    w = cache.weights()
+    pipe.dispatch_cached(scene, cam, bufs, cache, 3)
+    img = bufs.output(api.Output.COLOR)
+    assert np.isfinite(img).all() and img[..., :3].mean() > 0
+    _same_render(img, _frozen_render(pipe, scene, cam, 40, 40, w, 3)[0])
