@@ -141,8 +141,13 @@ lmb_status nrc_ensure_overlap(lmb_nrc n, uint64_t npix) {
         if (e != hipSuccess) return fail_hip(e, "hipMemset");
     }
     if (!n->inf_sets.ptr) {
-        lmb_status st = n->inf_sets.alloc(2 * (size_t)kNrcFwdFrags * kFragElems * 2);
+        // zeroed: the optimiser's publishing writes weight elements only,
+        // the padding elements stay zero
+        const size_t bytes = 2 * (size_t)kNrcFwdFrags * kFragElems * 2;
+        lmb_status st = n->inf_sets.alloc(bytes);
         if (st) return st;
+        e = hipMemset(n->inf_sets.ptr, 0, bytes);
+        if (e != hipSuccess) return fail_hip(e, "hipMemset");
         n->inf_valid = false;
     }
     const uint64_t grid = std::max<uint64_t>(n->opts.train_paths, 256);
@@ -383,6 +388,25 @@ lmb_status lmb_nrc_create(const lmb_nrc_opts* opts, lmb_nrc* out) {
     if (!st) st = n->bwd.alloc(kNrcBwdFrags * kFragElems * 2);
     if (!st) st = n->ctl.alloc(kNrcCtlWords * 4);
     if (!st) st = n->loss.alloc(kNrcMaxSteps * sizeof(float));
+    if (!st) {  // the fragment element of every weight, for the Adam kernel's re-pack
+        std::vector<int32_t> fpos(kNrcNumWeights, -1), bpos(kNrcNumWeights, -1);
+        for (int f = 0; f < kNrcFwdFrags; ++f)
+            for (int l = 0; l < 64; ++l)
+                for (int e = 0; e < 8; ++e) {
+                    const int w = nrc_fwd_index(f, l, e);
+                    if (w >= 0) fpos[w] = f * kFragElems + l * 8 + e;
+                }
+        for (int f = 0; f < kNrcBwdFrags; ++f)
+            for (int l = 0; l < 64; ++l)
+                for (int e = 0; e < 8; ++e) {
+                    const int w = nrc_bwd_index(f, l, e);
+                    if (w >= 0) bpos[w] = f * kFragElems + l * 8 + e;
+                }
+        for (int w = 0; w < kNrcNumWeights; ++w)
+            if (fpos[w] < 0) st = fail(LMB_ERR_DEVICE, "weight without a forward fragment element");
+        if (!st) st = n->fwd_pos.upload(fpos.data(), fpos.size() * 4);
+        if (!st) st = n->bwd_pos.upload(bpos.data(), bpos.size() * 4);
+    }
     if (!st) st = ensure_records(n, o.train_paths);
     if (st) {
         delete n;
@@ -396,6 +420,8 @@ lmb_status lmb_nrc_create(const lmb_nrc_opts* opts, lmb_nrc* out) {
     d.bwd = n->bwd.as<uint16_t>();
     d.ctl = n->ctl.as<uint32_t>();
     d.loss = n->loss.as<float>();
+    d.fwd_pos = n->fwd_pos.as<int32_t>();
+    d.bwd_pos = n->bwd_pos.as<int32_t>();
     d.fwd_inf = d.fwd;
     hipError_t e = hipMemset(n->ctl.ptr, 0, kNrcCtlWords * 4);
     if (e != hipSuccess) {

@@ -75,6 +75,7 @@ struct lmb_nrc_s {
     lmb_nrc_opts opts{};
     lmb::NrcDev dev{};
     lmb::DeviceBuffer master, adam_m, adam_v, fwd, bwd, ema, fwd_inf;
+    lmb::DeviceBuffer fwd_pos, bwd_pos;  // fragment element of each weight
     lmb::DeviceBuffer x0, h, dz, g, partial;  // training scratch for batch_cap samples
     uint32_t batch_cap = 0;
     lmb::DeviceBuffer ctl, loss;

@@ -137,6 +137,11 @@ struct NrcDev {
     float* ema;
     uint16_t* fwd_inf;  // read by the inference kernels
     uint16_t* fwd_pub;  // written by k_nrc_pack from the inference weights (null: not written)
+    // Fragment element of each master weight (nrc_fwd_index / nrc_bwd_index
+    // inverted; bwd_pos -1 for layer 0): the Adam kernel re-packs the
+    // weights it updates.
+    const int32_t* fwd_pos;
+    const int32_t* bwd_pos;
     float ema_decay;
     // Training scratch, feature-major bf16 with row stride `batch`:
     // x0 [320][batch], h [5][64][batch], dz [5][64][batch], g [3][batch].

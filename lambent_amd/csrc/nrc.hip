@@ -11,7 +11,8 @@
 //                  pre-activation gradients feature-major for
 //   k_nrc_dw       dW = X . dZ^T per 32 x 32 weight tile and 256-sample chunk
 //                  (K = samples), partial sums per chunk
-//   k_nrc_adam     sums the chunk partials and applies Adam to the masters
+//   k_nrc_adam     sums the chunk partials, applies Adam to the masters and
+//                  writes the updated weights' bf16 fragment elements
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -442,7 +443,25 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_adam(NrcDev d, AdamParams p,
     const float vhat = v / p.bias2;
     const float wn = d.master[w] - p.lr * mhat / (sqrtf(vhat) + p.eps);
     d.master[w] = wn;
-    if (d.ema) d.ema[w] = d.ema_decay * d.ema[w] + (1.0f - d.ema_decay) * wn;
+    float we = wn;
+    if (d.ema) {
+        we = d.ema_decay * d.ema[w] + (1.0f - d.ema_decay) * wn;
+        d.ema[w] = we;
+    }
+    // the bf16 fragment elements of this weight (k_nrc_pack's, one weight at a time)
+    const int32_t fp = d.fwd_pos[w], bp = d.bwd_pos[w];
+    d.fwd[fp] = bits((__bf16)wn);
+    if (bp >= 0) d.bwd[bp] = bits((__bf16)wn);
+    if (d.fwd_pub) d.fwd_pub[fp] = bits((__bf16)we);
+}
+
+// Publishes the inference fragments of a step whose batch was empty (no
+// update): the fragments of the unchanged weights.
+__global__ __launch_bounds__(kNrcBlock) void k_nrc_publish(NrcDev d, const uint32_t* __restrict__ count_ptr,
+                                                           uint32_t step) {
+    const int w = blockIdx.x * kNrcBlock + threadIdx.x;
+    if (w >= kNrcNumWeights || !d.fwd_pub || *count_ptr > step) return;
+    d.fwd_pub[d.fwd_pos[w]] = bits((__bf16)(d.ema ? d.ema[w] : d.master[w]));
 }
 
 hipError_t nrc_launch_pack(const NrcDev& d, hipStream_t stream) {
@@ -485,14 +504,13 @@ hipError_t nrc_launch_step(const NrcDev& d, const NrcQuery* in, const float4* ta
         const uint32_t gw = (d.batch / kNrcChunk) * kNrcGradTiles;
         hipLaunchKernelGGL(k_nrc_dw, dim3((gw + 3) / 4), dim3(kNrcBlock), 0, stream, d);
     }
-    hipLaunchKernelGGL(k_nrc_adam, dim3((kNrcNumWeights + kNrcBlock - 1) / kNrcBlock),
-                       dim3(kNrcBlock), 0, stream, d, adam, count, step, apply ? 1 : 0, grad_out,
-                       grad_in);
-    if (apply) {
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        return nrc_launch_pack(d, stream);
-    }
+    const dim3 wgrid((kNrcNumWeights + kNrcBlock - 1) / kNrcBlock);
+    hipLaunchKernelGGL(k_nrc_adam, wgrid, dim3(kNrcBlock), 0, stream, d, adam, count, step,
+                       apply ? 1 : 0, grad_out, grad_in);
+    // k_nrc_adam re-packs the fragments of the weights it updates; a step
+    // that found no records still publishes the inference fragments
+    if (apply && d.fwd_pub && !grad_in)
+        hipLaunchKernelGGL(k_nrc_publish, wgrid, dim3(kNrcBlock), 0, stream, d, count, step);
     return hipGetLastError();
 }
 

@@ -87,6 +87,31 @@ def test_gradient_matches_oracle():
     assert cos > 0.99, cos
 
 
+def test_optimiser_steps_keep_the_fragments_current():
+    """The Adam kernel writes the bf16 fragment elements of the weights it
+    updates (forward and backward sets): after optimiser steps, inference and
+    the gradient match the oracle at the new master weights."""
+    cache = api.NeuralCache(api.NeuralCacheOpts(learning_rate=0.02))
+    cache.set_weights(he_weights(10))
+    rng = np.random.default_rng(11)
+    n = 600
+    x = rng.random((n, 13)).astype(np.float32)
+    t = (rng.random((n, 3)) * 2).astype(np.float32)
+    for _ in range(3):
+        cache.train(x, t)
+    w = cache.weights()
+    assert np.abs(w - he_weights(10)).max() > 0.01  # the steps moved the weights
+    y = cache.infer(x)
+    ref = oracle.nrc_forward(w.astype(np.float64), x, oracle.NRC_BF16)
+    scale = max(1.0, float(np.abs(ref).max()))
+    assert np.abs(y - ref).max() < 2e-2 * scale
+    g, _ = cache.gradient(x, t)
+    gref, _ = oracle.nrc_grad(w.astype(np.float64), x, t, oracle.NRC_BF16)
+    for a, b in LAYERS:
+        rel = np.linalg.norm(g[a:b].astype(np.float64) - gref[a:b]) / np.linalg.norm(gref[a:b])
+        assert rel < 3e-2, (a, rel)
+
+
 def test_adam_step_matches_oracle_exactly():
     cache = api.NeuralCache()
     w = he_weights(8)
