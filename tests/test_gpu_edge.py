@@ -86,3 +86,23 @@ def test_resize_between_multi_sample_traces():
     _trace(spec_a, scene, 40, 24, samples=4, seed=3, bufs=bufs)
     again = _trace(spec_a, scene, 40, 24, samples=4, seed=3)
     np.testing.assert_array_equal(bufs.output(api.Output.COLOR), again.output(api.Output.COLOR))
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (7, 3), (33, 17)])
+def test_cached_single_bounce_equals_plain(w, h):
+    """bounces = 1: no path reaches a cache vertex, so the cached tracer
+    (samples grouped, training beside them) renders what the plain tracer
+    does: each sample is the camera ray's emission or the sky."""
+    spec = scenes.cornell_box(w, h)
+    scene = spec.build()
+    opts = api.PathTracerOpts(samples=4, bounces=1, guiding=api.Guiding.OFF)
+    pipe = api.RayTracer(api.Medium).create_pipeline(opts)
+    plain = api.DataBuffers(w, h, 1)
+    pipe.dispatch(scene, spec.camera(), plain, 2)
+    cached = api.DataBuffers(w, h, 1)
+    cache = api.NeuralCache(api.NeuralCacheOpts(train_paths=256, train_steps=2))
+    for f in range(2):
+        pipe.dispatch_cached(scene, spec.camera(), cached, cache, 2)
+    np.testing.assert_array_equal(cached.output(api.Output.COLOR), plain.output(api.Output.COLOR))
+    np.testing.assert_array_equal(cached.output(api.Output.ALBEDO), plain.output(api.Output.ALBEDO))
+    assert cache.stats()["queries"] == 0
