@@ -551,6 +551,17 @@ static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera
     // caller's stream is.
     st = nrc_ensure_overlap(nrc, npix);
     if (st) return st;
+    // Samples traced up to kWideSamples at a time (path id pixel * w + j must
+    // fit 31 bits); their state and w * npix queries, allocated before the
+    // device views below are taken.
+    const uint64_t w_max = std::min<uint64_t>(kWideSamples, 0x7FFFFFFFull / std::max<uint64_t>(npix, 1));
+    const uint32_t w = (uint32_t)std::min<uint64_t>(p.samples, w_max);
+    const bool wide = w > 1 && bufs->dev.hits_primary;
+    if (wide) {
+        st = nrc_ensure_wide(nrc, (uint64_t)w * npix);
+        if (!st) st = nrc_ensure_queries(nrc, (uint64_t)w * npix);
+        if (st) return st;
+    }
     const hipStream_t ts = nrc->train_stream;
     uint16_t* sets = nrc->inf_sets.as<uint16_t>();
     uint16_t* cur = sets + (size_t)nrc->inf_cur * kNrcFwdFrags * kFragElems;
@@ -575,17 +586,7 @@ static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera
     if (e == hipSuccess) e = hipEventRecord(nrc->ev_start, hs);
     if (e != hipSuccess) return fail_hip(e, "training stream order");
     np.train_expect = 0.0f;  // no training paths among the samples
-    // Samples traced up to kWideSamples at a time (path id j * npix + pixel
-    // must fit 31 bits).
-    const uint64_t w_max = std::min<uint64_t>(kWideSamples, 0x7FFFFFFFull / std::max<uint64_t>(npix, 1));
-    const uint32_t w = (uint32_t)std::min<uint64_t>(p.samples, w_max);
-    if (w > 1 && bufs->dev.hits_primary) {
-        st = nrc_ensure_wide(nrc, (uint64_t)w * npix);
-        if (!st) st = nrc_ensure_queries(nrc, (uint64_t)w * npix);
-        if (st) return st;
-        da.queries = nrc->dev.queries;
-        da.query_weight = nrc->dev.query_weight;
-        da.query_cap = nrc->dev.query_cap;
+    if (wide) {
         FrameDev fw = bufs->dev;
         fw.ps_seed = nrc->w_seed.as<uint32_t>();
         fw.ps_through = nrc->w_through.as<float4>();
