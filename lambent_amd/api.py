@@ -1004,7 +1004,10 @@ NRC_INPUTS = 13
 @dataclass
 class NeuralCacheOpts:
     """lmb_nrc_opts: Adam parameters, training paths and optimiser steps per
-    cached frame, weight-initialisation seed."""
+    cached frame (steps 1..256), weight-initialisation seed, EMA decay of the
+    inference weights, the spread rule's bound (defer_specular) and whether
+    the training runs on the cache's own stream beside the frame's samples
+    (overlap_training)."""
     learning_rate: float = 1e-3
     beta1: float = 0.9
     beta2: float = 0.99
