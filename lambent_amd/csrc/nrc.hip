@@ -44,6 +44,29 @@ __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
 __device__ __forceinline__ bf16x8 to_bf16(f32x8 v) { return __builtin_convertvector(v, bf16x8); }
 __device__ __forceinline__ uint16_t bits(__bf16 x) { return __builtin_bit_cast(uint16_t, x); }
 
+// Copies n 16-byte fragment elements from global memory into LDS with a
+// BLOCK-thread block, issuing each thread's loads back to back in batches of
+// kFillBatch before their LDS writes (a load-wait-write loop would pay one
+// L2 round trip per element).
+constexpr int kFillBatch = 8;
+template <int BLOCK>
+__device__ __forceinline__ void fill_lds(bf16x8* __restrict__ lds, const bf16x8* __restrict__ src,
+                                         int n) {
+    for (int base = 0; base < n; base += BLOCK * kFillBatch) {
+        bf16x8 v[kFillBatch];
+#pragma unroll
+        for (int k = 0; k < kFillBatch; ++k) {
+            const int i = base + k * BLOCK + (int)threadIdx.x;
+            if (i < n) v[k] = src[i];
+        }
+#pragma unroll
+        for (int k = 0; k < kFillBatch; ++k) {
+            const int i = base + k * BLOCK + (int)threadIdx.x;
+            if (i < n) lds[i] = v[k];
+        }
+    }
+}
+
 struct FragPtr {
     const bf16x8* p;
     __device__ __forceinline__ bf16x8 operator()(int frag, int lane) const {
@@ -200,7 +223,7 @@ __global__ __launch_bounds__(kNrcInferBlock) void k_nrc_infer(NrcDev d, const Nr
                                                          float4* __restrict__ dst) {
     __shared__ bf16x8 lds[kNrcFwdFrags * 64];
     const bf16x8* src = reinterpret_cast<const bf16x8*>(d.fwd_inf);
-    for (int i = threadIdx.x; i < kNrcFwdFrags * 64; i += kNrcInferBlock) lds[i] = src[i];
+    fill_lds<kNrcInferBlock>(lds, src, kNrcFwdFrags * 64);
     __syncthreads();
     const uint32_t count = count_ptr ? *count_ptr : count_fixed;
     const int lane = threadIdx.x & 63, n = lane & 31, h = lane >> 5;
@@ -281,9 +304,8 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_fwd_bwd(NrcDev d, const NrcQu
     {
         const bf16x8* sf = reinterpret_cast<const bf16x8*>(d.fwd);
         const bf16x8* sb = reinterpret_cast<const bf16x8*>(d.bwd);
-        for (int i = threadIdx.x; i < kNrcFwdFrags * 64; i += kNrcBlock) lds[i] = sf[i];
-        for (int i = threadIdx.x; i < kNrcBwdFrags * 64; i += kNrcBlock)
-            lds[kNrcFwdFrags * 64 + i] = sb[i];
+        fill_lds<kNrcBlock>(lds, sf, kNrcFwdFrags * 64);
+        fill_lds<kNrcBlock>(lds + kNrcFwdFrags * 64, sb, kNrcBwdFrags * 64);
     }
     __syncthreads();
     const FragPtr fr{lds}, br{lds + kNrcFwdFrags * 64};
