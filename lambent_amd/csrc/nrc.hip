@@ -387,12 +387,9 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_fwd_bwd(NrcDev d, const NrcQu
 // dW tile (32 input rows x 32 output columns) of one layer over one chunk of
 // kNrcChunk samples: K = samples, both operands loaded straight from the
 // feature-major scratch (16 contiguous bytes per lane).
-__global__ __launch_bounds__(kNrcBlock) void k_nrc_dw(NrcDev d) {
-    const uint32_t wave = (blockIdx.x * kNrcBlock + threadIdx.x) >> 6;
-    const uint32_t chunks = d.batch / kNrcChunk;
-    if (wave >= chunks * kNrcGradTiles) return;
-    const uint32_t tile = wave % kNrcGradTiles, chunk = wave / kNrcGradTiles;
-    int layer, rt, ct;
+// Weight-gradient tile `tile` (32 x 32; layer 0: 10 x 2 tiles, hidden
+// layers 2 x 2 each, output 2 x 1): its layer and row / column tile.
+__device__ __forceinline__ void grad_tile(uint32_t tile, int& layer, int& rt, int& ct) {
     if (tile < 20) {
         layer = 0;
         rt = tile >> 1;
@@ -406,8 +403,26 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_dw(NrcDev d) {
         rt = tile - 36;
         ct = 0;
     }
+}
+
+// Master-weight index of accumulator register r of lane `lane` of a tile's
+// product, or -1 (output layer columns past the third).
+__device__ __forceinline__ int grad_tile_weight(int layer, int rt, int ct, int r, int lane) {
+    const int h = lane >> 5;
+    const int inr = rt * 32 + (r >> 2) * 8 + h * 4 + (r & 3);
+    const int out = ct * 32 + (lane & 31);
+    if (layer == 0) return kNrcW0 + inr * kNrcWidth + out;
+    if (layer < 5) return kNrcWH + (layer - 1) * kNrcWidth * kNrcWidth + inr * kNrcWidth + out;
+    return out < kNrcOut ? kNrcW5 + inr * kNrcOut + out : -1;
+}
+
+// One wave's partial dW of a tile over sample chunk `chunk` (K = samples,
+// 16-byte operand loads straight from the feature-major layer inputs and
+// pre-activation gradients).
+__device__ __forceinline__ f32x16 grad_tile_chunk(const NrcDev& d, int layer, int rt, int ct,
+                                                  uint32_t chunk, int lane) {
     const uint32_t S = d.batch;
-    const int lane = threadIdx.x & 63, h = lane >> 5;
+    const int h = lane >> 5;
     const uint16_t* A = layer == 0 ? d.x0 : d.h + (size_t)(layer - 1) * kNrcWidth * S;
     const uint16_t* B = layer == 5 ? d.g : d.dz + (size_t)layer * kNrcWidth * S;
     const int arow = rt * 32 + (lane & 31);
@@ -423,20 +438,46 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_dw(NrcDev d) {
         const bf16x8 b = bvalid ? bp[2 * k] : bf16x8{};
         acc = mfma(a, b, acc);
     }
+    return acc;
+}
+
+__global__ __launch_bounds__(kNrcBlock) void k_nrc_dw(NrcDev d) {
+    const uint32_t wave = (blockIdx.x * kNrcBlock + threadIdx.x) >> 6;
+    const uint32_t chunks = d.batch / kNrcChunk;
+    if (wave >= chunks * kNrcGradTiles) return;
+    const uint32_t tile = wave % kNrcGradTiles, chunk = wave / kNrcGradTiles;
+    int layer, rt, ct;
+    grad_tile(tile, layer, rt, ct);
+    const int lane = threadIdx.x & 63;
+    const f32x16 acc = grad_tile_chunk(d, layer, rt, ct, chunk, lane);
     float* dst = d.partial + (size_t)chunk * kNrcNumWeights;
-    const int out = bcol;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        const int inr = rt * 32 + (r >> 2) * 8 + h * 4 + (r & 3);
-        int w;
-        if (layer == 0)
-            w = kNrcW0 + inr * kNrcWidth + out;
-        else if (layer < 5)
-            w = kNrcWH + (layer - 1) * kNrcWidth * kNrcWidth + inr * kNrcWidth + out;
-        else
-            w = out < kNrcOut ? kNrcW5 + inr * kNrcOut + out : -1;
+        const int w = grad_tile_weight(layer, rt, ct, r, lane);
         if (w >= 0) dst[w] = acc[r];
     }
+}
+
+// Adam on master weight w with gradient g; writes the weight's bf16
+// fragment elements (k_nrc_pack's, one weight at a time).
+__device__ __forceinline__ void adam_update(const NrcDev& d, const AdamParams& p, int w, float g) {
+    const float m = p.beta1 * d.adam_m[w] + (1.0f - p.beta1) * g;
+    const float v = p.beta2 * d.adam_v[w] + (1.0f - p.beta2) * (g * g);
+    d.adam_m[w] = m;
+    d.adam_v[w] = v;
+    const float mhat = m / p.bias1;
+    const float vhat = v / p.bias2;
+    const float wn = d.master[w] - p.lr * mhat / (sqrtf(vhat) + p.eps);
+    d.master[w] = wn;
+    float we = wn;
+    if (d.ema) {
+        we = d.ema_decay * d.ema[w] + (1.0f - d.ema_decay) * wn;
+        d.ema[w] = we;
+    }
+    const int32_t fp = d.fwd_pos[w], bp = d.bwd_pos[w];
+    d.fwd[fp] = bits((__bf16)wn);
+    if (bp >= 0) d.bwd[bp] = bits((__bf16)wn);
+    if (d.fwd_pub) d.fwd_pub[fp] = bits((__bf16)we);
 }
 
 __global__ __launch_bounds__(kNrcBlock) void k_nrc_adam(NrcDev d, AdamParams p,
@@ -456,25 +497,7 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_adam(NrcDev d, AdamParams p,
         for (uint32_t c = 0; c < chunks; ++c) g += d.partial[(size_t)c * kNrcNumWeights + w];
     }
     if (grad_out) grad_out[w] = g;
-    if (!apply) return;
-    const float m = p.beta1 * d.adam_m[w] + (1.0f - p.beta1) * g;
-    const float v = p.beta2 * d.adam_v[w] + (1.0f - p.beta2) * (g * g);
-    d.adam_m[w] = m;
-    d.adam_v[w] = v;
-    const float mhat = m / p.bias1;
-    const float vhat = v / p.bias2;
-    const float wn = d.master[w] - p.lr * mhat / (sqrtf(vhat) + p.eps);
-    d.master[w] = wn;
-    float we = wn;
-    if (d.ema) {
-        we = d.ema_decay * d.ema[w] + (1.0f - d.ema_decay) * wn;
-        d.ema[w] = we;
-    }
-    // the bf16 fragment elements of this weight (k_nrc_pack's, one weight at a time)
-    const int32_t fp = d.fwd_pos[w], bp = d.bwd_pos[w];
-    d.fwd[fp] = bits((__bf16)wn);
-    if (bp >= 0) d.bwd[bp] = bits((__bf16)wn);
-    if (d.fwd_pub) d.fwd_pub[fp] = bits((__bf16)we);
+    if (apply) adam_update(d, p, w, g);
 }
 
 // Publishes the inference fragments of a step whose batch was empty (no
