@@ -504,8 +504,21 @@ __global__ __launch_bounds__(kGBlock) void k_g_finish(FrameDev f, GuidedDev g, T
         const Chain wc = grid_load(f.wmc, f.wmc_old, k, false);
         const Chain st = update_markov_state(point, vis_point, length3(sout), seed, false, wc, 1.0f,
                                              g.sum_mc[pix]);
-        grid_store(f.wmc, k, st);
-        grid_add_radiance(f.wmc, k, sout);
+        // Lanes of a wave that update one grid cell: in lock step their
+        // stores land as one lane's and only one lane takes the cell's
+        // try-lock, so only the lowest such lane writes.  (Every pixel whose
+        // path left for the sky at once shares one cell: done lane by lane,
+        // their same-address atomics queue in L2 for milliseconds.)
+        bool first = true;
+        const uint64_t live = __ballot(1);
+        for (int l = 0; l < 64; ++l) {
+            const uint32_t other = __shfl(k.idx, l);
+            if (l < (int)__lane_id() && ((live >> l) & 1ull) && other == k.idx) first = false;
+        }
+        if (first) {
+            grid_store(f.wmc, k, st);
+            grid_add_radiance(f.wmc, k, sout);
+        }
         // rt_main
         const float4 ps = f.pix_sum[pix];
         const V3 scol = xyz(g.scol[pix]);
