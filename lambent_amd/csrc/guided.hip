@@ -147,6 +147,22 @@ __global__ __launch_bounds__(kGBlock) void k_g_generate(CameraDev cam, FrameDev 
 
 // World-grid training of the previous vertex when the path reaches
 // `new_pos` carrying radiance_out (+ reflected estimate).
+// Whether this lane is the lowest active lane of its wave that writes grid
+// cell k.  Lanes of a wave that write one cell: in lock step their stores
+// land as one lane's and only one lane takes the cell's try-lock, so only
+// the lowest such lane need write (lane by lane, same-address atomics queue
+// in L2: every pixel whose path went to the sky shares one cell in the final
+// update; the per-bounce updates, at distinct hit points, gain nothing).
+__device__ __forceinline__ bool first_writer_of_cell(const GridKey& k) {
+    bool first = true;
+    const uint64_t live = __ballot(1);
+    for (int l = 0; l < 64; ++l) {
+        const uint32_t other = __shfl(k.idx, l);
+        if (l < (int)__lane_id() && ((live >> l) & 1ull) && other == k.idx) first = false;
+    }
+    return first;
+}
+
 __device__ __forceinline__ void train_previous(const FrameDev& f, const GuidedDev& g,
                                                V3 camera_loc, uint32_t pid, V3 prev_pos,
                                                V3 new_pos, V3 total, V3 radiance_out,
@@ -338,25 +354,28 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
                 const uint32_t x = pid % f.width, y = pid / f.width;
                 const ScreenCands sc = screen_candidates(g.cand_seed[pid], x, y, f.width);
                 const Chain lane_chain = sc.lane_ok ? chain_load(g.tent_snap[sc.lane_pix]) : chain_zero();
-                // resample_mcmm (general.wesl:404-425)
+                // resample_mcmm (general.wesl:404-425).  Candidate i: the
+                // shuffled lane chain (i < 4), last frame's screen chains
+                // (12), the world-space chains (4), the tentative chain; the
+                // loops run in that order with constant array indices, so
+                // the candidate arrays stay in registers.
                 Chain sel = chain_zero();
                 float sum_mc = 0.0f;
-                for (uint32_t i = 0; i < kNumResamples; ++i) {
+                auto resample = [&](const Chain& cand) {
                     seed = rand_u32(seed);
-                    Chain cand;
-                    if (i < kScreenResamples)
-                        cand = lane_chain;
-                    else if (i < kScreenResamples + kScreenLastFrame)
-                        cand = chain_load(load_mc_or_zero(f.mc_old, f.mc_count,
-                                                          sc.idx[i - kScreenResamples]));
-                    else if (i < kNumResamples - 1)
-                        cand = wc[i - kScreenResamples - kScreenLastFrame];
-                    else
-                        cand = tent;
                     sum_mc += cand.weight_sum;
                     if (rand_f32(seed) * sum_mc < cand.weight_sum) sel = cand;
                     seed = rand_u32(seed);
-                }
+                };
+                for (uint32_t i = 0; i < kScreenResamples; ++i) resample(lane_chain);
+#pragma unroll
+                for (uint32_t i = 0; i < kScreenLastFrame; ++i)
+                    resample(chain_load(load_mc_or_zero(f.mc_old, f.mc_count, sc.idx[i])));
+#pragma unroll
+                for (uint32_t i = 0; i < kWorldResamples; ++i) resample(wc[i]);
+                resample(tent);
+                static_assert(kNumResamples == kScreenResamples + kScreenLastFrame + kWorldResamples + 1,
+                              "candidate groups");
                 g.sum_mc[pid] = sum_mc;
                 float4 c4 = g.color[pid];
                 V3 color = xyz(c4);
@@ -406,22 +425,20 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
                         // all_vmf_pdfs (general.wesl:984-998) over the same candidates
                         float mix_pdf = 0.0f;
                         if (conf1 != 0.0f && sum_mc > 0.0f) {
-                            for (uint32_t i = 0; i < kNumResamples; ++i) {
-                                Chain cand;
-                                if (i < kScreenResamples)
-                                    cand = lane_chain;
-                                else if (i < kScreenResamples + kScreenLastFrame)
-                                    cand = chain_load(load_mc_or_zero(f.mc_old, f.mc_count,
-                                                                      sc.idx[i - kScreenResamples]));
-                                else if (i < kNumResamples - 1)
-                                    cand = wc[i - kScreenResamples - kScreenLastFrame];
-                                else
-                                    cand = tent;
-                                if (!(cand.weight_sum > 0.0f)) continue;
+                            // the candidates in resample_mcmm's order
+                            auto add_pdf = [&](const Chain& cand) {
+                                if (!(cand.weight_sum > 0.0f)) return;
                                 mix_pdf += vmf_pdf(normalize3(get_light_dir(cand, origin)),
                                                    get_kappa(cand), d) *
                                            (cand.weight_sum / sum_mc);
-                            }
+                            };
+                            for (uint32_t i = 0; i < kScreenResamples; ++i) add_pdf(lane_chain);
+#pragma unroll
+                            for (uint32_t i = 0; i < kScreenLastFrame; ++i)
+                                add_pdf(chain_load(load_mc_or_zero(f.mc_old, f.mc_count, sc.idx[i])));
+#pragma unroll
+                            for (uint32_t i = 0; i < kWorldResamples; ++i) add_pdf(wc[i]);
+                            add_pdf(tent);
                         }
                         const float avg_pdf =
                             conf0 * cos_pdf + (conf1 == 0.0f ? 0.0f : conf1 * fmaxf(mix_pdf, 0.0f));
@@ -504,18 +521,7 @@ __global__ __launch_bounds__(kGBlock) void k_g_finish(FrameDev f, GuidedDev g, T
         const Chain wc = grid_load(f.wmc, f.wmc_old, k, false);
         const Chain st = update_markov_state(point, vis_point, length3(sout), seed, false, wc, 1.0f,
                                              g.sum_mc[pix]);
-        // Lanes of a wave that update one grid cell: in lock step their
-        // stores land as one lane's and only one lane takes the cell's
-        // try-lock, so only the lowest such lane writes.  (Every pixel whose
-        // path left for the sky at once shares one cell: done lane by lane,
-        // their same-address atomics queue in L2 for milliseconds.)
-        bool first = true;
-        const uint64_t live = __ballot(1);
-        for (int l = 0; l < 64; ++l) {
-            const uint32_t other = __shfl(k.idx, l);
-            if (l < (int)__lane_id() && ((live >> l) & 1ull) && other == k.idx) first = false;
-        }
-        if (first) {
+        if (first_writer_of_cell(k)) {
             grid_store(f.wmc, k, st);
             grid_add_radiance(f.wmc, k, sout);
         }
