@@ -139,6 +139,9 @@ __device__ __forceinline__ void layer0(const F& fr, const float (&x)[NT][kNrcInp
             c[0][t] = mfma(a0, b, c[0][t]);
             c[1][t] = mfma(a1, b, c[1][t]);
         }
+        // one K step's fragment reads at a time: hoisting all 40 of them
+        // ahead of the MFMAs needs more than 256 registers (spills)
+        __builtin_amdgcn_sched_barrier(0);
     });
 }
 
@@ -207,13 +210,11 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_pack(NrcDev d) {
 // Inference.  kAccumulate: pix_sum[pid] += weight * max(y, 0) for the
 // queries counted in *count; otherwise out[i] = y for i < count.
 // Inference blocks: 8 waves share one LDS copy of the forward fragments,
-// one block per CU (2 waves per SIMD at 164 VGPRs).
-#ifndef LMB_NRC_INFER_WAVES
-#define LMB_NRC_INFER_WAVES 8
-#endif
-constexpr int kNrcInferWaves = LMB_NRC_INFER_WAVES;
+// one block per CU (2 waves per SIMD; 4 waves at two blocks per CU, or
+// 4-wave blocks, measured slower).
+constexpr int kNrcInferWaves = 8;
 constexpr int kNrcInferBlock = 64 * kNrcInferWaves;
-constexpr int kNrcInferBlocksPerCu = kNrcInferWaves > 4 ? 1 : 2;
+constexpr int kNrcInferBlocksPerCu = 1;
 
 template <bool kAccumulate>
 __global__ __launch_bounds__(kNrcInferBlock) void k_nrc_infer(NrcDev d, const NrcQuery* __restrict__ q,
