@@ -506,22 +506,43 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
 
 // Final world-grid update of the sample (general.wesl:944-948) and the
 // per-sample tail of rt_main (general.wesl:535-581).
+// Claims of the grid cells the block writes in one pass of k_g_finish: a
+// small open table in LDS (key + 1, 0 = empty), one probe.
+constexpr uint32_t kClaimSlots = 128;
+
 template <uint32_t Q>
 __global__ __launch_bounds__(kGBlock) void k_g_finish(FrameDev f, GuidedDev g, TraceParams p,
                                                       V3 camera_loc) {
+    __shared__ uint32_t claim[kClaimSlots];
     const uint64_t stride = (uint64_t)gridDim.x * kGBlock;
-    for (uint64_t pix = (uint64_t)blockIdx.x * kGBlock + threadIdx.x; pix < f.npix;
-         pix += stride) {
+    const uint64_t n_r = block_uniform_count<kGBlock>((uint32_t)f.npix);
+    for (uint64_t pix0 = (uint64_t)blockIdx.x * kGBlock + threadIdx.x; pix0 < n_r; pix0 += stride) {
+        for (uint32_t i = threadIdx.x; i < kClaimSlots; i += kGBlock) claim[i] = 0u;
+        __syncthreads();
+        const bool live = pix0 < f.npix;
+        const uint64_t pix = live ? pix0 : 0;
+        const GridKey k = grid_key(xyz(g.s_point[pix]), xyz(g.s_normal[pix]),
+                                   xyz(g.s_start_dir[pix]), camera_loc, f.world_size);
+        // The lanes of the block that write one cell: as in lock step, one
+        // lane's writes land; the first lane of each wave claims the cell in
+        // the block's table, and a wave whose claim finds the cell already
+        // taken leaves it (a different cell in the slot: it writes anyway).
+        bool writer = false;
+        if (live && first_writer_of_cell(k)) {
+            const uint32_t prev = atomicCAS(&claim[k.idx % kClaimSlots], 0u, k.idx + 1u);
+            writer = prev == 0u || prev != k.idx + 1u;
+        }
+        __syncthreads();
+        if (!live) continue;
         uint32_t seed = f.ps_seed[pix];
         const float4 sp = g.s_point[pix], sn = g.s_normal[pix];
         const V3 point = xyz(sp), normal = xyz(sn);
         const V3 vis_point = xyz(g.s_vis_point[pix]);
         const V3 sout = xyz(g.sout[pix]);
-        const GridKey k = grid_key(point, normal, xyz(g.s_start_dir[pix]), camera_loc, f.world_size);
         const Chain wc = grid_load(f.wmc, f.wmc_old, k, false);
         const Chain st = update_markov_state(point, vis_point, length3(sout), seed, false, wc, 1.0f,
                                              g.sum_mc[pix]);
-        if (first_writer_of_cell(k)) {
+        if (writer) {
             grid_store(f.wmc, k, st);
             grid_add_radiance(f.wmc, k, sout);
         }
