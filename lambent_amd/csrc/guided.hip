@@ -145,8 +145,6 @@ __global__ __launch_bounds__(kGBlock) void k_g_generate(CameraDev cam, FrameDev 
     }
 }
 
-// World-grid training of the previous vertex when the path reaches
-// `new_pos` carrying radiance_out (+ reflected estimate).
 // Whether this lane is the lowest active lane of its wave that writes grid
 // cell k.  Lanes of a wave that write one cell: in lock step their stores
 // land as one lane's and only one lane takes the cell's try-lock, so only
@@ -163,6 +161,8 @@ __device__ __forceinline__ bool first_writer_of_cell(const GridKey& k) {
     return first;
 }
 
+// World-grid training of the previous vertex when the path reaches
+// `new_pos` carrying radiance_out (+ reflected estimate).
 __device__ __forceinline__ void train_previous(const FrameDev& f, const GuidedDev& g,
                                                V3 camera_loc, uint32_t pid, V3 prev_pos,
                                                V3 new_pos, V3 total, V3 radiance_out,
