@@ -175,6 +175,9 @@ lmb_status alloc_window(lmb_data_buffers b, uint32_t width, uint32_t height) {
     width = width ? width : 1;
     height = height ? height : 1;
     const uint64_t n = (uint64_t)width * height;
+    // Ray queues carry the pixel index as a u32 and the kernels split it
+    // into (x, y) with 32-bit arithmetic.
+    if (n >= (1ull << 31)) return fail(LMB_ERR_INVALID_ARGUMENT, "width * height must be below 2^31");
     lmb_status st;
     if ((st = alloc_zero(b->out_color, n * 16))) return st;
     if ((st = alloc_zero(b->out_normal, n * 16))) return st;

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kGBlock) void k_g_generate(CameraDev cam, FrameDev 
         if (sample == 0) *reinterpret_cast<uint64_t*>(f.counters + kCounterVertices) = 0;
     }
     for (uint64_t pix = gtid; pix < f.npix; pix += stride) {
-        const uint32_t x = (uint32_t)(pix % f.width), y = (uint32_t)(pix / f.width);
+        const uint32_t x = (uint32_t)pix % f.width, y = (uint32_t)pix / f.width;
         uint32_t own_seed;
         if (sample == 0) {
             own_seed = rand_u32(x * y) + p.seed_offset + y;
@@ -617,7 +617,7 @@ __global__ __launch_bounds__(kGBlock) void k_g_shuffle(FrameDev f, GuidedDev g) 
     const uint64_t stride = (uint64_t)gridDim.x * kGBlock;
     for (uint64_t pix = (uint64_t)blockIdx.x * kGBlock + threadIdx.x; pix < f.npix;
          pix += stride) {
-        const uint32_t x = (uint32_t)(pix % f.width), y = (uint32_t)(pix / f.width);
+        const uint32_t x = (uint32_t)pix % f.width, y = (uint32_t)pix / f.width;
         const uint32_t base = (x / kSubgroup) * kSubgroup;
         const uint32_t active = min(kSubgroup, f.width - base);
         const uint32_t src = base + ((x - base + 1u) % active);

@@ -287,7 +287,7 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_nrc_generate_train(CameraDev
         uint32_t idx[1];
         block_reserve<kNrcTraceBlock, 1>({sel}, {f.counters}, rl, idx);
         if (!sel) continue;
-        const uint32_t x = (uint32_t)(pix % f.width), y = (uint32_t)(pix / f.width);
+        const uint32_t x = (uint32_t)pix % f.width, y = (uint32_t)pix / f.width;
         const uint32_t seed = rand_u32(x * y) + p.seed_offset + y;
         float o[3], dd[3];
         create_ray(cam, x, y, f.width, f.height, o, dd);

@@ -32,7 +32,7 @@ __global__ __launch_bounds__(kRsBlock) void k_temporal(FrameDev f, uint32_t seed
     if (f.npix > (uint64_t)f.gi_count) return;  // reservoir buffers hold one pixel each
     for (uint64_t pix = (uint64_t)blockIdx.x * kRsBlock + threadIdx.x; pix < f.npix;
          pix += stride) {
-        const uint32_t x = (uint32_t)(pix % f.width), y = (uint32_t)(pix / f.width);
+        const uint32_t x = (uint32_t)pix % f.width, y = (uint32_t)pix / f.width;
         const uint32_t own_seed = rand_u32(x * y) + rand_u32(seed_offset + y);
         Reservoir Rs = f.gi[pix];
         Reservoir R = f.gi_old[pix];
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kRsBlock) void k_spatial(FrameDev f, uint32_t is_sa
     const uint32_t self_idx = min_space + min_space * is_space;
     for (uint64_t pix = (uint64_t)blockIdx.x * kRsBlock + threadIdx.x; pix < f.npix;
          pix += stride) {
-        const uint32_t x = (uint32_t)(pix % f.width), y = (uint32_t)(pix / f.width);
+        const uint32_t x = (uint32_t)pix % f.width, y = (uint32_t)pix / f.width;
         uint32_t own_seed = rand_u32(x * y) + rand_u32(seed_offset) + y;
         Reservoir Rs = f.gi[pix];
         if (Rs.ty == kMatSky) continue;

@@ -88,6 +88,21 @@ def test_resize_between_multi_sample_traces():
     np.testing.assert_array_equal(bufs.output(api.Output.COLOR), again.output(api.Output.COLOR))
 
 
+def test_pixel_count_limit_is_rejected_before_allocating():
+    # Pixel indices travel as u32 in the ray queues: 2^31 pixels and more are
+    # refused, and a refused resize leaves the buffers at their old size.
+    with pytest.raises(api.LambentError, match="2\\^31"):
+        api.DataBuffers(65536, 32768, 1)
+    spec = scenes.cornell_box(40, 24)
+    scene = spec.build()
+    bufs = api.DataBuffers(40, 24, 1)
+    with pytest.raises(api.LambentError, match="2\\^31"):
+        bufs.resize(1 << 16, 1 << 15)
+    _trace(spec, scene, 40, 24, samples=2, seed=3, bufs=bufs)
+    fresh = _trace(spec, scene, 40, 24, samples=2, seed=3)
+    np.testing.assert_array_equal(bufs.output(api.Output.COLOR), fresh.output(api.Output.COLOR))
+
+
 @pytest.mark.parametrize("w,h", [(1, 1), (7, 3), (33, 17)])
 def test_cached_single_bounce_equals_plain(w, h):
     """bounces = 1: no path reaches a cache vertex, so the cached tracer
