@@ -373,8 +373,9 @@ def main() -> int:
     if args.calibrate_copy:
         return calibrate_copy()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(image rows would shard across GPUs; see DESIGN.md). Run with --gpus 1.",
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(image rows would shard across GPUs, one band of rows per rank). "
+              "Run with --gpus 1.",
               file=sys.stderr)
         return 2
 
