@@ -509,18 +509,25 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
 // Claims of the grid cells the block writes in one pass of k_g_finish: a
 // small open table in LDS (key + 1, 0 = empty), one probe.
 constexpr uint32_t kClaimSlots = 128;
+// A pass of k_g_finish covers a square tile of pixels: grid cells are
+// coherent in screen space, so a cell's pixels fall into fewer blocks than
+// with row strips, and fewer blocks write each shared cell.
+constexpr uint32_t kFinishTile = 16;
+static_assert(kFinishTile * kFinishTile == kGBlock, "one lane per pixel of a finish tile");
 
 template <uint32_t Q>
 __global__ __launch_bounds__(kGBlock) void k_g_finish(FrameDev f, GuidedDev g, TraceParams p,
                                                       V3 camera_loc) {
     __shared__ uint32_t claim[kClaimSlots];
-    const uint64_t stride = (uint64_t)gridDim.x * kGBlock;
-    const uint64_t n_r = block_uniform_count<kGBlock>((uint32_t)f.npix);
-    for (uint64_t pix0 = (uint64_t)blockIdx.x * kGBlock + threadIdx.x; pix0 < n_r; pix0 += stride) {
+    const uint32_t tiles_x = (f.width + kFinishTile - 1) / kFinishTile;
+    const uint32_t tiles = tiles_x * ((f.height + kFinishTile - 1) / kFinishTile);
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         for (uint32_t i = threadIdx.x; i < kClaimSlots; i += kGBlock) claim[i] = 0u;
         __syncthreads();
-        const bool live = pix0 < f.npix;
-        const uint64_t pix = live ? pix0 : 0;
+        const uint32_t x = (tile % tiles_x) * kFinishTile + threadIdx.x % kFinishTile;
+        const uint32_t y = (tile / tiles_x) * kFinishTile + threadIdx.x / kFinishTile;
+        const bool live = x < f.width && y < f.height;
+        const uint64_t pix = live ? (uint64_t)y * f.width + x : 0;
         const GridKey k = grid_key(xyz(g.s_point[pix]), xyz(g.s_normal[pix]),
                                    xyz(g.s_start_dir[pix]), camera_loc, f.world_size);
         // The lanes of the block that write one cell: as in lock step, one
