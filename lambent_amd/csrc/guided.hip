@@ -112,8 +112,10 @@ __global__ __launch_bounds__(kGBlock) void k_g_generate(CameraDev cam, FrameDev 
             g.sum_mc[pix] = 0.0f;
             f.pix_sum[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
             g.out_rad_sum[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
-            g.pix_res[pix] = Reservoir{};
-            g.pix_sample[pix] = Reservoir{};
+            // The running reservoir lives in HBM only when every sample
+            // adds to it (per-sample reuse); the per-pixel sample is written
+            // whole by k_g_finish before k_g_resolve reads it.
+            if (p.restir == 1u) g.pix_res[pix] = Reservoir{};
             g.pix_pdf[pix] = 0.0f;
         } else {
             own_seed = f.own_seed[pix];
@@ -661,7 +663,7 @@ __global__ __launch_bounds__(kGBlock) void k_g_resolve(FrameDev f, GuidedDev g, 
         const float4 s4 = f.pix_sum[pix];
         f.out_color[pix] = make_float4(fmaxf(s4.x / ns, 0.f), fmaxf(s4.y / ns, 0.f),
                                        fmaxf(s4.z / ns, 0.f), 1.f);
-        Reservoir R = g.pix_res[pix];
+        Reservoir R = p.restir == 1u ? g.pix_res[pix] : Reservoir{};
         if (p.restir == 2u) {
             const Reservoir smp = g.pix_sample[pix];
             if (smp.ty != kMatSky && smp.packed_confidence_valid != 0u) {
