@@ -115,7 +115,6 @@ lmb_status ensure_guided(lmb_data_buffers b) {
         {(void**)&g.tent_snap, n * sizeof(MarkovChain)},
         {(void**)&g.sum_mc, n * 4},
         {(void**)&g.cand_seed, n * 4},
-        {(void**)&g.pix_pdf, n * 4},
         {(void**)&g.pix_res, n * sizeof(Reservoir)},
         {(void**)&g.pix_sample, n * sizeof(Reservoir)},
         {(void**)&g.out_rad_sum, n * 16},

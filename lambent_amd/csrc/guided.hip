@@ -116,7 +116,6 @@ __global__ __launch_bounds__(kGBlock) void k_g_generate(CameraDev cam, FrameDev 
             // adds to it (per-sample reuse); the per-pixel sample is written
             // whole by k_g_finish before k_g_resolve reads it.
             if (p.restir == 1u) g.pix_res[pix] = Reservoir{};
-            g.pix_pdf[pix] = 0.0f;
         } else {
             own_seed = f.own_seed[pix];
         }
@@ -582,8 +581,7 @@ __global__ __launch_bounds__(kGBlock) void k_g_finish(FrameDev f, GuidedDev g, T
             smp.ty = ty;
             smp.roughness = conv_rough<Q>(0.0f);
             smp.packed_confidence_valid = 1u;  // sample_valid
-            g.pix_sample[pix] = smp;
-            g.pix_pdf[pix] += sn.w;
+            g.pix_sample[pix] = smp;  // rt_main's pdf sum (:561) is never read
         } else if (p.restir == 1u) {  // RESTIR_OPTION_PER_SAMPLE
             Reservoir R = g.pix_res[pix];
             if (ty != kMatSky) {

@@ -20,7 +20,6 @@ struct GuidedDev {
     float4* out_rad_sum;     // rt_main's out_radiance accumulator
     Reservoir* pix_res;      // rt_main's reservoir R
     Reservoir* pix_sample;   // rt_main's per-pixel Sample (RESTIR per pixel)
-    float* pix_pdf;
     float4* pix_last_normal;    // xyz, w = ty bits
     float4* pix_last_albedo;
     float4* pix_last_emission;
