@@ -187,10 +187,12 @@ __device__ __forceinline__ void train_previous(const FrameDev& f, const GuidedDe
     }
 }
 
+// At most 128 VGPRs (4 waves per SIMD, one 8-byte spill for the spectral
+// and high qualities): 3 waves at 142 VGPRs were slower.
 template <uint32_t Q>
-__global__ __launch_bounds__(kGBlock) void k_g_shade(SceneDev s, FrameDev f, GuidedDev g,
-                                                     TraceParams p, uint32_t bounce,
-                                                     V3 camera_loc) {
+__global__ __launch_bounds__(kGBlock, 4) void k_g_shade(SceneDev s, FrameDev f, GuidedDev g,
+                                                        TraceParams p, uint32_t bounce,
+                                                        V3 camera_loc) {
     const uint64_t gtid = (uint64_t)blockIdx.x * kGBlock + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * kGBlock;
     const uint32_t count = f.counters[bounce];
