@@ -511,12 +511,15 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
 // per-sample tail of rt_main (general.wesl:535-581).
 // Claims of the grid cells the block writes in one pass of k_g_finish: a
 // small open table in LDS (key + 1, 0 = empty), one probe.
-constexpr uint32_t kClaimSlots = 128;
-// A pass of k_g_finish covers a square tile of pixels: grid cells are
-// coherent in screen space, so a cell's pixels fall into fewer blocks than
-// with row strips, and fewer blocks write each shared cell.
-constexpr uint32_t kFinishTile = 16;
-static_assert(kFinishTile * kFinishTile == kGBlock, "one lane per pixel of a finish tile");
+constexpr uint32_t kClaimSlots = 256;
+// A pass of k_g_finish covers a square tile of pixels in 16 x 16 sub-tiles,
+// one lane per pixel, with one claim table for the tile: grid cells are
+// coherent in screen space, so a cell's pixels fall into few tiles and few
+// blocks write each shared cell.
+constexpr uint32_t kFinishSub = 16;
+constexpr uint32_t kFinishTile = 32;
+static_assert(kFinishSub * kFinishSub == kGBlock, "one lane per pixel of a finish sub-tile");
+static_assert(kFinishTile % kFinishSub == 0, "whole sub-tiles per finish tile");
 
 template <uint32_t Q>
 __global__ __launch_bounds__(kGBlock) void k_g_finish(FrameDev f, GuidedDev g, TraceParams p,
@@ -527,97 +530,102 @@ __global__ __launch_bounds__(kGBlock) void k_g_finish(FrameDev f, GuidedDev g, T
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         for (uint32_t i = threadIdx.x; i < kClaimSlots; i += kGBlock) claim[i] = 0u;
         __syncthreads();
-        const uint32_t x = (tile % tiles_x) * kFinishTile + threadIdx.x % kFinishTile;
-        const uint32_t y = (tile / tiles_x) * kFinishTile + threadIdx.x / kFinishTile;
-        const bool live = x < f.width && y < f.height;
-        const uint64_t pix = live ? (uint64_t)y * f.width + x : 0;
-        const GridKey k = grid_key(xyz(g.s_point[pix]), xyz(g.s_normal[pix]),
-                                   xyz(g.s_start_dir[pix]), camera_loc, f.world_size);
-        // The lanes of the block that write one cell: as in lock step, one
-        // lane's writes land; the first lane of each wave claims the cell in
-        // the block's table, and a wave whose claim finds the cell already
-        // taken leaves it (a different cell in the slot: it writes anyway).
-        bool writer = false;
-        if (live && first_writer_of_cell(k)) {
-            const uint32_t prev = atomicCAS(&claim[k.idx % kClaimSlots], 0u, k.idx + 1u);
-            writer = prev == 0u || prev != k.idx + 1u;
-        }
-        __syncthreads();
-        if (!live) continue;
-        uint32_t seed = f.ps_seed[pix];
-        const float4 sp = g.s_point[pix], sn = g.s_normal[pix];
-        const V3 point = xyz(sp), normal = xyz(sn);
-        const V3 vis_point = xyz(g.s_vis_point[pix]);
-        const V3 sout = xyz(g.sout[pix]);
-        const Chain wc = grid_load(f.wmc, f.wmc_old, k, false);
-        const Chain st = update_markov_state(point, vis_point, length3(sout), seed, false, wc, 1.0f,
-                                             g.sum_mc[pix]);
-        if (writer) {
-            grid_store(f.wmc, k, st);
-            grid_add_radiance(f.wmc, k, sout);
-        }
-        // rt_main
-        const float4 ps = f.pix_sum[pix];
-        const V3 scol = xyz(g.scol[pix]);
-        f.pix_sum[pix] = make_float4(ps.x + scol.x, ps.y + scol.y, ps.z + scol.z, 0.f);
-        const float4 ors = g.out_rad_sum[pix];
-        const V3 out_rad = xyz(ors) + sout;
-        g.out_rad_sum[pix] = f4(out_rad, 0.f);
-        uint32_t own_seed = rand_u32(f.own_seed[pix]);
-        const uint32_t ty = __float_as_uint(sp.w);
-        if (p.restir == 2u) {  // RESTIR_OPTION_PER_PIXEL: remember the last sample
-            Reservoir smp{};
-            const V3 vp = vis_point, vn = xyz(g.s_vis_normal[pix]);
-            smp.visible_point[0] = vp.x;
-            smp.visible_point[1] = vp.y;
-            smp.visible_point[2] = vp.z;
-            smp.visible_normal = normalised_to_u32(vn);
-            smp.sample_point[0] = point.x;
-            smp.sample_point[1] = point.y;
-            smp.sample_point[2] = point.z;
-            smp.sample_normal = normalised_to_u32(normal);
-            const V3 L = out_rad / (float)p.samples;
-            smp.out_radiance[0] = L.x;
-            smp.out_radiance[1] = L.y;
-            smp.out_radiance[2] = L.z;
-            smp.ty = ty;
-            smp.roughness = conv_rough<Q>(0.0f);
-            smp.packed_confidence_valid = 1u;  // sample_valid
-            g.pix_sample[pix] = smp;  // rt_main's pdf sum (:561) is never read
-        } else if (p.restir == 1u) {  // RESTIR_OPTION_PER_SAMPLE
-            Reservoir R = g.pix_res[pix];
-            if (ty != kMatSky) {
+        constexpr uint32_t kSubs = kFinishTile / kFinishSub;
+        for (uint32_t sub = 0; sub < kSubs * kSubs; ++sub) {
+            const uint32_t x = (tile % tiles_x) * kFinishTile + (sub % kSubs) * kFinishSub +
+                               threadIdx.x % kFinishSub;
+            const uint32_t y = (tile / tiles_x) * kFinishTile + (sub / kSubs) * kFinishSub +
+                               threadIdx.x / kFinishSub;
+            const bool live = x < f.width && y < f.height;
+            const uint64_t pix = live ? (uint64_t)y * f.width + x : 0;
+            const GridKey k = grid_key(xyz(g.s_point[pix]), xyz(g.s_normal[pix]),
+                                       xyz(g.s_start_dir[pix]), camera_loc, f.world_size);
+            // The lanes of the tile that write one cell: as in lock step, one
+            // lane's writes land; the first lane of each wave claims the cell in
+            // the tile's table, and a wave whose claim finds the cell already
+            // taken leaves it (a different cell in the slot: it writes anyway).
+            bool writer = false;
+            if (live && first_writer_of_cell(k)) {
+                const uint32_t prev = atomicCAS(&claim[k.idx % kClaimSlots], 0u, k.idx + 1u);
+                writer = prev == 0u || prev != k.idx + 1u;
+            }
+            if (!live) continue;
+            uint32_t seed = f.ps_seed[pix];
+            const float4 sp = g.s_point[pix], sn = g.s_normal[pix];
+            const V3 point = xyz(sp), normal = xyz(sn);
+            const V3 vis_point = xyz(g.s_vis_point[pix]);
+            const V3 sout = xyz(g.sout[pix]);
+            const Chain wc = grid_load(f.wmc, f.wmc_old, k, false);
+            const Chain st = update_markov_state(point, vis_point, length3(sout), seed, false, wc, 1.0f,
+                                                 g.sum_mc[pix]);
+            if (writer) {
+                grid_store(f.wmc, k, st);
+                grid_add_radiance(f.wmc, k, sout);
+            }
+            // rt_main
+            const float4 ps = f.pix_sum[pix];
+            const V3 scol = xyz(g.scol[pix]);
+            f.pix_sum[pix] = make_float4(ps.x + scol.x, ps.y + scol.y, ps.z + scol.z, 0.f);
+            const float4 ors = g.out_rad_sum[pix];
+            const V3 out_rad = xyz(ors) + sout;
+            g.out_rad_sum[pix] = f4(out_rad, 0.f);
+            uint32_t own_seed = rand_u32(f.own_seed[pix]);
+            const uint32_t ty = __float_as_uint(sp.w);
+            if (p.restir == 2u) {  // RESTIR_OPTION_PER_PIXEL: remember the last sample
                 Reservoir smp{};
-                smp.visible_point[0] = vis_point.x;
-                smp.visible_point[1] = vis_point.y;
-                smp.visible_point[2] = vis_point.z;
-                smp.visible_normal = normalised_to_u32(xyz(g.s_vis_normal[pix]));
+                const V3 vp = vis_point, vn = xyz(g.s_vis_normal[pix]);
+                smp.visible_point[0] = vp.x;
+                smp.visible_point[1] = vp.y;
+                smp.visible_point[2] = vp.z;
+                smp.visible_normal = normalised_to_u32(vn);
                 smp.sample_point[0] = point.x;
                 smp.sample_point[1] = point.y;
                 smp.sample_point[2] = point.z;
                 smp.sample_normal = normalised_to_u32(normal);
-                smp.out_radiance[0] = sout.x;
-                smp.out_radiance[1] = sout.y;
-                smp.out_radiance[2] = sout.z;
+                const V3 L = out_rad / (float)p.samples;
+                smp.out_radiance[0] = L.x;
+                smp.out_radiance[1] = L.y;
+                smp.out_radiance[2] = L.z;
                 smp.ty = ty;
                 smp.roughness = conv_rough<Q>(0.0f);
-                const float w = p_hat(smp);
-                R.w = R.w + w;
-                R.packed_confidence_valid =
-                    ((R.packed_confidence_valid + 1u) & kMaxConfidence) |
-                    (R.packed_confidence_valid & ~kMaxConfidence);
-                if (rand_f32(own_seed) * R.w < w) assign_sample(R, smp);
-                own_seed = rand_u32(own_seed);
-                R.W = safe_div(R.w, (float)confidence_of(R.packed_confidence_valid) * p_hat(R));
-            } else {
-                R.ty = kMatSky;
+                smp.packed_confidence_valid = 1u;  // sample_valid
+                g.pix_sample[pix] = smp;  // rt_main's pdf sum (:561) is never read
+            } else if (p.restir == 1u) {  // RESTIR_OPTION_PER_SAMPLE
+                Reservoir R = g.pix_res[pix];
+                if (ty != kMatSky) {
+                    Reservoir smp{};
+                    smp.visible_point[0] = vis_point.x;
+                    smp.visible_point[1] = vis_point.y;
+                    smp.visible_point[2] = vis_point.z;
+                    smp.visible_normal = normalised_to_u32(xyz(g.s_vis_normal[pix]));
+                    smp.sample_point[0] = point.x;
+                    smp.sample_point[1] = point.y;
+                    smp.sample_point[2] = point.z;
+                    smp.sample_normal = normalised_to_u32(normal);
+                    smp.out_radiance[0] = sout.x;
+                    smp.out_radiance[1] = sout.y;
+                    smp.out_radiance[2] = sout.z;
+                    smp.ty = ty;
+                    smp.roughness = conv_rough<Q>(0.0f);
+                    const float w = p_hat(smp);
+                    R.w = R.w + w;
+                    R.packed_confidence_valid =
+                        ((R.packed_confidence_valid + 1u) & kMaxConfidence) |
+                        (R.packed_confidence_valid & ~kMaxConfidence);
+                    if (rand_f32(own_seed) * R.w < w) assign_sample(R, smp);
+                    own_seed = rand_u32(own_seed);
+                    R.W = safe_div(R.w, (float)confidence_of(R.packed_confidence_valid) * p_hat(R));
+                } else {
+                    R.ty = kMatSky;
+                }
+                g.pix_res[pix] = R;
             }
-            g.pix_res[pix] = R;
+            f.own_seed[pix] = own_seed;
+            g.pix_last_normal[pix] = f4(normal, __uint_as_float(ty));
+            g.pix_last_albedo[pix] = g.s_albedo[pix];
+            g.pix_last_emission[pix] = g.s_emission[pix];
         }
-        f.own_seed[pix] = own_seed;
-        g.pix_last_normal[pix] = f4(normal, __uint_as_float(ty));
-        g.pix_last_albedo[pix] = g.s_albedo[pix];
-        g.pix_last_emission[pix] = g.s_emission[pix];
+        __syncthreads();  // every claim of this tile made before the table is cleared
     }
 }
 
