@@ -301,4 +301,91 @@ __device__ __forceinline__ float vmf_pdf(V3 mean, float kappa, V3 dir) {
     return fmaxf(fraction * expf(kappa * (c - 1.0f)), 0.0f);
 }
 
+// general.wesl:797-808: Jakob (2012) vMF sample about `mean` from the draws
+// at seed (azimuth) and seed + 1 (cosine); the unnormalised Xs, whose length
+// the caller tests.  The caller advances its seed by 2.
+__device__ __forceinline__ V3 vmf_sample(V3 mean, float kappa, uint32_t seed) {
+    const float theta = rand_f32(seed) * kTau;
+    const float vx = cosf(theta), vy = sinf(theta);
+    const float xi = rand_f32(seed + 1u);
+    const float m2k = -2.0f * kappa;
+    const float wm1 = logf(1.0f + xi * exp_minus_one(m2k)) / kappa;
+    const float W = 1.0f + wm1;
+    const float norm_v = sqrtf(-fmaf(wm1, wm1, 2.0f * wm1));
+    return from_onb_no_tangent(v3(strip_nan(vx * norm_v), W, strip_nan(vy * norm_v)), mean);
+}
+
+// One candidate of resample_mcmm (general.wesl:408-421): advances the seed
+// twice and the running weight sum; true when the candidate replaces the
+// selection.
+__device__ __forceinline__ bool resample_step(const Chain& cand, uint32_t& seed, float& sum_mc) {
+    seed = rand_u32(seed);
+    sum_mc += cand.weight_sum;
+    const bool take = rand_f32(seed) * sum_mc < cand.weight_sum;
+    seed = rand_u32(seed);
+    return take;
+}
+
+// One candidate's term of all_vmf_pdfs (general.wesl:989-995).
+__device__ __forceinline__ float mixture_term(const Chain& cand, V3 origin, V3 d, float sum_mc) {
+    if (!(cand.weight_sum > 0.0f)) return 0.0f;
+    return vmf_pdf(normalize3(get_light_dir(cand, origin)), get_kappa(cand), d) *
+           (cand.weight_sum / sum_mc);
+}
+
+// Screen-space candidates of general.wesl:367-385 drawn from the seed the
+// fill starts with: the lane picked 4x (the first loop does not advance the
+// seed) and the 12 previous-frame neighbours in a 91 x 91 window.  A
+// logical subgroup is `subgroup` consecutive pixels of a row.
+struct ScreenCands {
+    uint32_t lane_pix;  // pixel whose sample-start tentative chain is shuffled in
+    bool lane_ok;
+    uint32_t idx[kScreenLastFrame];
+    uint32_t seed_after;
+};
+
+__device__ __forceinline__ ScreenCands screen_candidates(uint32_t seed, uint32_t x, uint32_t y,
+                                                         uint32_t width, uint32_t subgroup) {
+    ScreenCands c;
+    const uint32_t base = (x / subgroup) * subgroup;
+    const uint32_t lane = rand_u32(seed) % subgroup;
+    c.lane_pix = y * width + base + lane;
+    c.lane_ok = base + lane < width;
+#pragma unroll
+    for (uint32_t i = 0; i < kScreenLastFrame; ++i) {
+        const uint32_t xr = rand_u32(seed);
+        seed = xr;
+        const uint32_t yr = rand_u32(seed);
+        seed = yr;
+        const int32_t dx = (int32_t)(xr % kResampleSize) - (int32_t)(kResampleSize / 2);
+        const int32_t dy = (int32_t)(yr % kResampleSize) - (int32_t)(kResampleSize / 2);
+        const uint32_t nx = (uint32_t)((int32_t)x + dx), ny = (uint32_t)((int32_t)y + dy);
+        c.idx[i] = nx + ny * width;  // u32 wrap-around as in the reference
+    }
+    c.seed_after = seed;
+    return c;
+}
+
+// A grid key as one 64-bit word (the secondary hash is odd, so never 0).
+__device__ __forceinline__ uint64_t key_word(const GridKey& k) {
+    return ((uint64_t)k.secondary << 32) | k.idx;
+}
+
+// Whether this lane is the lowest active lane of its wave holding key k
+// (index and secondary hash).  In the reference's lock-step execution the
+// lanes of a wave that write one cell have one lane's stores land and one
+// lane take its try-lock; lanes whose keys share an index but not a
+// secondary hash are different keys, and each keeps its own writer.
+__device__ __forceinline__ bool first_writer_of_key(const GridKey& k) {
+    bool first = true;
+    const uint64_t live = __ballot(1);
+    for (int l = 0; l < 64; ++l) {
+        const uint32_t oi = __shfl(k.idx, l);
+        const uint32_t os = __shfl(k.secondary, l);
+        if (l < (int)__lane_id() && ((live >> l) & 1ull) && oi == k.idx && os == k.secondary)
+            first = false;
+    }
+    return first;
+}
+
 }  // namespace lmb

@@ -61,38 +61,6 @@ __device__ __forceinline__ MarkovChain load_mc_or_zero(const MarkovChain* p, uin
     return z;
 }
 
-// Screen-space candidates of general.wesl:367-385 regenerated from the seed
-// the fill started with: the lane picked 4x (the seed is not advanced in the
-// first loop) and the 12 previous-frame neighbours in a 91 x 91 window.
-struct ScreenCands {
-    uint32_t lane_pix;  // pixel whose sample-start tentative chain is shuffled in
-    bool lane_ok;
-    uint32_t idx[kScreenLastFrame];
-    uint32_t seed_after;
-};
-
-__device__ __forceinline__ ScreenCands screen_candidates(uint32_t seed, uint32_t x, uint32_t y,
-                                                         uint32_t width) {
-    ScreenCands c;
-    const uint32_t base = (x / kSubgroup) * kSubgroup;
-    const uint32_t lane = rand_u32(seed) % kSubgroup;
-    c.lane_pix = y * width + base + lane;
-    c.lane_ok = base + lane < width;
-#pragma unroll
-    for (uint32_t i = 0; i < kScreenLastFrame; ++i) {
-        const uint32_t xr = rand_u32(seed);
-        seed = xr;
-        const uint32_t yr = rand_u32(seed);
-        seed = yr;
-        const int32_t dx = (int32_t)(xr % kResampleSize) - (int32_t)(kResampleSize / 2);
-        const int32_t dy = (int32_t)(yr % kResampleSize) - (int32_t)(kResampleSize / 2);
-        const uint32_t nx = (uint32_t)((int32_t)x + dx), ny = (uint32_t)((int32_t)y + dy);
-        c.idx[i] = nx + ny * width;  // u32 wrap-around as in the reference
-    }
-    c.seed_after = seed;
-    return c;
-}
-
 __global__ __launch_bounds__(kGBlock) void k_g_generate(CameraDev cam, FrameDev f, GuidedDev g,
                                                         TraceParams p, uint32_t sample) {
     const uint64_t gtid = (uint64_t)blockIdx.x * kGBlock + threadIdx.x;
@@ -121,7 +89,7 @@ __global__ __launch_bounds__(kGBlock) void k_g_generate(CameraDev cam, FrameDev 
         }
         g.tent[pix] = g.tent_snap[pix];
         g.cand_seed[pix] = own_seed;
-        const ScreenCands sc = screen_candidates(own_seed, x, y, f.width);
+        const ScreenCands sc = screen_candidates(own_seed, x, y, f.width, kSubgroup);
         own_seed = sc.seed_after;
         f.own_seed[pix] = own_seed;
         float o[3], d[3];
@@ -144,22 +112,6 @@ __global__ __launch_bounds__(kGBlock) void k_g_generate(CameraDev cam, FrameDev 
         g.s_vis_normal[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
         g.s_start_dir[pix] = make_float4(d[0], d[1], d[2], 0.f);
     }
-}
-
-// Whether this lane is the lowest active lane of its wave that writes grid
-// cell k.  Lanes of a wave that write one cell: in lock step their stores
-// land as one lane's and only one lane takes the cell's try-lock, so only
-// the lowest such lane need write (lane by lane, same-address atomics queue
-// in L2: every pixel whose path went to the sky shares one cell in the final
-// update; the per-bounce updates, at distinct hit points, gain nothing).
-__device__ __forceinline__ bool first_writer_of_cell(const GridKey& k) {
-    bool first = true;
-    const uint64_t live = __ballot(1);
-    for (int l = 0; l < 64; ++l) {
-        const uint32_t other = __shfl(k.idx, l);
-        if (l < (int)__lane_id() && ((live >> l) & 1ull) && other == k.idx) first = false;
-    }
-    return first;
 }
 
 // World-grid training of the previous vertex when the path reaches
@@ -355,7 +307,7 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
                 }
                 const Chain tent = chain_load(g.tent[pid]);
                 const uint32_t x = pid % f.width, y = pid / f.width;
-                const ScreenCands sc = screen_candidates(g.cand_seed[pid], x, y, f.width);
+                const ScreenCands sc = screen_candidates(g.cand_seed[pid], x, y, f.width, kSubgroup);
                 const Chain lane_chain = sc.lane_ok ? chain_load(g.tent_snap[sc.lane_pix]) : chain_zero();
                 // resample_mcmm (general.wesl:404-425).  Candidate i: the
                 // shuffled lane chain (i < 4), last frame's screen chains
@@ -365,10 +317,7 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
                 Chain sel = chain_zero();
                 float sum_mc = 0.0f;
                 auto resample = [&](const Chain& cand) {
-                    seed = rand_u32(seed);
-                    sum_mc += cand.weight_sum;
-                    if (rand_f32(seed) * sum_mc < cand.weight_sum) sel = cand;
-                    seed = rand_u32(seed);
+                    if (resample_step(cand, seed, sum_mc)) sel = cand;
                 };
                 for (uint32_t i = 0; i < kScreenResamples; ++i) resample(lane_chain);
 #pragma unroll
@@ -396,17 +345,8 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
                         const float kappa = get_kappa(sel);
                         const V3 dir_cos = cosine_weighted_hemisphere(seed, inormal, tangent);
                         const float cos_pdf = fmaxf(dot3(dir_cos, inormal) / kPi, 0.0f);
-                        const float theta = rand_f32(seed) * kTau;
-                        const float vx = cosf(theta), vy = sinf(theta);
-                        seed += 1u;
-                        const float xi = rand_f32(seed);
-                        seed += 1u;
-                        const float m2k = -2.0f * kappa;
-                        const float wm1 = logf(1.0f + xi * exp_minus_one(m2k)) / kappa;
-                        const float W = 1.0f + wm1;
-                        const float norm_v = sqrtf(-fmaf(wm1, wm1, 2.0f * wm1));
-                        const V3 Xs = from_onb_no_tangent(
-                            v3(strip_nan(vx * norm_v), W, strip_nan(vy * norm_v)), mean);
+                        const V3 Xs = vmf_sample(mean, kappa, seed);
+                        seed += 2u;
                         const V3 dir_markov = normalize3(Xs);
                         float c0 = 0.2f, c1 = 0.8f;
                         if (!(sel.weight_sum > 0.0f) || near_zero(length3(light_dir)) ||
@@ -430,10 +370,7 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
                         if (conf1 != 0.0f && sum_mc > 0.0f) {
                             // the candidates in resample_mcmm's order
                             auto add_pdf = [&](const Chain& cand) {
-                                if (!(cand.weight_sum > 0.0f)) return;
-                                mix_pdf += vmf_pdf(normalize3(get_light_dir(cand, origin)),
-                                                   get_kappa(cand), d) *
-                                           (cand.weight_sum / sum_mc);
+                                mix_pdf += mixture_term(cand, origin, d, sum_mc);
                             };
                             for (uint32_t i = 0; i < kScreenResamples; ++i) add_pdf(lane_chain);
 #pragma unroll
@@ -509,13 +446,19 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
 
 // Final world-grid update of the sample (general.wesl:944-948) and the
 // per-sample tail of rt_main (general.wesl:535-581).
-// Claims of the grid cells the block writes in one pass of k_g_finish: a
-// small open table in LDS (key + 1, 0 = empty), one probe.
+//
+// Writers of a shared cell.  In the reference's lock-step execution, the
+// lanes of a wave that write one cell have one lane's stores land and one
+// lane take the cell's try-lock: one write per wave and key.  That is kept
+// for every cell but one: the cell of the pixels whose camera ray missed
+// (point and normal zero, general.wesl:649-652), which every sky pixel of the
+// frame shares; lane by lane its same-address atomics queue in L2 (7.3 ms of
+// a 1080p frame).  For that cell the waves of a pixel tile claim it in a
+// small LDS table (the 64-bit key, 0 = empty, one probe), so it is written
+// once per tile instead of once per wave.
 constexpr uint32_t kClaimSlots = 256;
 // A pass of k_g_finish covers a square tile of pixels in 16 x 16 sub-tiles,
-// one lane per pixel, with one claim table for the tile: grid cells are
-// coherent in screen space, so a cell's pixels fall into few tiles and few
-// blocks write each shared cell.
+// one lane per pixel, with one claim table for the tile.
 constexpr uint32_t kFinishSub = 16;
 constexpr uint32_t kFinishTile = 32;
 static_assert(kFinishSub * kFinishSub == kGBlock, "one lane per pixel of a finish sub-tile");
@@ -524,11 +467,11 @@ static_assert(kFinishTile % kFinishSub == 0, "whole sub-tiles per finish tile");
 template <uint32_t Q>
 __global__ __launch_bounds__(kGBlock) void k_g_finish(FrameDev f, GuidedDev g, TraceParams p,
                                                       V3 camera_loc) {
-    __shared__ uint32_t claim[kClaimSlots];
+    __shared__ unsigned long long claim[kClaimSlots];
     const uint32_t tiles_x = (f.width + kFinishTile - 1) / kFinishTile;
     const uint32_t tiles = tiles_x * ((f.height + kFinishTile - 1) / kFinishTile);
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        for (uint32_t i = threadIdx.x; i < kClaimSlots; i += kGBlock) claim[i] = 0u;
+        for (uint32_t i = threadIdx.x; i < kClaimSlots; i += kGBlock) claim[i] = 0ull;
         __syncthreads();
         constexpr uint32_t kSubs = kFinishTile / kFinishSub;
         for (uint32_t sub = 0; sub < kSubs * kSubs; ++sub) {
@@ -538,16 +481,18 @@ __global__ __launch_bounds__(kGBlock) void k_g_finish(FrameDev f, GuidedDev g, T
                                threadIdx.x / kFinishSub;
             const bool live = x < f.width && y < f.height;
             const uint64_t pix = live ? (uint64_t)y * f.width + x : 0;
-            const GridKey k = grid_key(xyz(g.s_point[pix]), xyz(g.s_normal[pix]),
-                                       xyz(g.s_start_dir[pix]), camera_loc, f.world_size);
-            // The lanes of the tile that write one cell: as in lock step, one
-            // lane's writes land; the first lane of each wave claims the cell in
-            // the tile's table, and a wave whose claim finds the cell already
-            // taken leaves it (a different cell in the slot: it writes anyway).
+            const float4 sp0 = g.s_point[pix];
+            const GridKey k = grid_key(xyz(sp0), xyz(g.s_normal[pix]), xyz(g.s_start_dir[pix]),
+                                       camera_loc, f.world_size);
             bool writer = false;
-            if (live && first_writer_of_cell(k)) {
-                const uint32_t prev = atomicCAS(&claim[k.idx % kClaimSlots], 0u, k.idx + 1u);
-                writer = prev == 0u || prev != k.idx + 1u;
+            if (live && first_writer_of_key(k)) {
+                if (__float_as_uint(sp0.w) == kMatSky) {
+                    const unsigned long long kw = key_word(k);
+                    const unsigned long long prev = atomicCAS(&claim[k.idx % kClaimSlots], 0ull, kw);
+                    writer = prev == 0ull || prev != kw;  // another key in the slot: write anyway
+                } else {
+                    writer = true;
+                }
             }
             if (!live) continue;
             uint32_t seed = f.ps_seed[pix];

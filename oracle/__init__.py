@@ -432,3 +432,136 @@ def spectral_pick(picked: float, lo: float, hi: float, seed: int):
     c = np.empty(3, np.float32)
     ri = _lib.orc_spectral_pick(_f(p), lo, hi, seed, _f(c))
     return float(ri), c, float(p[0])
+
+
+# ---------------------------------------------------------------- guided-mode hooks
+_sig("orc_strip_nan", [ctypes.c_float], ctypes.c_float)
+_sig("orc_grid_key", [_f32p, _f32p, _f32p, _f32p, ctypes.c_uint32, _u32p])
+_sig("orc_get_kappa", [_u32p], ctypes.c_float)
+_sig("orc_expm1", [ctypes.c_float], ctypes.c_float)
+_sig("orc_x_div_expm1", [ctypes.c_float], ctypes.c_float)
+_sig("orc_vmf_pdf_many", [_f32p, ctypes.c_float, _f32p, ctypes.c_uint64, _f32p])
+_sig("orc_vmf_sample_many", [_f32p, ctypes.c_float, _u32p, ctypes.c_uint64, _f32p])
+_sig("orc_update_state", [_f32p, _f32p, ctypes.c_float, ctypes.c_uint32, ctypes.c_int, _u32p,
+                          ctypes.c_float, ctypes.c_float, _u32p])
+_sig("orc_resample_mcmm", [_u32p, _u32p, _f32p], ctypes.c_int)
+_sig("orc_all_vmf_pdfs", [_u32p, ctypes.c_float, _f32p, _f32p], ctypes.c_float)
+_sig("orc_grid_op", [ctypes.c_int, _vp, _vp, ctypes.c_uint32, _f32p, _f32p, _f32p, _f32p,
+                     ctypes.c_int, _u32p, _f32p, _u32p, _f32p])
+
+NUM_RESAMPLES = 21
+
+
+def _v3(a):
+    return np.ascontiguousarray(a, np.float32).reshape(3)
+
+
+def chain_words(light_source=(0.0, 0.0, 0.0), mean_cosine=0.0, weight_sum=0.0, num_samples=0,
+                score=0.0) -> np.ndarray:
+    """One MarcovChainState (bindings.wesl:3-9) as 8 u32 words (last = pad)."""
+    w = np.zeros(8, np.uint32)
+    w[:5] = np.array([*light_source, mean_cosine, weight_sum], np.float32).view(np.uint32)
+    w[5] = num_samples
+    w[6] = np.float32(score).view(np.uint32)
+    return w
+
+
+def chain_fields(w) -> dict:
+    w = np.asarray(w, np.uint32)
+    f = w[:7].view(np.float32)
+    return {"light_source": f[:3].copy(), "mean_cosine": float(f[3]), "weight_sum": float(f[4]),
+            "num_samples": int(w[5]), "score": float(f[6])}
+
+
+def strip_nan(x: float) -> float:
+    return float(_lib.orc_strip_nan(x))
+
+
+def grid_key(pos, normal, incoming, cam, table: int) -> dict:
+    out = np.zeros(6, np.uint32)
+    _lib.orc_grid_key(_f(_v3(pos)), _f(_v3(normal)), _f(_v3(incoming)), _f(_v3(cam)), table,
+                      _u(out))
+    return {"idx": int(out[0]), "secondary": int(out[1]), "primary": int(out[2]),
+            "loc": out[3:6].view(np.int32).copy()}
+
+
+def get_kappa(chain) -> float:
+    return float(_lib.orc_get_kappa(_u(np.ascontiguousarray(chain, np.uint32))))
+
+
+def expm1(x: float) -> float:
+    return float(_lib.orc_expm1(x))
+
+
+def x_div_expm1(x: float) -> float:
+    return float(_lib.orc_x_div_expm1(x))
+
+
+def vmf_pdf(mean, kappa: float, dirs) -> np.ndarray:
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    out = np.empty(len(d), np.float32)
+    _lib.orc_vmf_pdf_many(_f(_v3(mean)), kappa, _f(d), len(d), _f(out))
+    return out
+
+
+def vmf_sample(mean, kappa: float, seeds) -> np.ndarray:
+    """Unnormalised Xs of general.wesl:797-808, one per seed."""
+    s = np.ascontiguousarray(seeds, np.uint32).reshape(-1)
+    out = np.empty((len(s), 3), np.float32)
+    _lib.orc_vmf_sample_many(_f(_v3(mean)), kappa, _u(s), len(s), _f(out))
+    return out
+
+
+def update_state(x1, x2, p_hat: float, seed: int, not_vmf: bool, chain, confidence: float,
+                 sum_mc: float) -> np.ndarray:
+    out = np.zeros(8, np.uint32)
+    _lib.orc_update_state(_f(_v3(x1)), _f(_v3(x2)), p_hat, seed, int(not_vmf),
+                          _u(np.ascontiguousarray(chain, np.uint32)), confidence, sum_mc, _u(out))
+    return out
+
+
+def resample_mcmm(chains, seed: int):
+    """(selected index or -1, seed after, sum_mc) over 21 chains."""
+    c = np.ascontiguousarray(chains, np.uint32).reshape(NUM_RESAMPLES, 8)
+    s = np.array([seed], np.uint32)
+    sm = np.zeros(1, np.float32)
+    k = _lib.orc_resample_mcmm(_u(c), _u(s), _f(sm))
+    return int(k), int(s[0]), float(sm[0])
+
+
+def all_vmf_pdfs(chains, sum_mc: float, hit, d) -> float:
+    c = np.ascontiguousarray(chains, np.uint32).reshape(NUM_RESAMPLES, 8)
+    return float(_lib.orc_all_vmf_pdfs(_u(c), sum_mc, _f(_v3(hit)), _f(_v3(d))))
+
+
+GRID_LOAD, GRID_STORE, GRID_RADIANCE, GRID_ADD = 0, 1, 2, 3
+
+
+def grid_op(op: int, world: np.ndarray, world_old: np.ndarray, cam, pos, normal, incoming,
+            force_old=False, chain=None, radiance=None):
+    """One sequential world-grid call on WORLD_DTYPE arrays (world in place)."""
+    assert world.dtype == WORLD_DTYPE and world_old.dtype == WORLD_DTYPE
+    assert len(world) == len(world_old)
+    cin = np.ascontiguousarray(chain if chain is not None else np.zeros(8), np.uint32)
+    rin = _v3(radiance if radiance is not None else (0, 0, 0))
+    cout = np.zeros(8, np.uint32)
+    rout = np.zeros(3, np.float32)
+    _lib.orc_grid_op(op, world.ctypes.data_as(_vp), world_old.ctypes.data_as(_vp), len(world),
+                     _f(_v3(cam)), _f(_v3(pos)), _f(_v3(normal)), _f(_v3(incoming)), int(force_old),
+                     _u(cin), _f(rin), _u(cout), _f(rout))
+    if op == GRID_LOAD:
+        return cout
+    if op == GRID_RADIANCE:
+        return rout
+    return None
+
+
+_sig("orc_screen_draw", [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                         ctypes.c_uint32, _u32p])
+
+
+def screen_draw(seed: int, x: int, y: int, width: int, subgroup: int = 64) -> np.ndarray:
+    """fill_mc_screen_space_states' draws: [lane pixel, lane valid, 12 indices, seed after]."""
+    out = np.zeros(15, np.uint32)
+    _lib.orc_screen_draw(seed, x, y, width, subgroup, _u(out))
+    return out

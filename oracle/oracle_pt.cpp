@@ -588,6 +588,8 @@ int32_t f2i_sat(float f) {
 
 struct GKey {
     uint32_t idx, secondary;
+    uint32_t primary;   // before the modulo
+    uint32_t loc[3];    // masked level-of-detail cell (i32 bits)
 };
 
 // general.wesl:148-188: level-of-detail cell, normal and incoming octant
@@ -618,7 +620,7 @@ GKey grid_key(F3 pos, F3 normal, F3 incoming, F3 cam, uint32_t table) {
                        (inc * 840057787u);
     uint32_t s3 = ((lx * 1699644091u) ^ (ly * 550105817u)) ^ (lz * 956114303u);
     uint32_t secondary = ((orc_rand_u32(s3) ^ (n * 155096983u)) ^ 0u) | 1u;
-    return GKey{primary % table, secondary};
+    return GKey{primary % table, secondary, primary, {lx, ly, lz}};
 }
 
 struct Grid {
@@ -761,6 +763,81 @@ F3 from_onb_no_tangent(F3 src, F3 up) {
     return from_onb(src, up, normalize(cross(up, other)));
 }
 
+// general.wesl:797-808: Jakob (2012) vMF sample around `mean` from the two
+// draws at seed and seed + 1; returns the unnormalised Xs (the caller
+// normalises it and tests its length).
+F3 vmf_sample(F3 mean, float kappa, uint32_t seed) {
+    float theta = orc_rand_f32(seed) * TAU;
+    float vx = std::cos(theta), vy = std::sin(theta);
+    float xi = orc_rand_f32(seed + 1u);
+    float wm1 = std::log(1.0f + xi * expm1_ref(-2.0f * kappa)) / kappa;
+    float W = 1.0f + wm1;
+    float norm_v = std::sqrt(-std::fma(wm1, wm1, 2.0f * wm1));
+    return from_onb_no_tangent(mk(strip_nan(vx * norm_v), W, strip_nan(vy * norm_v)), mean);
+}
+
+constexpr int kNumResamples = 21;  // general.wesl:341-345
+
+// general.wesl:404-425 resample_mcmm: one pass of weighted reservoir
+// selection over the 21 candidates; returns the selected index (-1 = none,
+// the zero chain) and leaves the running weight sum in *sum_mc.
+int resample_mcmm(const Chain* all, uint32_t* seed, float* sum_mc) {
+    int sel = -1;
+    float sum = 0.0f;
+    uint32_t s = *seed;
+    for (int k = 0; k < kNumResamples; ++k) {
+        s = orc_rand_u32(s);
+        sum += all[k].weight_sum;
+        if (orc_rand_f32(s) * sum < all[k].weight_sum) sel = k;
+        s = orc_rand_u32(s);
+    }
+    *seed = s;
+    *sum_mc = sum;
+    return sel;
+}
+
+// general.wesl:984-998 all_vmf_pdfs: the mixture pdf of direction d over the
+// valid candidates, each weighted by weight_sum / sum_mc.
+float all_vmf_pdfs(const Chain* all, float sum_mc, F3 hit, F3 d) {
+    float pdfs = 0.0f;
+    if (sum_mc > 0.0f) {
+        for (int k = 0; k < kNumResamples; ++k) {
+            if (!(all[k].weight_sum > 0.0f)) continue;
+            pdfs += vmf_pdf(normalize(light_dir(all[k], hit)), get_kappa(all[k]), d) *
+                    (all[k].weight_sum / sum_mc);
+        }
+    }
+    return pdfs;
+}
+
+// fill_mc_screen_space_states (general.wesl:367-385): the lane drawn once
+// and used 4x (the first loop does not advance the seed), then 12
+// previous-frame pixels in a 91 x 91 window, indices wrapping as u32.
+struct ScreenDraw {
+    uint32_t lane_pix;
+    bool lane_ok;
+    uint32_t idx[12];
+    uint32_t seed_after;
+};
+ScreenDraw screen_draw(uint32_t seed, uint32_t x, uint32_t y, uint32_t width, uint32_t subgroup) {
+    ScreenDraw d;
+    uint32_t base = (x / subgroup) * subgroup;
+    uint32_t lane = orc_rand_u32(seed) % subgroup;
+    d.lane_pix = y * width + base + lane;
+    d.lane_ok = base + lane < width;
+    for (int k = 0; k < 12; ++k) {
+        uint32_t xr = orc_rand_u32(seed);
+        seed = xr;
+        uint32_t yr = orc_rand_u32(seed);
+        seed = yr;
+        int32_t dx = (int32_t)(xr % 91u) - 45, dy = (int32_t)(yr % 91u) - 45;
+        uint32_t nx = (uint32_t)((int32_t)x + dx), ny = (uint32_t)((int32_t)y + dy);
+        d.idx[k] = nx + ny * width;
+    }
+    d.seed_after = seed;
+    return d;
+}
+
 struct GuidedOpts {
     uint32_t quality, bounces, rr_start, samples, restir, subgroup;
     float tmin, tmax;
@@ -871,15 +948,8 @@ SampleRet rt_sample_guided(const Scene& s, Grid& grid, const GuidedOpts& o, F3 o
                 self_seed++;
             }
             all[20] = tent;
-            // resample_mcmm (general.wesl:404-425)
-            Chain sel = czero();
-            sum_mc = 0.0f;
-            for (int k = 0; k < 21; ++k) {
-                self_seed = orc_rand_u32(self_seed);
-                sum_mc += all[k].weight_sum;
-                if (orc_rand_f32(self_seed) * sum_mc < all[k].weight_sum) sel = all[k];
-                self_seed = orc_rand_u32(self_seed);
-            }
+            const int sel_k = resample_mcmm(all, &self_seed, &sum_mc);
+            const Chain sel = sel_k < 0 ? czero() : all[sel_k];
             bool reflecting = should_reflect || it.ty == 1u;
             if (!reflecting) {
                 if (it.ty == 0u) {
@@ -888,15 +958,8 @@ SampleRet rt_sample_guided(const Scene& s, Grid& grid, const GuidedOpts& o, F3 o
                     float kappa = get_kappa(sel);
                     F3 dir_cos = cosine_weighted_hemisphere(self_seed, it.normal, it.tangent);
                     float cos_pdf = std::fmax(dot(dir_cos, it.normal) / PI, 0.0f);
-                    float theta = orc_rand_f32(self_seed) * TAU;
-                    float vx = std::cos(theta), vy = std::sin(theta);
-                    self_seed += 1u;
-                    float xi = orc_rand_f32(self_seed);
-                    self_seed += 1u;
-                    float wm1 = std::log(1.0f + xi * expm1_ref(-2.0f * kappa)) / kappa;
-                    float W = 1.0f + wm1;
-                    float norm_v = std::sqrt(-std::fma(wm1, wm1, 2.0f * wm1));
-                    F3 Xs = from_onb_no_tangent(mk(strip_nan(vx * norm_v), W, strip_nan(vy * norm_v)), mean);
+                    F3 Xs = vmf_sample(mean, kappa, self_seed);
+                    self_seed += 2u;
                     F3 dir_markov = normalize(Xs);
                     float c0 = 0.2f, c1 = 0.8f;
                     if (!(sel.weight_sum > 0.0f) || near_zero(length(ld)) || near_zero(sel.weight_sum) ||
@@ -915,14 +978,7 @@ SampleRet rt_sample_guided(const Scene& s, Grid& grid, const GuidedOpts& o, F3 o
                     } else {
                         d = dir_markov;
                     }
-                    float mix_pdf = 0.0f;
-                    if (conf1 != 0.0f && sum_mc > 0.0f) {
-                        for (int k = 0; k < 21; ++k) {
-                            if (!(all[k].weight_sum > 0.0f)) continue;
-                            mix_pdf += vmf_pdf(normalize(light_dir(all[k], org)), get_kappa(all[k]), d) *
-                                       (all[k].weight_sum / sum_mc);
-                        }
-                    }
+                    float mix_pdf = conf1 != 0.0f ? all_vmf_pdfs(all, sum_mc, org, d) : 0.0f;
                     float avg_pdf = conf0 * cos_pdf + (conf1 == 0.0f ? 0.0f : conf1 * std::fmax(mix_pdf, 0.0f));
                     F3 diffuse = make_diffuse(o.quality, it.color, dir, it.normal, it.tangent, it.rx, d);
                     cached_pdf = avg_pdf;
@@ -1059,21 +1115,12 @@ uint64_t orc_trace_guided(void* h, const float* cam, uint32_t width, uint32_t he
         snap = tent;
         for (uint64_t p = 0; p < n; ++p) {
             uint32_t x = (uint32_t)(p % width), y = (uint32_t)(p / width);
-            // fill_mc_screen_space_states (general.wesl:367-385)
             Chain cands[16];
-            uint32_t base = (x / subgroup) * subgroup;
-            uint32_t lane = orc_rand_u32(own[p]) % subgroup;
-            Chain lc = base + lane < width ? snap[(uint64_t)y * width + base + lane] : czero();
+            ScreenDraw sd = screen_draw(own[p], x, y, width, subgroup);
+            own[p] = sd.seed_after;
+            Chain lc = sd.lane_ok ? snap[sd.lane_pix] : czero();
             for (int k = 0; k < 4; ++k) cands[k] = lc;
-            for (int k = 4; k < 16; ++k) {
-                uint32_t xr = orc_rand_u32(own[p]);
-                own[p] = xr;
-                uint32_t yr = orc_rand_u32(own[p]);
-                own[p] = yr;
-                int32_t dx = (int32_t)(xr % 91u) - 45, dy = (int32_t)(yr % 91u) - 45;
-                uint32_t nx = (uint32_t)((int32_t)x + dx), ny = (uint32_t)((int32_t)y + dy);
-                cands[k] = mc_at((uint64_t)(uint32_t)(nx + ny * width));
-            }
+            for (int k = 4; k < 16; ++k) cands[k] = mc_at(sd.idx[k - 4]);
             float o3[3], d3[3];
             orc_create_ray(cam, x, y, width, height, o3, d3);
             SampleRet rs = rt_sample_guided(s, grid, o, mk(o3[0], o3[1], o3[2]), mk(d3[0], d3[1], d3[2]),
@@ -1347,6 +1394,99 @@ void orc_background(void* h, const float* d, float* out) {
     out[0] = c.x;
     out[1] = c.y;
     out[2] = c.z;
+}
+
+// Guided-mode hooks.  Chains travel as 8 u32 words (bindings.wesl:3-9 plus
+// a zero pad word), world-grid cells as 56-byte WorldMarcovChainState.
+float orc_strip_nan(float f) { return strip_nan(f); }
+
+// out: idx, secondary, primary (before the modulo), loc.x, loc.y, loc.z.
+void orc_grid_key(const float* pos, const float* normal, const float* incoming, const float* cam,
+                  uint32_t table, uint32_t* out) {
+    GKey k = grid_key(mk(pos[0], pos[1], pos[2]), mk(normal[0], normal[1], normal[2]),
+                      mk(incoming[0], incoming[1], incoming[2]), mk(cam[0], cam[1], cam[2]), table);
+    out[0] = k.idx;
+    out[1] = k.secondary;
+    out[2] = k.primary;
+    out[3] = k.loc[0];
+    out[4] = k.loc[1];
+    out[5] = k.loc[2];
+}
+
+float orc_get_kappa(const uint32_t* chain) { return get_kappa(chain_from_words(chain)); }
+float orc_expm1(float x) { return expm1_ref(x); }
+float orc_x_div_expm1(float x) { return x_div_expm1(x); }
+
+void orc_vmf_pdf_many(const float* mean, float kappa, const float* dirs, uint64_t n, float* out) {
+    const F3 m = mk(mean[0], mean[1], mean[2]);
+    for (uint64_t i = 0; i < n; ++i)
+        out[i] = vmf_pdf(m, kappa, mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+}
+
+// Unnormalised Xs per seed (general.wesl:797-808).
+void orc_vmf_sample_many(const float* mean, float kappa, const uint32_t* seeds, uint64_t n,
+                         float* out) {
+    const F3 m = mk(mean[0], mean[1], mean[2]);
+    for (uint64_t i = 0; i < n; ++i) {
+        F3 x = vmf_sample(m, kappa, seeds[i]);
+        out[3 * i] = x.x;
+        out[3 * i + 1] = x.y;
+        out[3 * i + 2] = x.z;
+    }
+}
+
+void orc_update_state(const float* x1, const float* x2, float p_hat, uint32_t seed, int not_vmf,
+                      const uint32_t* chain_in, float confidence, float sum_mc, uint32_t* chain_out) {
+    Chain c = update_state(mk(x1[0], x1[1], x1[2]), mk(x2[0], x2[1], x2[2]), p_hat, seed,
+                           not_vmf != 0, chain_from_words(chain_in), confidence, sum_mc);
+    chain_to_words(c, chain_out);
+}
+
+// out: lane pixel, lane valid, 12 neighbour indices, seed after (15 words).
+void orc_screen_draw(uint32_t seed, uint32_t x, uint32_t y, uint32_t width, uint32_t subgroup,
+                     uint32_t* out) {
+    ScreenDraw d = screen_draw(seed, x, y, width, subgroup);
+    out[0] = d.lane_pix;
+    out[1] = d.lane_ok ? 1u : 0u;
+    for (int k = 0; k < 12; ++k) out[2 + k] = d.idx[k];
+    out[14] = d.seed_after;
+}
+
+// 21 chains in; returns the selected index (-1: none); *seed is advanced.
+int orc_resample_mcmm(const uint32_t* chains, uint32_t* seed, float* sum_mc) {
+    Chain all[kNumResamples];
+    for (int k = 0; k < kNumResamples; ++k) all[k] = chain_from_words(chains + 8 * k);
+    return resample_mcmm(all, seed, sum_mc);
+}
+
+float orc_all_vmf_pdfs(const uint32_t* chains, float sum_mc, const float* hit, const float* d) {
+    Chain all[kNumResamples];
+    for (int k = 0; k < kNumResamples; ++k) all[k] = chain_from_words(chains + 8 * k);
+    return all_vmf_pdfs(all, sum_mc, mk(hit[0], hit[1], hit[2]), mk(d[0], d[1], d[2]));
+}
+
+// One sequential world-grid operation on cur / old (table cells each):
+// op 0 grid_load (-> chain_out), 1 grid_store_markov_chain (chain_in),
+// 2 grid_load_radiance_estimate (-> rad_out), 3 grid_add_radiance (rad_in).
+void orc_grid_op(int op, void* cur, const void* old, uint32_t table, const float* cam,
+                 const float* pos, const float* normal, const float* incoming, int force_old,
+                 const uint32_t* chain_in, const float* rad_in, uint32_t* chain_out,
+                 float* rad_out) {
+    Grid g{(WorldC*)cur, (const WorldC*)old, table, mk(cam[0], cam[1], cam[2])};
+    GKey k = grid_key(mk(pos[0], pos[1], pos[2]), mk(normal[0], normal[1], normal[2]),
+                      mk(incoming[0], incoming[1], incoming[2]), g.cam, table);
+    if (op == 0) {
+        chain_to_words(grid_load(g, k, force_old != 0), chain_out);
+    } else if (op == 1) {
+        grid_store(g, k, chain_from_words(chain_in));
+    } else if (op == 2) {
+        F3 r = grid_radiance(g, k);
+        rad_out[0] = r.x;
+        rad_out[1] = r.y;
+        rad_out[2] = r.z;
+    } else {
+        grid_add(g, k, mk(rad_in[0], rad_in[1], rad_in[2]));
+    }
 }
 
 }  // extern "C"
