@@ -232,6 +232,20 @@ lmb_status lmb_scene_intersect(lmb_scene scene, const float *rays, uint64_t n,
 lmb_status lmb_scene_intersect_device(lmb_scene scene, const void *rays_dev,
                                       uint64_t n, void *hits_dev, void *stream);
 
+/* Allocation-free variant of lmb_scene_intersect_device, with the same
+ * results.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The traversal's scratch (fetch counters
+ * and the per-thread stack spill) lives in a static 16 MiB device array that
+ * each GPU the library runs on holds until the process ends; the call
+ * allocates nothing.  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU (several streams or threads) race: serialise them.  The
+ * launch uses as many query blocks as fit the array; a scene whose BVH is so
+ * deep that fewer than one block per CU fits (more than 39 node levels) is
+ * rejected with LMB_ERR_INVALID_ARGUMENT. */
+lmb_status lmb_scene_intersect_device_noalloc(lmb_scene scene, const void *rays_dev,
+                                              uint64_t n, void *hits_dev, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Tracer  (lib.rs:260-376, path_tracing/mod.rs:7-147,                 */
 /* low_level.rs:77-113)                                                */

@@ -194,6 +194,7 @@ _sig("lmb_scene_create_instanced", [ctypes.POINTER(_InstancedDescC), ctypes.POIN
 _sig("lmb_scene_bvh_stats", [_vp, ctypes.POINTER(_BvhStatsC)])
 _sig("lmb_scene_intersect", [_vp, _f32p, ctypes.c_uint64, _vp])
 _sig("lmb_scene_intersect_device", [_vp, _vp, ctypes.c_uint64, _vp, _vp])
+_sig("lmb_scene_intersect_device_noalloc", [_vp, _vp, ctypes.c_uint64, _vp, _vp])
 _sig("lmb_primary_hits", [_vp, ctypes.POINTER(_CameraC), ctypes.c_uint32, ctypes.c_uint32,
                           ctypes.c_float, ctypes.c_float, _u32p, _f32p])
 _sig("lmb_sync", [_vp])
@@ -282,8 +283,8 @@ HEADER_FUNCTIONS = [
     "lmb_textures_create", "lmb_textures_add", "lmb_textures_set_recolour",
     "lmb_textures_set_background", "lmb_textures_destroy", "lmb_scene_create",
     "lmb_scene_destroy", "lmb_scene_bvh_stats", "lmb_scene_intersect",
-    "lmb_scene_intersect_device", "lmb_sync", "lmb_primary_hits", "lmb_scene_create_instanced",
-    *list(_OPTIONAL),
+    "lmb_scene_intersect_device", "lmb_scene_intersect_device_noalloc", "lmb_sync",
+    "lmb_primary_hits", "lmb_scene_create_instanced", *list(_OPTIONAL),
 ]
 
 
@@ -687,6 +688,14 @@ class Scene:
         hits = np.empty(len(r), dtype=HIT_DTYPE)
         _check(_lib.lmb_scene_intersect(self._h, _fptr(r), len(r), hits.ctypes.data_as(_vp)))
         return hits
+
+    def intersect_device(self, rays_dev: int, n: int, hits_dev: int, stream=None,
+                         allocate: bool = True) -> None:
+        """Closest hits of n device rays (32 B each) into device hits (16 B
+        each), enqueued on `stream`.  allocate=False takes the allocation-free
+        entry point (static scratch; calls on one GPU must not overlap)."""
+        fn = _lib.lmb_scene_intersect_device if allocate else _lib.lmb_scene_intersect_device_noalloc
+        _check(fn(self._h, rays_dev, n, hits_dev, stream))
 
     def primary_hits(self, camera: Camera, width: int, height: int, t_min: float = 1e-4,
                      t_max: float = 1000.0):

@@ -421,6 +421,31 @@ lmb_status lmb_scene_intersect_device(lmb_scene scene, const void* rays_dev, uin
     return LMB_OK;
 }
 
+lmb_status lmb_scene_intersect_device_noalloc(lmb_scene scene, const void* rays_dev, uint64_t n,
+                                              void* hits_dev, void* stream) {
+    if (!scene) return fail(LMB_ERR_INVALID_ARGUMENT, "scene is NULL");
+    if (n == 0) return LMB_OK;
+    if (!rays_dev || !hits_dev) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL buffer");
+    hipError_t e = hipSetDevice(scene->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    const uint32_t levels = scene->stats.max_depth;
+    const uint64_t fit = noalloc_max_blocks(levels);
+    if (fit < (uint64_t)device_cu_count())
+        return fail(LMB_ERR_INVALID_ARGUMENT,
+                    "BVH of " + std::to_string(levels) + " node levels: its stack spill for one "
+                    "query block per CU exceeds the " +
+                        std::to_string(noalloc_scratch_bytes() >> 20) +
+                        " MiB static scratch of the allocation-free query (use "
+                        "lmb_scene_intersect_device)");
+    uint8_t* scratch_dev = nullptr;
+    e = noalloc_scratch(scene->device, &scratch_dev);
+    if (e != hipSuccess) return fail_hip(e, "hipGetSymbolAddress of the static scratch");
+    e = launch_intersect_noalloc(scene->dev, rays_dev, n, hits_dev, scratch_dev, levels,
+                                 (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "intersect kernel launch");
+    return LMB_OK;
+}
+
 lmb_status lmb_scene_intersect(lmb_scene scene, const float* rays, uint64_t n, lmb_hit* hits) {
     if (!scene) return fail(LMB_ERR_INVALID_ARGUMENT, "scene is NULL");
     if (n == 0) return LMB_OK;

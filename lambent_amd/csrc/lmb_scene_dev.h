@@ -38,6 +38,18 @@ uint64_t query_threads(uint64_t n);
 hipError_t launch_intersect(const SceneDev& s, const void* rays, uint64_t n, void* hits,
                             uint2* spill, uint32_t* fetch, hipStream_t stream);
 
+// Allocation-free intersection (query.hip): the fetch counters and stack
+// spill live in a static device array per GPU.  noalloc_scratch returns its
+// device address on `device`, which must be current (looked up once per
+// device); noalloc_max_blocks is the number of query blocks whose spill fits
+// for a BVH of `bvh_levels` node levels.
+hipError_t noalloc_scratch(int device, uint8_t** out);
+uint64_t noalloc_max_blocks(uint32_t bvh_levels);
+uint64_t noalloc_scratch_bytes();
+int device_cu_count();
+hipError_t launch_intersect_noalloc(const SceneDev& s, const void* rays, uint64_t n, void* hits,
+                                    uint8_t* scratch_dev, uint32_t bvh_levels,
+                                    hipStream_t stream);
 hipError_t launch_primary(const SceneDev& s, const CameraDev& cam, uint32_t width,
                           uint32_t height, float tmin, float tmax, uint32_t* prim,
                           float* t, uint2* spill, uint32_t* fetch, hipStream_t stream);

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "dev_common.h"
 #include "dev_anyhit.h"
@@ -75,15 +76,18 @@ struct CameraSource {
     }
 };
 
+// spill_stride: stack entries per thread in `spill` (at least the BVH's
+// node levels minus kStackLds).
 __global__ __launch_bounds__(kQueryBlock) void k_intersect(SceneDev s, const Ray* __restrict__ rays,
                                                            uint64_t n, uint4* __restrict__ hits,
                                                            uint2* __restrict__ spill,
+                                                           uint32_t spill_stride,
                                                            uint32_t* __restrict__ fetch) {
     __shared__ uint2 lds_stack[kStackLds * kQueryBlock];
     const uint64_t gtid = (uint64_t)blockIdx.x * kQueryBlock + threadIdx.x;
     ShortStack<kQueryBlock> st;
     st.lds = lds_stack + threadIdx.x;
-    st.spill = spill + gtid * (kStackMax - kStackLds);
+    st.spill = spill + gtid * spill_stride;
     LMB_TRI_SHARE(kQueryBlock, ts);
     RaySource src{rays, hits};
     traverse_scene<kQueryBlock, kQueryRefill>(s, n, fetch, st, ts, src, true);
@@ -113,7 +117,69 @@ hipError_t launch_intersect(const SceneDev& s, const void* rays, uint64_t n, voi
     if (e != hipSuccess) return e;
     uint64_t threads = query_threads(n);
     hipLaunchKernelGGL(k_intersect, dim3((uint32_t)(threads / kQueryBlock)), dim3(kQueryBlock),
-                       0, stream, s, (const Ray*)rays, n, (uint4*)hits, spill, fetch);
+                       0, stream, s, (const Ray*)rays, n, (uint4*)hits, spill,
+                       (uint32_t)(kStackMax - kStackLds), fetch);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------
+// Allocation-free intersection.  The traversal's scratch - the fetch
+// counters (first kNoAllocSpillOffset bytes) and the per-thread stack spill
+// (the rest) - lives in one static device array per GPU, for callers that
+// cannot allocate.  Every call on a GPU shares it, so calls that overlap on
+// one GPU race.  Host code uses the array's device address from
+// hipGetSymbolAddress, never its name.
+constexpr uint64_t kNoAllocScratchBytes = 16ull << 20;
+constexpr uint64_t kNoAllocSpillOffset = 4096;
+static_assert(kFetchWords * sizeof(uint32_t) <= kNoAllocSpillOffset, "fetch counters fit");
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+
+hipError_t noalloc_scratch(int device, uint8_t** out) {
+    static std::mutex mu;
+    static uint8_t* ptrs[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptrs[device]) {
+        // the device must be current: each GPU holds its own copy of the array
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        ptrs[device] = static_cast<uint8_t*>(p);
+    }
+    *out = ptrs[device];
+    return hipSuccess;
+}
+
+uint32_t noalloc_spill_stride(uint32_t bvh_levels) {
+    return bvh_levels > (uint32_t)kStackLds ? bvh_levels - (uint32_t)kStackLds : 1u;
+}
+
+uint64_t noalloc_max_blocks(uint32_t bvh_levels) {
+    const uint64_t per_block = (uint64_t)kQueryBlock * noalloc_spill_stride(bvh_levels) * sizeof(uint2);
+    return (kNoAllocScratchBytes - kNoAllocSpillOffset) / per_block;
+}
+
+uint64_t noalloc_scratch_bytes() { return kNoAllocScratchBytes; }
+
+__global__ void k_zero_words(uint32_t* __restrict__ p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+hipError_t launch_intersect_noalloc(const SceneDev& s, const void* rays, uint64_t n, void* hits,
+                                    uint8_t* scratch_dev, uint32_t bvh_levels,
+                                    hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint64_t fit = noalloc_max_blocks(bvh_levels);
+    if (fit == 0) return hipErrorInvalidValue;
+    const uint64_t blocks = std::min<uint64_t>(query_threads(n) / kQueryBlock, fit);
+    uint32_t* fetch = reinterpret_cast<uint32_t*>(scratch_dev);
+    uint2* spill = reinterpret_cast<uint2*>(scratch_dev + kNoAllocSpillOffset);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, stream, fetch, kFetchWords);
+    hipLaunchKernelGGL(k_intersect, dim3((uint32_t)blocks), dim3(kQueryBlock), 0, stream, s,
+                       (const Ray*)rays, n, (uint4*)hits, spill, noalloc_spill_stride(bvh_levels),
+                       fetch);
     return hipGetLastError();
 }
 
