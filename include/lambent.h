@@ -316,8 +316,10 @@ lmb_status lmb_tracer_destroy(lmb_tracer tracer);
 typedef struct lmb_data_buffers_s *lmb_data_buffers;
 /* data_buffer/mod.rs:151-252 DataBuffers::new; also owns the three output
  * images and the wavefront ray queues sized for width x height.  width *
- * height must be below 2^31 (LMB_ERR_INVALID_ARGUMENT otherwise, for resize
- * too, which then leaves the buffers as they were). */
+ * height must be below 2^31 (LMB_ERR_INVALID_ARGUMENT otherwise).  A resize
+ * that fails - refused, or out of device memory part way - leaves every
+ * buffer and the size as they were; it holds the old and the new window's
+ * memory at once. */
 lmb_status lmb_data_buffers_create(uint32_t width, uint32_t height,
                                    uint32_t world_size, uint32_t buffer_flags,
                                    lmb_data_buffers *out);

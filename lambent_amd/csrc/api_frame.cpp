@@ -168,8 +168,16 @@ lmb_status alloc_zero(DeviceBuffer& b, size_t bytes) {
     return LMB_OK;
 }
 
+void swap_buffers(DeviceBuffer& a, DeviceBuffer& b) {
+    std::swap(a.ptr, b.ptr);
+    std::swap(a.bytes, b.bytes);
+}
+
 // data_buffer/mod.rs:116-148 create_window_required_buffers (+ the output
-// images and wavefront queues that the caller owns in the reference).
+// images and wavefront queues that the caller owns in the reference).  The
+// whole window is allocated before any of it replaces the current one: a
+// refused or failed resize (out of memory part way) leaves every buffer and
+// the size as they were.
 lmb_status alloc_window(lmb_data_buffers b, uint32_t width, uint32_t height) {
     width = width ? width : 1;
     height = height ? height : 1;
@@ -177,28 +185,37 @@ lmb_status alloc_window(lmb_data_buffers b, uint32_t width, uint32_t height) {
     // Ray queues carry the pixel index as a u32 and the kernels split it
     // into (x, y) with 32-bit arithmetic.
     if (n >= (1ull << 31)) return fail(LMB_ERR_INVALID_ARGUMENT, "width * height must be below 2^31");
+    const uint64_t gi_count = (b->flags & LMB_BUFFER_SPATIAL_RESAMPLING) ? n : 1;
+    const uint64_t mc_count = (b->flags & LMB_BUFFER_MARKOV_CHAIN_SCREEN_SPACE) ? n : 1;
+    struct Part {
+        DeviceBuffer* cur;
+        uint64_t bytes;
+    };
+    const Part parts[] = {
+        {&b->out_color, n * 16}, {&b->out_normal, n * 16}, {&b->out_albedo, n * 16},
+        {&b->own_seed, n * 4},   {&b->pix_sum, n * 16},    {&b->ps_seed, n * 4},
+        {&b->ps_through, n * 16}, {&b->ps_accum, n * 16},  {&b->queue0, n * 32},
+        {&b->queue1, n * 32},    {&b->hits, n * 16},       {&b->counters, kNumCounters * 4},
+        {&b->gi, gi_count * sizeof(Reservoir)},
+        {&b->gi_old, gi_count * sizeof(Reservoir)},
+        {&b->info, n * sizeof(CompressedInfo)},
+        {&b->mc, mc_count * sizeof(MarkovChain)},
+        {&b->mc_old, mc_count * sizeof(MarkovChain)},
+    };
+    constexpr size_t kParts = sizeof(parts) / sizeof(parts[0]);
+    DeviceBuffer fresh[kParts];  // freed on return: the new window on failure, the old on success
+    SpillBuffer spill;
     lmb_status st;
-    if ((st = alloc_zero(b->out_color, n * 16))) return st;
-    if ((st = alloc_zero(b->out_normal, n * 16))) return st;
-    if ((st = alloc_zero(b->out_albedo, n * 16))) return st;
-    if ((st = alloc_zero(b->own_seed, n * 4))) return st;
-    if ((st = alloc_zero(b->pix_sum, n * 16))) return st;
-    if ((st = alloc_zero(b->ps_seed, n * 4))) return st;
-    if ((st = alloc_zero(b->ps_through, n * 16))) return st;
-    if ((st = alloc_zero(b->ps_accum, n * 16))) return st;
-    if ((st = alloc_zero(b->queue0, n * 32))) return st;
-    if ((st = alloc_zero(b->queue1, n * 32))) return st;
-    if ((st = alloc_zero(b->hits, n * 16))) return st;
+    for (size_t i = 0; i < kParts; ++i)
+        if ((st = alloc_zero(fresh[i], parts[i].bytes))) return st;
+    if ((st = ensure_spill(&spill, extend_threads(n)))) return st;
+    for (size_t i = 0; i < kParts; ++i) swap_buffers(*parts[i].cur, fresh[i]);
+    swap_buffers(b->spill.buf, spill.buf);
+    std::swap(b->spill.threads, spill.threads);
+    std::swap(b->spill.ptr, spill.ptr);
     b->hits_primary.release();
-    if ((st = alloc_zero(b->counters, kNumCounters * 4))) return st;
-    b->gi_count = (b->flags & LMB_BUFFER_SPATIAL_RESAMPLING) ? n : 1;
-    b->mc_count = (b->flags & LMB_BUFFER_MARKOV_CHAIN_SCREEN_SPACE) ? n : 1;
-    if ((st = alloc_zero(b->gi, b->gi_count * sizeof(Reservoir)))) return st;
-    if ((st = alloc_zero(b->gi_old, b->gi_count * sizeof(Reservoir)))) return st;
-    if ((st = alloc_zero(b->info, n * sizeof(CompressedInfo)))) return st;
-    if ((st = alloc_zero(b->mc, b->mc_count * sizeof(MarkovChain)))) return st;
-    if ((st = alloc_zero(b->mc_old, b->mc_count * sizeof(MarkovChain)))) return st;
-    if ((st = ensure_spill(&b->spill, extend_threads(n)))) return st;
+    b->gi_count = gi_count;
+    b->mc_count = mc_count;
     b->width = width;
     b->height = height;
     return LMB_OK;

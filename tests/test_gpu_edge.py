@@ -103,6 +103,24 @@ def test_pixel_count_limit_is_rejected_before_allocating():
     np.testing.assert_array_equal(bufs.output(api.Output.COLOR), fresh.output(api.Output.COLOR))
 
 
+def test_resize_out_of_device_memory_keeps_the_old_window():
+    # 46340^2 pixels, just below 2^31, need about 1 TB of window buffers: a
+    # legal size that runs out of the GPU's memory part way through.  The
+    # failed resize must leave the old window whole and usable.
+    spec = scenes.cornell_box(40, 24)
+    scene = spec.build()
+    bufs = api.DataBuffers(40, 24, 1000, api.BufferType.all())
+    _trace(spec, scene, 40, 24, samples=2, seed=3, bufs=bufs)
+    with pytest.raises(api.LambentError) as err:
+        bufs.resize(46340, 46340)
+    assert err.value.code == api.ERR_OUT_OF_MEMORY, err.value
+    assert (bufs.width, bufs.height) == (40, 24)
+    _trace(spec, scene, 40, 24, samples=2, seed=7, bufs=bufs)
+    fresh = _trace(spec, scene, 40, 24, samples=2, seed=7)
+    np.testing.assert_array_equal(bufs.output(api.Output.COLOR), fresh.output(api.Output.COLOR))
+    np.testing.assert_array_equal(bufs.output(api.Output.ALBEDO), fresh.output(api.Output.ALBEDO))
+
+
 @pytest.mark.parametrize("w,h", [(1, 1), (7, 3), (33, 17)])
 def test_cached_single_bounce_equals_plain(w, h):
     """bounces = 1: no path reaches a cache vertex, so the cached tracer
