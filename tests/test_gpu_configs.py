@@ -1,0 +1,318 @@
+"""Every measured configuration at its full workload, against the oracle:
+
+* Cornell box, 256x256, 16 spp, depth 8: same-seed images;
+* procedural 1M triangles at 1920x1080: camera-ray closest hits bit-exact
+  (ids and t) against the oracle's BVH for all 2,073,600 pixels, and the
+  whole 1-spp, 5-bounce, RR-from-3 frame on the same seeds as the oracle;
+* spectral + ReSTIR GI: the 60-frame 1080p orbit (guided, four wavelengths,
+  dispersive dielectrics, temporal + 5 spatial neighbours in 61 px, M cap 20)
+  keeps the state invariants; at 128x72 the same guided spectral tracer
+  agrees with the oracle's guided mode within Monte-Carlo error;
+* 64 uneven scenes in one batch (4 spp, 6 bounces): every image bit-exact
+  against one lmb_trace per image, and the smallest against the oracle;
+* a BVH deeper than the traversal's 8 LDS stack entries (the global spill
+  path): same-seed frame against the oracle.
+
+Slow: each builds the full scene (up to 7.4M triangles) and runs the oracle
+on the host's cores.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from lambent_amd import api, scenes
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+
+def same_seed_fraction(img, ref):
+    return float(np.all(np.abs(img - ref) <= 1e-4 * (1 + np.abs(ref)), axis=-1).mean())
+
+
+def issued_as_per_sample(bufs, samples):
+    """The GPU traces a pixel's camera ray once for all its samples (the
+    camera rays carry no jitter, general.wesl:1158-1173); the oracle counts
+    one camera-ray query per sample."""
+    return bufs.path_vertices() + (samples - 1) * bufs.width * bufs.height
+
+
+def plain(spec, scene, samples, bounces, rr_start=None, quality=api.Medium, seed=0, cam=None):
+    pipe = api.RayTracer(quality).create_pipeline(
+        api.PathTracerOpts(samples=samples, bounces=bounces, rr_start=rr_start,
+                           guiding=api.Guiding.OFF,
+                           restir_temporal=api.RestirTemporalMode.Disabled))
+    b = api.DataBuffers(spec.width, spec.height, 1, api.BufferType(0))
+    pipe.dispatch(scene, cam or spec.camera(), b, seed)
+    return b
+
+
+def test_cfg1_cornell_256_16spp_depth8_same_seeds():
+    spec = scenes.cornell_box(256, 256)
+    b = plain(spec, spec.build(), 16, 8, seed=1)
+    c_o, _, a_o, v_o = oracle.Scene(spec).trace_plain(spec.camera().to_bytes(), 256, 256, quality=1,
+                                                      samples=16, bounces=8, seed_offset=1)
+    np.testing.assert_array_equal(b.output(api.Output.ALBEDO), a_o)
+    frac = same_seed_fraction(b.output(api.Output.COLOR), c_o)
+    print(f"cfg1: {frac:.4f} of pixels within 1e-4, vertices {b.path_vertices()} vs {v_o}")
+    assert frac > 0.97
+    assert abs(issued_as_per_sample(b, 16) - v_o) <= 0.01 * v_o
+
+
+@pytest.fixture(scope="module")
+def cfg2():
+    spec = scenes.procedural(1_000_000, seed=2, width=1920, height=1080)
+    return spec, spec.build()
+
+
+def test_cfg2_primary_hits_bit_exact_at_1080p(cfg2):
+    spec, scene = cfg2
+    cam = spec.camera()
+    prim, t = scene.primary_hits(cam, 1920, 1080)
+    rays = oracle.create_rays(cam.to_bytes(), 1920, 1080)
+    ref = oracle.Bvh(spec.vertices, spec.indices).intersect(rays)
+    np.testing.assert_array_equal(prim.reshape(-1), ref["prim"])
+    hit = ref["prim"] != api.MISS
+    print(f"cfg2 primary hits: {hit.mean():.3f} of 2073600 rays hit")
+    assert hit.mean() > 0.3
+    np.testing.assert_array_equal(t.reshape(-1)[hit].view(np.uint32), ref["t"][hit].view(np.uint32))
+
+
+def test_cfg2_full_frame_same_seeds(cfg2):
+    spec, scene = cfg2
+    b = plain(spec, scene, 1, 5, rr_start=3, seed=4)
+    c_o, _, a_o, v_o = oracle.Scene(spec).trace_plain(spec.camera().to_bytes(), 1920, 1080,
+                                                      quality=1, samples=1, bounces=5, rr_start=3,
+                                                      seed_offset=4)
+    np.testing.assert_array_equal(b.output(api.Output.ALBEDO), a_o)
+    frac = same_seed_fraction(b.output(api.Output.COLOR), c_o)
+    print(f"cfg2 frame: {frac:.4f} of pixels within 1e-4, vertices {b.path_vertices()} vs {v_o}")
+    assert frac > 0.97
+    assert abs(b.path_vertices() - v_o) <= 0.01 * v_o
+
+
+def orbit_cameras(width, height, frames):
+    proj = api.perspective(45.0, width / height, 0.1, 200.0)
+    cams = []
+    for f in range(frames):
+        a = math.radians(6.0 * f)
+        eye = (-25.0 * math.sin(a), 0.0, -25.0 * math.cos(a))
+        d = (-eye[0] / 25.0, 0.0, -eye[2] / 25.0)
+        cams.append(api.Camera.from_proj_view(proj, api.look_to_rh(eye, d, (0.0, 1.0, 0.0))))
+    return cams
+
+
+@pytest.fixture(scope="module")
+def cfg3():
+    spec = scenes.procedural(1_000_000, seed=2, width=1920, height=1080, dispersive=0.2)
+    return spec, spec.build()
+
+
+def guided_spectral_pipe():
+    return api.RayTracer(api.Spectral).create_pipeline(
+        api.PathTracerOpts(samples=1, bounces=5, rr_start=3, guiding=api.Guiding.MARKOV_CHAIN,
+                           restir_temporal=api.RestirTemporalMode.PerPixel))
+
+
+def check_w(gi):
+    """W = safe_div(w, M p_hat) with p_hat = max(|L|, 1e-5) (shared.wesl:160-168,
+    294-304), as the temporal pass leaves it."""
+    m = (gi["packed_confidence_valid"] & 0x7FFFFFFF).astype(np.float64)
+    p = np.maximum(np.linalg.norm(gi["out_radiance"].astype(np.float64), axis=1), 1e-5)
+    denom = m * p
+    expect = np.where(np.abs(denom) < 1e-4, 0.0, gi["w"] / np.where(denom == 0, 1, denom))
+    live = m > 0
+    np.testing.assert_allclose(gi["W"][live], expect[live], rtol=2e-4, atol=1e-6)
+
+
+def test_cfg3_orbit_keeps_state_invariants(cfg3):
+    spec, scene = cfg3
+    W, H = 1920, 1080
+    pipe = guided_spectral_pipe()
+    bufs = api.DataBuffers(W, H, 1_500_000, api.BufferType.all())
+    cap = api.AdvanceOptions(20)
+    for f, cam in enumerate(orbit_cameras(W, H, 60)):
+        pipe.dispatch(scene, cam, bufs, f)
+        bufs.temporal_resample(f)
+        bufs.spatial_resample(5, 61, f)
+        if f % 15 == 14:
+            # the spatial pass writes albedo L W + emission unclamped
+            # (spatial_resampling.wesl:186-193); spectral weights can be negative
+            assert np.isfinite(bufs.output(api.Output.COLOR)).all(), f
+            gi = np.frombuffer(bufs.state_bytes(api.StateBuffer.GI_RESERVOIRS), oracle.RESERVOIR_DTYPE)
+            m = gi["packed_confidence_valid"] & 0x7FFFFFFF
+            assert np.isfinite(gi["w"]).all() and (gi["w"] >= 0).all()
+            # this frame's sample merged with last frame's (capped at 20) reservoir
+            assert m.max() <= 21, f
+            check_w(gi)
+        bufs.advance_frame(api.BufferType.all(), cap)
+    old = np.frombuffer(bufs.state_bytes(api.StateBuffer.OLD_GI_RESERVOIRS), oracle.RESERVOIR_DTYPE)
+    m = old["packed_confidence_valid"] & 0x7FFFFFFF
+    assert m.max() <= 20 and (m > 0).mean() > 0.3
+    check_w(old)   # the cap rescales w with M, so W = w / (M p_hat) still holds
+    world = np.frombuffer(bufs.state_bytes(api.StateBuffer.OLD_WORLD_MARKOV_CHAINS), oracle.WORLD_DTYPE)
+    used = world["secondary_hash"] != 0
+    print(f"cfg3 orbit: {used.sum()} world cells in use, max M {m.max()}")
+    assert used.sum() > 10_000
+    assert (world["secondary_hash"][used] & 1).all()
+    assert (world["lock"] == 0).all()
+    assert (world["timeout"] <= 30).all()
+    lum = world["luminance"][used]
+    assert np.isfinite(lum).all() and (lum >= 0).all()
+    assert (lum <= 100.0 * np.maximum(world["num_samples"][used], 1)[:, None] + 1e-2).all()
+    assert (world["chain_num_samples"] <= 1024).all()
+
+
+def test_cfg3_guided_spectral_matches_oracle_statistically():
+    """The cfg3 scene and tracer (guided, four wavelengths, dispersive
+    dielectrics) at 128x72, against the oracle's guided mode, whose
+    sequential world grid makes it too slow for 1080p.  Eight independent
+    16-spp GPU frames give each 16x12 block's run-to-run spread; one 16-spp
+    oracle frame must lie within it."""
+    W, H, spp, runs = 128, 72, 16, 8
+    spec = scenes.procedural(1_000_000, seed=2, width=W, height=H, dispersive=0.2)
+    scene = spec.build()
+    pipe = api.RayTracer(api.Spectral).create_pipeline(
+        api.PathTracerOpts(samples=spp, bounces=5, rr_start=3, guiding=api.Guiding.MARKOV_CHAIN))
+    bw, bh = 16, 12
+
+    def blocks(img):
+        return img[..., :3].astype(np.float64).reshape(H // bh, bh, W // bw, bw, 3).mean(axis=(1, 3))
+
+    g = []
+    for r in range(runs):
+        bufs = api.DataBuffers(W, H, 1_500_000, api.BufferType.all())
+        pipe.dispatch(scene, spec.camera(), bufs, 100 + r)
+        g.append(blocks(bufs.output(api.Output.COLOR)))
+    g = np.array(g)
+    st = oracle.GuidedState(W, H, 1_500_000)
+    o, _, _, _ = oracle.trace_guided(oracle.Scene(spec), spec.camera().to_bytes(), W, H, st,
+                                     quality=3, samples=spp, bounces=5, rr_start=3, seed_offset=7)
+    o = blocks(o)
+    g_mean, g_sd = g.mean(axis=0), g.std(axis=0, ddof=1)
+    noisy = g_sd > 1e-6
+    np.testing.assert_allclose(o[~noisy], g_mean[~noisy], rtol=1e-5)   # sky-only blocks
+    z = (o - g_mean)[noisy] / (g_sd[noisy] * math.sqrt(1.0 + 1.0 / runs))
+    print(f"cfg3 guided: image mean gpu {g_mean.mean():.4f} oracle {o.mean():.4f}; "
+          f"mean z^2 {np.mean(z ** 2):.2f}, max |z| {np.abs(z).max():.2f} over {z.size} values")
+    assert abs(o.mean() / g_mean.mean() - 1.0) < 0.03
+    assert np.mean(z ** 2) < 2.0
+    assert np.abs(z).max() < 5.0
+
+
+def test_cfg4_full_batch_bit_exact_against_per_image_traces():
+    specs = scenes.batch_workload()
+    items = [(sp.build(), sp.camera(), sp.width, sp.height) for sp in specs]
+    opts = api.PathTracerOpts(samples=4, bounces=6, guiding=api.Guiding.OFF,
+                              restir_temporal=api.RestirTemporalMode.Disabled)
+    batch = api.Batch(items, opts, api.Medium)
+    batch.trace(9)
+    pipe = api.RayTracer(api.Medium).create_pipeline(opts)
+    verts = 0
+    for k, (scene, cam, w, h) in enumerate(items):
+        b = api.DataBuffers(w, h, 1, api.BufferType(0))
+        pipe.dispatch(scene, cam, b, 9)
+        verts += b.path_vertices()
+        np.testing.assert_array_equal(batch.output(k), b.output(api.Output.COLOR), err_msg=f"item {k}")
+    assert batch.path_vertices() == verts
+    # the smallest item against the oracle on the same seeds
+    k = min(range(len(specs)), key=lambda i: specs[i].width * specs[i].height * specs[i].num_triangles)
+    sp = specs[k]
+    c_o, _, _, v_o = oracle.Scene(sp).trace_plain(sp.camera().to_bytes(), sp.width, sp.height,
+                                                  quality=1, samples=4, bounces=6, seed_offset=9)
+    frac = same_seed_fraction(batch.output(k), c_o)
+    print(f"cfg4: {len(items)} items bit-exact; item {k} ({sp.width}^2, {sp.num_triangles} "
+          f"triangles) {frac:.4f} of pixels within 1e-4 of the oracle")
+    assert frac > 0.97
+
+
+def test_deep_bvh_frame_same_seeds_as_oracle():
+    # triangles at geometric scales: a BVH of more node levels than the 8
+    # traversal stack entries in LDS, so deep paths use the global spill
+    n, ratio = 1000, 1.018
+    x = ratio ** -np.arange(n, dtype=np.float64)
+    c = np.stack([x, 0.3 * x, -0.2 * x], axis=1)
+    s = (0.05 * x)[:, None]
+    tri = np.stack([c, c + s * np.array([1.0, 0.0, 0.0]), c + s * np.array([0.0, 1.0, 1.0])], axis=1)
+    rng = np.random.default_rng(3)
+    centre = rng.uniform(3.0, 6.0, (20_000, 1, 3))
+    off = rng.normal(size=(20_000, 3, 3))
+    off *= (rng.lognormal(-3.0, 0.5, (20_000, 3)) / np.linalg.norm(off, axis=2))[..., None]
+    tri = np.concatenate([tri, centre + off]).astype(np.float32)
+    base = scenes.procedural(len(tri), seed=8, width=160, height=120)
+    v = np.zeros((len(tri) * 3, 4), np.float32)
+    v[:, :3] = tri.reshape(-1, 3)
+    base.vertices = v
+    base.view = api.look_at_rh((1.2, 0.5, -1.5), (0.05, 0.01, -0.01), (0.0, 1.0, 0.0))
+    scene = base.build()
+    depth = scene.bvh_stats().max_depth
+    assert depth >= 12, depth
+    b = plain(base, scene, 4, 5, seed=6)
+    c_o, _, a_o, v_o = oracle.Scene(base).trace_plain(base.camera().to_bytes(), 160, 120, quality=1,
+                                                      samples=4, bounces=5, seed_offset=6)
+    np.testing.assert_array_equal(b.output(api.Output.ALBEDO), a_o)
+    frac = same_seed_fraction(b.output(api.Output.COLOR), c_o)
+    print(f"deep BVH ({depth} levels): {frac:.4f} of pixels within 1e-4")
+    assert frac > 0.97
+    assert abs(issued_as_per_sample(b, 4) - v_o) <= 0.02 * v_o
+
+
+def test_cfg5_cache_beats_the_plain_tracer_at_equal_time(cfg2):
+    """The neural-cache tracer against the plain tracer on the 1M-triangle
+    1080p scene (8 bounces, RR from 3), both given the same wall time: the
+    plain tracer one dispatch of 128 spp; the cache starts untrained and spends
+    that time on warm-up training frames and 8-spp frames.  Judged against a
+    1024-spp plain reference accumulated in float64."""
+    import time
+    spec, scene = cfg2
+    W, H = spec.width, spec.height
+    cam = spec.camera()
+    popts = dict(bounces=8, rr_start=3, guiding=api.Guiding.OFF,
+                 restir_temporal=api.RestirTemporalMode.Disabled)
+
+    def rel_mse(img, ref):
+        return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+
+    ref = np.zeros((H, W, 3))
+    chunk = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=64, **popts))
+    rb = api.DataBuffers(W, H, 1, api.BufferType(0))
+    for i in range(16):
+        chunk.dispatch(scene, cam, rb, 50_000_000 + i)
+        ref += rb.output(api.Output.COLOR)[..., :3]
+    ref /= 16
+    pb = api.DataBuffers(W, H, 1, api.BufferType(0))
+    plain_pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=128, **popts))
+    api.sync()
+    t0 = time.perf_counter()
+    plain_pipe.dispatch(scene, cam, pb, 7)
+    api.sync()
+    budget = time.perf_counter() - t0
+    e_plain = rel_mse(pb.output(api.Output.COLOR)[..., :3].astype(np.float64), ref)
+
+    cache = api.NeuralCache(api.NeuralCacheOpts(train_paths=524288, train_steps=256))
+    warm = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=1, **popts))
+    pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=8, **popts))
+    b = api.DataBuffers(W, H, 1, api.BufferType(0))
+    spent, frames = 0.0, 0
+    acc = np.zeros((H, W, 3))
+    for f in range(4):
+        api.sync()
+        t0 = time.perf_counter()
+        warm.train_cached(scene, cam, b, cache, f)
+        api.sync()
+        spent += time.perf_counter() - t0
+    cache.set_opts(api.NeuralCacheOpts(train_steps=32))
+    while spent < budget:
+        api.sync()
+        t0 = time.perf_counter()
+        pipe.dispatch_cached(scene, cam, b, cache, 10 + frames)
+        api.sync()
+        spent += time.perf_counter() - t0
+        acc += b.output(api.Output.COLOR)[..., :3]
+        frames += 1
+    e_cache = rel_mse(acc / frames, ref)
+    print(f"cfg5 at {budget * 1e3:.0f} ms: plain 128 spp relMSE {e_plain:.5f}, cache "
+          f"{frames} x 8 spp (+4 warm-up frames) relMSE {e_cache:.5f}")
+    assert frames >= 4
+    assert e_cache < 0.7 * e_plain
