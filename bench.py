@@ -7,9 +7,11 @@ from bounce 3, Medium quality, guiding off.  One step = one lmb_trace of one
 frame (seed_offset = step index) with the scene resident in HBM.
 
 value = path vertices (closest-hit queries) of the K timed frames / wall time.
-roofline: the traversal ("extend") kernel, algorithmic bytes per vertex
-84 + 80 * L (ray 32 B in, hit 16 B out, one 36 B triangle, one 80 B node per
-BVH level, L = ceil(log8(N/4)) = 6) over its HIP-event launch durations.
+roofline: the traversal ("extend") kernel, algorithmic bytes per path vertex
+184 + 80 * L = 664 (48 B ray in, 48 B ray / hit out, one 36 B triangle, 4 B
+material index, 32 B material, 16 B accumulate, one 80 B node per BVH level,
+L = ceil(log8(N/4)) = 6) over its HIP-event launch durations; the traversal
+alone moves 84 + 80 * L = 564 of them (traversal_only_frac).
 cpu_baseline: the CPU oracle (C++/OpenMP restatement of the reference
 shaders, "port") on a bounded band of rows of the same frame.
 
@@ -18,11 +20,16 @@ wavelengths, Cauchy dispersion, temporal + spatial reuse) over a 60-frame
 camera orbit, in ms per frame.
 batched_uneven: 64 uneven scenes in one batch of persistent launches
 (hipGraph, autotuned launch shape) against one lmb_trace per image.
+cornell_cfg1: the Cornell box at 256x256, 16 spp, depth 8, GPU and CPU.
+guided_medium: the reference's own path (Markov-chain guiding, world grid,
+per-pixel reservoir, no RR) on the 1080p scene.
 time_to_relmse: the neural-cache tracer against the plain tracer (both 8
 bounces) on the same scene: GPU time for each to reach the relMSE that the
 plain tracer has at 16, 64 and 256 spp, judged against a 4096-spp plain
 reference accumulated in fp64; plus the matrix-core rate of the cache's
-inference and training kernels against the bf16 dense peak.
+inference and training kernels against the bf16 dense peak (with the spread
+rule, and as configured: every path ends at its second vertex), and the CPU
+oracle's time to the 256-spp relMSE (a band of rows, scaled to the frame).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--triangles N]
 """
@@ -80,21 +87,9 @@ def rel_mse(img, ref) -> float:
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
 
 
-def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
-              lr: float = 1e-3, spp: int = 8, steps: int = 32, overlap: bool = True,
-              warm_paths: int = 524288, warm_steps: int = 256, ema: float = 0.0) -> dict:
-    """Time-to-relMSE of the cached tracer against the plain tracer, both with
-    8-bounce paths (RR from bounce 3), judged against a plain reference of
-    ref_spp samples accumulated in fp64.  The cache starts untrained: `warm`
-    training frames (lmb_trace_nrc_train: about warm_paths paths at random
-    pixels train it with warm_steps Adam steps, no image; timed), then
-    `frames` frames of `spp` samples (the camera rays are traced once per
-    frame), `steps` Adam steps per frame over the frame's 65536 training
-    records (overlap: a training pass of its own and the steps run on the
-    cache's own stream beside the frame's samples, which are traced eight at
-    a time; otherwise the training paths come from the first sample).  Times
-    are wall times of the dispatches (host read-back of the images for
-    scoring is not counted)."""
+def cache_reference(api, scene, spec, cam, ref_spp: int):
+    """The plain tracer's 4096-spp fp64 reference and its 16 / 64 / 256-spp
+    points (8 bounces, RR from 3) for the time-to-relMSE legs."""
     import numpy as np
     W, H = spec.width, spec.height
     popts = dict(bounces=8, rr_start=3, guiding=api.Guiding.OFF,
@@ -122,9 +117,33 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
         img, dt = plain(n_spp, 20_000 + n_spp)
         plain_pts.append({"spp": n_spp, "seconds": dt, "relmse": rel_mse(img, ref)})
         log(f"cache leg: plain {n_spp} spp {dt * 1e3:.1f} ms relMSE {plain_pts[-1]['relmse']:.5f}")
+    return ref, plain_pts, f"plain tracer {ref_spp} spp, fp64 accumulation of {chunk}-spp dispatches"
 
+
+def cache_leg(api, scene, spec, cam, reference, frames: int, warm: int,
+              lr: float = 1e-3, spp: int = 8, steps: int = 32, overlap: bool = True,
+              warm_paths: int = 524288, warm_steps: int = 256, ema: float = 0.0,
+              defer_specular: int = 1) -> dict:
+    """Time-to-relMSE of the cached tracer against the plain tracer, both with
+    8-bounce paths (RR from bounce 3), judged against the plain reference of
+    cache_reference (`reference`).  The cache starts untrained: `warm`
+    training frames (lmb_trace_nrc_train: about warm_paths paths at random
+    pixels train it with warm_steps Adam steps, no image; timed), then
+    `frames` frames of `spp` samples (the camera rays are traced once per
+    frame), `steps` Adam steps per frame over the frame's 65536 training
+    records (overlap: a training pass of its own and the steps run on the
+    cache's own stream beside the frame's samples, which are traced eight at
+    a time; otherwise the training paths come from the first sample).  Times
+    are wall times of the dispatches (host read-back of the images for
+    scoring is not counted).  defer_specular: how many specular scatters may
+    move a path's cache vertex one vertex on (0: always the second vertex)."""
+    import numpy as np
+    W, H = spec.width, spec.height
+    popts = dict(bounces=8, rr_start=3, guiding=api.Guiding.OFF,
+                 restir_temporal=api.RestirTemporalMode.Disabled)
+    ref, plain_pts, ref_text = reference
     frame_opts = api.NeuralCacheOpts(learning_rate=lr, train_steps=steps, overlap_training=overlap,
-                                     ema_decay=ema)
+                                     ema_decay=ema, defer_specular=defer_specular)
     cache = api.NeuralCache(api.NeuralCacheOpts(**{**frame_opts.__dict__, "train_paths": warm_paths,
                                                    "train_steps": warm_steps}))
     warm_pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=1, **popts))
@@ -152,7 +171,8 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
         if n in checks:
             pts.append({"frames": n, "samples": n * spp, "seconds": t,
                         "relmse": rel_mse(acc / n, ref)})
-    log(f"cache leg: {warm} + {frames} frames in {t:.2f} s, final relMSE {pts[-1]['relmse']:.5f}")
+    log(f"cache leg (spread bound {defer_specular}): {warm} + {frames} frames in {t:.2f} s, "
+        f"final relMSE {pts[-1]['relmse']:.5f}")
     # Matrix-core figures from a few more frames with per-launch events (kept
     # out of the timed run: the events add about a quarter to a frame).
     b.set_profiling(True)
@@ -177,12 +197,14 @@ def cache_leg(api, scene, spec, cam, ref_spp: int, frames: int, warm: int,
     train_tf = train_flops / (train_ms * 1e-3) / 1e12 if train_ms else 0.0
     return {
         "workload": f"procedural {W}x{H}, 8 bounces, RR from 3; cache: paths end at the second "
-                    f"vertex (a specular scatter moves it one vertex on), 65536 training paths "
+                    + ("vertex (a specular scatter moves it one vertex on, at most "
+                       f"{defer_specular} times)" if defer_specular else "vertex, always") +
+                    f", 65536 training paths "
                     f"and {steps} Adam steps (lr {lr:g}) per frame"
                     f"{' on a second stream beside the samples' if overlap else ''}; {warm} "
                     f"training frames of {warm_paths} paths and {warm_steps} steps (no image, "
                     f"timed), then frames of {spp} spp",
-        "reference": f"plain tracer {ref_spp} spp, fp64 accumulation of {chunk}-spp dispatches",
+        "reference": ref_text,
         "plain": plain_pts,
         "cache_points": pts,
         "warm_ms_per_frame": t_warm / max(warm, 1) * 1e3,
@@ -259,10 +281,119 @@ def spectral_leg(api, scenes, triangles: int, width: int, height: int, frames: i
     }
 
 
+def cornell_leg(api, scenes, frames: int, cpu: bool) -> dict:
+    """Config 1: the Cornell box built in code, 256x256, 16 spp, depth 8
+    (no RR), Medium, plain; one frame = one dispatch of 16 spp.  The CPU row
+    is the oracle (C++/OpenMP restatement, "port") on the same frame."""
+    spec = scenes.cornell_box(256, 256)
+    scene = spec.build()
+    opts = api.PathTracerOpts(samples=16, bounces=8, guiding=api.Guiding.OFF,
+                              restir_temporal=api.RestirTemporalMode.Disabled)
+    pipe = api.RayTracer(api.Medium).create_pipeline(opts)
+    bufs = api.DataBuffers(256, 256, 1, api.BufferType(0))
+    pipe.dispatch(scene, spec.camera(), bufs, 0)
+    api.sync()
+    bufs.set_profiling(True)
+    bufs.kernel_stats()
+    t0 = time.perf_counter()
+    for f in range(frames):
+        pipe.dispatch(scene, spec.camera(), bufs, 1 + f)
+    api.sync()
+    dt = time.perf_counter() - t0
+    verts = bufs.kernel_stats()["path_vertices"]
+    per_sample = verts + frames * 15 * 256 * 256   # camera rays counted once per sample
+    out = {
+        "workload": "Cornell box (36 triangles), 256x256, 16 spp, depth 8, no RR, Medium, plain",
+        "frames": frames, "ms_per_frame": dt / frames * 1e3,
+        "path_vertices_per_s": verts / dt,
+        "path_vertices_per_s_counting_camera_rays_per_sample": per_sample / dt,
+        "note": "the GPU traces each pixel's camera ray once for all 16 samples (no jitter); "
+                "the second rate counts it once per sample, as the CPU row does",
+    }
+    if cpu:
+        import oracle
+        osc = oracle.Scene(spec)
+        t0 = time.perf_counter()
+        _, _, _, v = osc.trace_plain(spec.camera().to_bytes(), 256, 256, quality=1, samples=16,
+                                     bounces=8, seed_offset=1)
+        cdt = time.perf_counter() - t0
+        out["cpu"] = {"value": v / cdt, "unit": "path-vertices/s", "seconds_per_frame": cdt,
+                      "cores": oracle.num_threads(), "kind": "port",
+                      "sample": "the whole 256x256 16-spp frame; CPU restatement, not lavapipe"}
+    return out
+
+
+def cpu_time_to_relmse(spec, reference, rows: int = 8) -> dict:
+    """The CPU oracle's time to reach the plain tracer's 256-spp relMSE on the
+    1080p scene: it renders 256 spp (8 bounces, RR from 3) on a band of rows
+    in the image centre; the full-frame time is that band's time scaled by
+    height / rows (the band's relMSE against the same reference is reported
+    beside the GPU's full-frame figure)."""
+    import numpy as np
+    import oracle
+    ref = reference[0]
+    osc = oracle.Scene(spec)
+    row0 = spec.height // 2 - rows // 2
+    t0 = time.perf_counter()
+    c, _, _, v = osc.trace_plain(spec.camera().to_bytes(), spec.width, spec.height, row0=row0,
+                                 rows=rows, quality=1, samples=256, bounces=8, rr_start=3,
+                                 seed_offset=20_256)
+    dt = time.perf_counter() - t0
+    band = c[..., :3].astype(np.float64)
+    return {"seconds_full_frame": dt * spec.height / rows, "band_seconds": dt,
+            "band_relmse": rel_mse(band, ref[row0:row0 + rows]),
+            "band_path_vertices_per_s": v / dt, "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"{rows} rows x {spec.width} px at 256 spp, 8 bounces, RR from 3, scaled to "
+                      f"{spec.height} rows; CPU restatement, not lavapipe"}
+
+
+def guided_leg(api, scene, spec, frames: int) -> dict:
+    """The reference's own path (rt_main): Markov-chain guiding with the
+    world grid (1.5M cells), per-pixel ReSTIR reservoir, Medium, one
+    wavelength, 1 spp, 5 bounces, no Russian roulette, on the 1M-triangle
+    1080p scene; one frame = trace + advance_frame (static camera)."""
+    W, H = spec.width, spec.height
+    opts = api.PathTracerOpts(samples=1, bounces=5, guiding=api.Guiding.MARKOV_CHAIN,
+                              restir_temporal=api.RestirTemporalMode.PerPixel)
+    pipe = api.RayTracer(api.Medium).create_pipeline(opts)
+    bufs = api.DataBuffers(W, H, 1_500_000, api.BufferType.all())
+    cam = spec.camera()
+    cap = api.AdvanceOptions(20)
+    for f in range(3):
+        pipe.dispatch(scene, cam, bufs, f)
+        bufs.advance_frame(api.BufferType.all(), cap)
+    api.sync()
+    bufs.set_profiling(True)
+    bufs.kernel_stats()
+    t0 = time.perf_counter()
+    for f in range(frames):
+        pipe.dispatch(scene, cam, bufs, 3 + f)
+        bufs.advance_frame(api.BufferType.all(), cap)
+    api.sync()
+    dt = time.perf_counter() - t0
+    ks = bufs.kernel_stats()
+    verts = ks["path_vertices"]
+    L = math.ceil(math.log(1_000_000 / 4, 8))
+    bpv = 184 + 80 * L + 768
+    log(f"guided leg: {frames} frames in {dt:.3f} s, {verts} path vertices")
+    return {
+        "workload": f"reference path (rt_main): procedural 1000000 triangles, {W}x{H}, Medium, "
+                    "Markov-chain guiding (world grid 1.5M), per-pixel reservoir, 1 spp, 5 "
+                    "bounces, no RR; frame = trace + advance_frame",
+        "frames": frames, "ms_per_frame": dt / frames * 1e3,
+        "path_vertices_per_s": verts / dt,
+        "extend_ms_per_frame": ks["extend_ms"] / frames,
+        "shade_and_guide_ms_per_frame": ks["shade_ms"] / frames,
+        "roofline": {"bound": "hbm", "scope": "whole frame", "bytes_per_vertex": bpv,
+                     "achieved": verts * bpv / dt / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": verts * bpv / dt / 1e9 / HBM_PEAK_GBS},
+    }
+
+
 def batch_leg(api, scenes, frames: int) -> dict:
     """64 uneven procedural scenes (1k-500k triangles log-uniform, 128^2 to
-    1024^2 images) rendered as one batch: 4 spp, 6 bounces, RR from bounce 3,
-    Medium, plain.  Compared with rendering the same 64 images one lmb_trace
+    1024^2 images) rendered as one batch: 4 spp, 6 bounces (no RR), Medium,
+    plain.  Compared with rendering the same 64 images one lmb_trace
     at a time; the batch's launch shape comes from an autotuner sweep."""
     t0 = time.perf_counter()
     specs = scenes.batch_workload()
@@ -271,7 +402,7 @@ def batch_leg(api, scenes, frames: int) -> dict:
     pixels = sum(sp.width * sp.height for sp in specs)
     log(f"batch leg: {len(items)} scenes, {tris} triangles, {pixels} pixels built in "
         f"{time.perf_counter() - t0:.1f} s")
-    opts = api.PathTracerOpts(samples=4, bounces=6, rr_start=3, guiding=api.Guiding.OFF,
+    opts = api.PathTracerOpts(samples=4, bounces=6, guiding=api.Guiding.OFF,
                               restir_temporal=api.RestirTemporalMode.Disabled)
     # one image at a time
     pipe = api.RayTracer(api.Medium).create_pipeline(opts)
@@ -313,7 +444,7 @@ def batch_leg(api, scenes, frames: int) -> dict:
         f"{graph_ms:.1f} ms graph, autotuned {best_ms:.1f} ms ({tune_s:.1f} s sweep)")
     return {
         "workload": f"{len(items)} procedural scenes, {tris} triangles (1k-500k log-uniform), "
-                    f"{pixels} pixels (128^2-1024^2), 4 spp, 6 bounces, RR from 3, Medium, "
+                    f"{pixels} pixels (128^2-1024^2), 4 spp, 6 bounces, no RR, Medium, "
                     f"2/3 diffuse 1/3 GGX",
         "one_by_one_ms_per_frame": seq_ms,
         "batched_eager_ms_per_frame": eager_ms,
@@ -354,6 +485,10 @@ def main() -> int:
     ap.add_argument("--no-spectral-leg", action="store_true")
     ap.add_argument("--spectral-frames", type=int, default=60)
     ap.add_argument("--no-batch-leg", action="store_true")
+    ap.add_argument("--no-cornell-leg", action="store_true")
+    ap.add_argument("--cornell-frames", type=int, default=10)
+    ap.add_argument("--no-guided-leg", action="store_true")
+    ap.add_argument("--guided-frames", type=int, default=10)
     ap.add_argument("--batch-frames", type=int, default=3)
     ap.add_argument("--ref-spp", type=int, default=4096)
     ap.add_argument("--cache-frames", type=int, default=96)
@@ -414,7 +549,11 @@ def main() -> int:
 
     value = verts / elapsed
     L = math.ceil(math.log(max(args.triangles, 4) / 4, 8))
-    bytes_per_vertex = 84 + 80 * L
+    # algorithmic bytes per path vertex: 48 ray in + 48 ray / hit out + 36
+    # triangle + 4 material index + 32 material + 16 accumulate + one 80-B
+    # node per BVH level; of these the traversal itself moves 84 + 80 L
+    bytes_per_vertex = 184 + 80 * L
+    traversal_bytes = 84 + 80 * L
     achieved = verts * bytes_per_vertex / (ks["extend_ms"] * 1e-3) / 1e9 if ks["extend_ms"] else 0.0
     out = {
         "metric": "path-vertices/s @1080p",
@@ -447,6 +586,8 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "bytes_per_vertex": bytes_per_vertex,
+            "traversal_only_bytes_per_vertex": traversal_bytes,
+            "traversal_only_frac": achieved * traversal_bytes / bytes_per_vertex / HBM_PEAK_GBS,
             "extend_ms_per_step": ks["extend_ms"] / args.steps,
             "shade_ms_per_step": ks["shade_ms"] / args.steps,
             "extend_launches": ks["extend_launches"],
@@ -467,12 +608,26 @@ def main() -> int:
             out["cpu_baseline"] = cpu_baseline(spec)
         except Exception as exc:  # the GPU figure stands on its own
             log(f"cpu baseline failed: {exc!r}")
+    if not args.no_cornell_leg:
+        out["cornell_cfg1"] = cornell_leg(api, scenes, args.cornell_frames,
+                                          not args.no_cpu_baseline)
+    if not args.no_guided_leg:
+        out["guided_medium"] = guided_leg(api, scene, spec, args.guided_frames)
     if not args.no_cache_leg:
-        out["time_to_relmse"] = cache_leg(api, scene, spec, cam, args.ref_spp, args.cache_frames,
-                                          args.cache_warmup, args.cache_lr, args.cache_spp,
-                                          args.cache_steps, not args.cache_no_overlap,
-                                          args.cache_warm_paths, args.cache_warm_steps,
-                                          args.cache_ema)
+        reference = cache_reference(api, scene, spec, cam, args.ref_spp)
+        leg = dict(frames=args.cache_frames, warm=args.cache_warmup, lr=args.cache_lr,
+                   spp=args.cache_spp, steps=args.cache_steps, overlap=not args.cache_no_overlap,
+                   warm_paths=args.cache_warm_paths, warm_steps=args.cache_warm_steps,
+                   ema=args.cache_ema)
+        out["time_to_relmse"] = cache_leg(api, scene, spec, cam, reference, **leg)
+        # the config as written: every path ends at its second vertex
+        out["time_to_relmse_vertex2"] = cache_leg(api, scene, spec, cam, reference,
+                                                  defer_specular=0, **leg)
+        if not args.no_cpu_baseline:
+            try:
+                out["time_to_relmse"]["cpu_plain_256spp"] = cpu_time_to_relmse(spec, reference)
+            except Exception as exc:
+                log(f"cpu time-to-relMSE failed: {exc!r}")
     if not args.no_spectral_leg:
         del scene
         out["spectral_restir"] = spectral_leg(api, scenes, args.triangles, args.width,

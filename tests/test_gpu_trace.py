@@ -322,7 +322,7 @@ def test_instanced_scene_equals_baked_geometry():
     twice with its own transforms and material tables, a quad once, and a
     fourth instance whose mask hides it, render bit-exactly like one scene
     holding the same triangles transformed ((m0 x + m1 y) + m2 z) + m3 in
-    float32."""
+    float32, and like the oracle given those triangles on the same seeds."""
     base = scenes.cornell_box(64, 64)
     box_v = np.array([[x, y, z, 0] for z in (-0.1, 0.1) for y in (0, 0.3) for x in (-0.1, 0.1)],
                      np.float32)
@@ -367,3 +367,14 @@ def test_instanced_scene_equals_baked_geometry():
         imgs.append((bufs.output(api.Output.COLOR), bufs.output(api.Output.ALBEDO)))
     np.testing.assert_array_equal(imgs[0][0], imgs[1][0])
     np.testing.assert_array_equal(imgs[0][1], imgs[1][1])
+    # and the instanced scene against the oracle, given the transformed triangles
+    flat_spec = scenes.SceneSpec(np.concatenate(flat_v + [room_v]), None,
+                                 np.concatenate(flat_m + [room_m]).astype(np.uint32),
+                                 base.materials, base.diffuse, base.emission, base.attributes,
+                                 base.recolour, base.background, base.proj, base.view, 64, 64)
+    c_o, _, a_o, _ = oracle.Scene(flat_spec).trace_plain(base.camera().to_bytes(), 64, 64,
+                                                         quality=1, samples=4, bounces=5,
+                                                         seed_offset=3)
+    np.testing.assert_array_equal(imgs[0][1], a_o)
+    close = np.all(np.abs(imgs[0][0] - c_o) <= 1e-4 * (1 + np.abs(c_o)), axis=-1)
+    assert close.mean() > 0.97, close.mean()
