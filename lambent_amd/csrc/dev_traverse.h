@@ -243,14 +243,14 @@ struct Traversal {
             constexpr float kChildSlack = 4.0f * 5.9604645e-8f;
             const float tn_lo = fmaf(-fabsf(tn), kChildSlack, tn);
             const float tf_hi = fmaf(fabsf(tf), kChildSlack, tf);
-            const bool hit = meta != kMetaEmpty && tn_lo <= fminf(tf_hi, best) && r.tmin <= tf_hi;
-            if (hit) {
-                if (meta == kMetaInternal) {
-                    node_hits |= 1u << (s ^ oct);
-                } else {
-                    tri_hits |= (meta >> 5) << (meta & 31u);
-                }
-            }
+            // Branch-free: an empty slot's meta byte is 0, so a hit on it adds
+            // no internal bit and an empty triangle run (a branch per slot
+            // cost about ten scalar instructions of exec-mask bookkeeping).
+            static_assert(kMetaEmpty == 0, "empty slots must carry no triangles");
+            const bool hit = tn_lo <= fminf(tf_hi, best) && r.tmin <= tf_hi;
+            const bool internal = meta == kMetaInternal;
+            node_hits |= (hit && internal) ? (1u << ((uint32_t)s ^ oct)) : 0u;
+            tri_hits |= (hit && !internal) ? ((meta >> 5) << (meta & 31u)) : 0u;
         }
         *tri_base_out = tri_base;
         *tri_mask_out = tri_hits;

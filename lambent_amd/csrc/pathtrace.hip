@@ -127,9 +127,10 @@ struct QueueSource {
 #endif
 constexpr int kRefill = LMB_REFILL;  // idle lanes that make a wave fetch more rays
 
-// Minimum waves per SIMD the traversal kernels are compiled for (register cap).
+// Blocks per CU the traversal kernel is compiled for (a register cap): at 5
+// blocks of 4 waves each SIMD holds 5 waves, which needs at most 96 VGPRs.
 #ifndef LMB_EXTEND_WAVES
-#define LMB_EXTEND_WAVES 1
+#define LMB_EXTEND_WAVES 5
 #endif
 
 // kAlpha: the scene's geometry is not opaque (any-hit alpha test).
@@ -441,7 +442,9 @@ struct TaggedQueueSource {
     }
 };
 
-__global__ __launch_bounds__(kPtBlock, LMB_EXTEND_WAVES) void k_batch_extend(const SceneDev* table, FrameDev f,
+// The batch traversal binds a scene per lane: 99 VGPRs uncapped (4 waves per
+// SIMD); the 96-VGPR cap of k_extend would spill.
+__global__ __launch_bounds__(kPtBlock) void k_batch_extend(const SceneDev* table, FrameDev f,
                                                            TraceParams p, uint32_t bounce,
                                                            uint32_t refill) {
     __shared__ uint2 lds_stack[kStackLds * kPtBlock];
