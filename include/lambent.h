@@ -175,15 +175,27 @@ lmb_status lmb_scene_create(const lmb_scene_desc *desc, lmb_scene *out);
  * are baked into one world-space BVH at creation (the ray query's closest
  * hit over instances, with single-level traversal); triangles are numbered
  * instance by instance in hits; an instance with (mask & 0xFF) == 0 is not
- * seen (the cull mask of the reference's ray queries is 0xFF).  Shading
- * uses the world-space vertices (the reference's get_vertices reads object
- * space, bindings.wesl:102-110, which is the same for identity
- * transforms). */
+ * seen (the cull mask of the reference's ray queries is 0xFF).  As the
+ * reference's vertex fetch (get_vertices, bindings.wesl:102-110, or
+ * getCommittedVertexPositions) returns object-space positions, the hit's
+ * tangent and normal come from the object-space triangle, and its facing is
+ * decided in object space (a mirroring transform reverses the winding seen
+ * in world space). */
 typedef struct lmb_geometry_desc {
     const float *vertices;      /* vec4 per vertex (xyz used) */
     uint64_t num_vertices;
     const uint32_t *indices;    /* 3 per triangle; NULL = non-indexed */
     uint64_t num_indices;
+    /* Procedural geometry instead of triangles (wgpu BlasGeometries::
+     * AabbGeometries): 6 floats per box, min xyz then max xyz.  The
+     * reference's ray query hands each candidate box to the pipeline's
+     * IntersectionHandler (low_level.rs:115-147, general.wesl:1068-1076);
+     * the handler this library implements is the default one
+     * (default_intersection_handler.wesl), which reports no hit, so boxes
+     * never take part in a closest hit.  Boxes must be finite with
+     * min <= max. */
+    const float *aabbs;
+    uint64_t num_aabbs;
 } lmb_geometry_desc;
 typedef struct lmb_instance {
     uint32_t geometry;

@@ -112,7 +112,16 @@ class _SceneDescC(ctypes.Structure):
 
 class _GeometryDescC(ctypes.Structure):
     _fields_ = [("vertices", _f32p), ("num_vertices", ctypes.c_uint64),
-                ("indices", _u32p), ("num_indices", ctypes.c_uint64)]
+                ("indices", _u32p), ("num_indices", ctypes.c_uint64),
+                ("aabbs", _f32p), ("num_aabbs", ctypes.c_uint64)]
+
+
+@dataclass
+class AabbGeometry:
+    """Procedural geometry (wgpu BlasGeometries::AabbGeometries): boxes as
+    (n, 6) floats, min xyz then max xyz.  With the default intersection
+    handler (default_intersection_handler.wesl) no box is ever hit."""
+    boxes: np.ndarray
 
 
 class _InstanceC(ctypes.Structure):
@@ -386,6 +395,11 @@ def materials_buffer(materials: Sequence[Material]) -> bytes:
     return b"".join(m.to_bytes() for m in materials)
 
 
+def descriptor(materials: Sequence[Material]) -> bytes:
+    """README.md:21-23 name of `<[Material]>::buffer_descriptor` (lib.rs:184-192)."""
+    return materials_buffer(materials)
+
+
 def materials_array(materials: Sequence[Material]) -> np.ndarray:
     return np.frombuffer(materials_buffer(materials), dtype=np.uint32).reshape(-1, 8).copy()
 
@@ -411,6 +425,17 @@ class Camera:
         c = _CameraC()
         _check(_lib.lmb_camera_from_proj_view(_fptr(p), _fptr(v), ctypes.byref(c)))
         return Camera(c)
+
+    @staticmethod
+    def new(proj, view) -> "Camera":
+        """README.md:48-50 name of from_proj_view."""
+        return Camera.from_proj_view(proj, view)
+
+    def buffer_descriptor(self) -> bytes:
+        """camera.rs:59-70: the 128-byte uniform buffer contents."""
+        return bytes(self._c)
+
+    descriptor = buffer_descriptor  # README.md:48-50 name
 
     def update_from_proj_view(self, proj=None, view=None) -> None:
         p = None if proj is None else Camera._cols(proj)
@@ -516,6 +541,18 @@ class TextureLoader:
         if a.ndim != 3 or a.shape[2] != 4:
             raise TextureError(ERR_TEXTURE, "expected an (h, w, 4) uint8 array")
         self.images.append(a)
+
+    def raw(self) -> list:
+        """README.md:36-41: the decoded RGBA8 images, in load order."""
+        return list(self.images)
+
+    def create_textures(self, emission: Sequence[np.ndarray] = (),
+                        attributes: Sequence[np.ndarray] = ()) -> "TextureSet":
+        """textures.rs:41-90: the loaded images as the diffuse textures (with
+        optional emission and attribute arrays) of a texture set."""
+        return TextureSet(self.images, emission, attributes)
+
+    create_texture = create_textures  # README.md:41-43 name
 
 
 class TextureSet:
@@ -633,11 +670,17 @@ class Scene:
     def from_instances(cls, geometries, instances, materials: Sequence[Material] | np.ndarray,
                        textures: Optional[TextureSet] = None, opaque: bool = True) -> "Scene":
         """TLAS instances (main.rs:560-563): geometries = [(vertices, indices
-        or None)], instances = [(geometry, 3x4 row-major transform,
-        material_idx per triangle, mask)], baked into one BVH."""
+        or None) or AabbGeometry], instances = [(geometry, 3x4 row-major
+        transform, material_idx per primitive, mask)], baked into one BVH."""
         keep = []
         geo = (_GeometryDescC * max(len(geometries), 1))()
-        for k, (v, idx) in enumerate(geometries):
+        for k, gd in enumerate(geometries):
+            if isinstance(gd, AabbGeometry):
+                b = np.ascontiguousarray(gd.boxes, np.float32).reshape(-1, 6)
+                keep.append(b)
+                geo[k].aabbs, geo[k].num_aabbs = _fptr(b), len(b)
+                continue
+            v, idx = gd
             v = np.ascontiguousarray(v, np.float32).reshape(-1, 4)
             idx = None if idx is None else np.ascontiguousarray(idx, np.uint32).reshape(-1)
             keep += [v, idx]
@@ -669,7 +712,8 @@ class Scene:
         self.num_triangles = sum(
             (len(np.asarray(geometries[g][1]).reshape(-1)) // 3 if geometries[g][1] is not None
              else len(np.asarray(geometries[g][0]).reshape(-1, 4)) // 3)
-            for g, _, _, mask in instances if int(mask) & 0xFF)
+            for g, _, _, mask in instances
+            if int(mask) & 0xFF and not isinstance(geometries[g], AabbGeometry))
         return self
 
     @property

@@ -35,11 +35,21 @@ __global__ __launch_bounds__(kDebugBlock) void k_debug(SceneDev s, FrameDev f) {
         if (h.x != kMissPrim) {
             const float4* tp = reinterpret_cast<const float4*>(s.tris + h.x);
             const float4 a = tp[0], b = tp[1], c = tp[2];
-            const V3 v0 = v3(a.x, a.y, a.z), v1 = v3(b.x, b.y, b.z), v2 = v3(c.x, c.y, c.z);
+            V3 v0 = v3(a.x, a.y, a.z), v1 = v3(b.x, b.y, b.z), v2 = v3(c.x, c.y, c.z);
+            bool mirrored = false;
+            if (s.obj_tris) {  // instanced: object-space vertices, as get_vertices
+                const float4* o = s.obj_tris + 3 * (uint64_t)__float_as_uint(a.w);
+                const float4 oa = o[0], ob = o[1], oc = o[2];
+                mirrored = oa.w != 0.0f;
+                if constexpr (K == LMB_QUALITY_DEBUG_TANGENT) {
+                    v0 = v3(oa.x, oa.y, oa.z);
+                    v1 = v3(ob.x, ob.y, ob.z);
+                }
+            }
             if constexpr (K == LMB_QUALITY_DEBUG_FRONT_FACE) {
                 const float4 d4 = q[2 * slot + 1];
                 const V3 dir = v3(d4.x, d4.y, d4.z);
-                const bool front = dot3(dir, cross3(v1 - v0, v2 - v0)) < 0.0f;
+                const bool front = (dot3(dir, cross3(v1 - v0, v2 - v0)) < 0.0f) != mirrored;
                 const float shade = 1.0f / (__uint_as_float(h.y) + 1.0f);
                 col = front ? v3(0.0f, shade, 0.0f) : v3(shade, 0.0f, 0.0f);
             } else if constexpr (K == LMB_QUALITY_DEBUG_REFLECTANCE) {

@@ -181,8 +181,20 @@ __device__ __forceinline__ Isect make_isect(const SceneDev& s, uint32_t leaf, fl
     const uint32_t prim = __float_as_uint(a.w);
     it.prim = prim;
     it.t = t;
-    const V3 v0 = v3(a.x, a.y, a.z), v1 = v3(b.x, b.y, b.z), v2 = v3(c.x, c.y, c.z);
+    V3 v0 = v3(a.x, a.y, a.z), v1 = v3(b.x, b.y, b.z), v2 = v3(c.x, c.y, c.z);
     it.front_face = dot3(dir, cross3(v1 - v0, v2 - v0)) < 0.0f;
+    if (s.obj_tris) {
+        // Instanced geometry: the reference fetches object-space vertices
+        // (bindings.wesl:102-110) for the tangent and normal, and a
+        // triangle's facing is decided in object space, where a mirroring
+        // transform reverses the winding seen in world space.
+        const float4* o = s.obj_tris + 3 * (uint64_t)prim;
+        const float4 oa = o[0], ob = o[1], oc = o[2];
+        v0 = v3(oa.x, oa.y, oa.z);
+        v1 = v3(ob.x, ob.y, ob.z);
+        v2 = v3(oc.x, oc.y, oc.z);
+        if (oa.w != 0.0f) it.front_face = !it.front_face;
+    }
     const Material m = s.materials[__float_as_uint(b.w)];  // material index (leaf record v1.w)
     const uint32_t idx_diffuse = m.tex_idx_diffuse_emission & 0xFFFFu;
     const uint32_t idx_emission = m.tex_idx_diffuse_emission >> 16;

@@ -62,6 +62,7 @@ struct lmb_scene_s {
     int device = 0;
     lmb::SceneDev dev{};
     lmb::DeviceBuffer nodes, tris, prim_material, materials, texels, tex;
+    lmb::DeviceBuffer obj_tris;  // instanced scenes: object-space triangles (or empty)
     lmb::DeviceBuffer fetch;  // ray counter of the query kernels
     lmb::SpillBuffer spill;
     lmb_bvh_stats stats{};

@@ -24,6 +24,10 @@ struct SceneDev {
     uint32_t recolour;              // TexDesc index of the recolour texture
     uint32_t bg_base;               // six TexDesc entries: +x -x +y -y +z -z
     uint32_t alpha_test;            // non-opaque geometry: any-hit alpha test
+    // Instanced scenes with a non-identity transform: per primitive the
+    // object-space triangle (3 float4, x of the fourth component of the first
+    // = 1 when the instance transform mirrors), else NULL.
+    const float4* obj_tris;
 };
 
 // Number of compute units of the current device (cached).

@@ -56,6 +56,7 @@ _sig("orc_scene_create", [_f32p, ctypes.c_uint64, _u32p, ctypes.c_uint64, _u32p,
                           ctypes.c_uint32], _vp)
 _sig("orc_scene_destroy", [_vp])
 _sig("orc_scene_set_alpha_test", [_vp, ctypes.c_int], None)
+_sig("orc_scene_set_object_space", [_vp, _f32p, _vp], None)
 _sig("orc_trace_debug", [_vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                          ctypes.c_float, ctypes.c_float, _f32p], None)
 _sig("orc_trace_plain", [_vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
@@ -158,7 +159,9 @@ class Scene:
     Texture arrays follow the order diffuse[], emission[], attributes[],
     recolour (default 1x1 white), 6 background faces (default black)."""
 
-    def __init__(self, spec):
+    def __init__(self, spec, object_space=None):
+        """object_space: (per-primitive object-space triangles (n, 9), per-
+        primitive mirrored flag) of an instanced scene, or None."""
         v, idx, ntri = _vidx(spec.vertices, spec.indices)
         images = list(spec.diffuse) + list(spec.emission) + list(spec.attributes)
         diffuse_base = 0
@@ -188,6 +191,12 @@ class Scene:
                                         _u(d), len(images), diffuse_base, emission_base,
                                         attr_base, recolour, bg_base)
         _lib.orc_scene_set_alpha_test(self._h, 0 if getattr(spec, "opaque", True) else 1)
+        if object_space is not None:
+            obj, mirrored = object_space
+            obj = np.ascontiguousarray(obj, np.float32).reshape(ntri, 9)
+            mir = np.ascontiguousarray(mirrored, np.uint8).reshape(ntri)
+            self._keep = self._keep + (obj, mir)
+            _lib.orc_scene_set_object_space(self._h, _f(obj), mir.ctypes.data_as(_vp))
         self.spec = spec
 
     def trace_plain(self, camera_bytes: bytes, width: int, height: int, quality: int = 1,

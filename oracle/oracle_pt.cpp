@@ -113,6 +113,13 @@ struct Scene {
     std::vector<uint32_t> descs;  // offset, width, height per texture
     uint32_t diffuse_base, emission_base, attr_base, recolour, bg_base;
     bool alpha_test = false;  // non-opaque geometry: any-hit alpha test
+    // Instanced geometry: per primitive the object-space triangle (9 floats)
+    // and whether its instance transform mirrors (determinant < 0).  The
+    // reference's vertex fetch returns object-space positions
+    // (bindings.wesl:102-110), so the tangent and normal come from these,
+    // and the facing of a triangle is decided in object space.
+    std::vector<float> obj;
+    std::vector<uint8_t> mirrored;
 };
 
 struct Tex4 {
@@ -253,6 +260,13 @@ Hit ray_hit(const Scene& s, F3 org, F3 dir, float tmin, float tmax,
     F3 v2 = mk(s.verts[4 * i2], s.verts[4 * i2 + 1], s.verts[4 * i2 + 2]);
     // front face: counter-clockwise seen from the ray origin
     h.front_face = dot(dir, cross(sub(v1, v0), sub(v2, v0))) < 0.0f;
+    if (!s.obj.empty()) {
+        const float* o = &s.obj[9 * p];
+        v0 = mk(o[0], o[1], o[2]);
+        v1 = mk(o[3], o[4], o[5]);
+        v2 = mk(o[6], o[7], o[8]);
+        if (s.mirrored[p]) h.front_face = !h.front_face;
+    }
     const uint32_t* m = &s.mats[8 * s.matidx[p]];
     uint32_t idx_diffuse = m[4] & 0xFFFFu, idx_emission = m[4] >> 16;
     uint32_t idx_attr = m[5] & 0xFFFFu;
@@ -1242,6 +1256,14 @@ void* orc_scene_create(const float* verts, uint64_t nverts, const uint32_t* idx,
 }
 
 void orc_scene_set_alpha_test(void* h, int on) { ((Scene*)h)->alpha_test = on != 0; }
+
+// obj: 9 floats per primitive (object-space v0, v1, v2); mirrored: 1 byte
+// per primitive.
+void orc_scene_set_object_space(void* h, const float* obj, const uint8_t* mirrored) {
+    Scene* s = (Scene*)h;
+    s->obj.assign(obj, obj + 9 * s->ntri);
+    s->mirrored.assign(mirrored, mirrored + s->ntri);
+}
 
 void orc_scene_destroy(void* h) {
     Scene* s = (Scene*)h;

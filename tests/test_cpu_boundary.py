@@ -188,3 +188,36 @@ def test_instanced_scene_rejects_a_missing_geometry():
                                  [api.Material.new([0, 0], [1, 0], [1, 1], [0, 0], 0, None, None,
                                                    api.EmissionScale(0.0, 0.0), None,
                                                    api.MaterialType.Diffuse)])
+
+
+def test_aabb_geometry_is_validated_before_any_device_work():
+    """lmb_geometry_desc boxes (6 floats, min then max) must be finite with
+    min <= max, and a geometry holds triangles or boxes, not both; refused
+    before the library touches a device (no GPU here)."""
+    ident = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0]], np.float32)
+    mats = api.materials_array([api.Material.new(
+        [0.5, 0.5], [0.5, 0.5], [0.5, 0.5], [0, 0], 0, None, None, api.EmissionScale(0.0, 0.0),
+        None, api.MaterialType.Diffuse)])
+    bad = [np.array([[0, 0, 0, 1, -1, 1]], np.float32),          # min > max
+           np.array([[0, 0, np.nan, 1, 1, 1]], np.float32),      # not finite
+           np.array([[0, 0, 0, np.inf, 1, 1]], np.float32)]
+    for boxes in bad:
+        with pytest.raises(api.LambentError, match="AABB 0 of geometry 0"):
+            api.Scene.from_instances([api.AabbGeometry(boxes)],
+                                     [(0, ident, np.zeros(1, np.uint32), 0xFF)], mats)
+
+
+def test_readme_names_alias_the_code_names():
+    """README.md:21-23, 36-50 names: Camera::new / descriptor, descriptor on a
+    slice of materials, TextureLoader raw / create_texture."""
+    proj = api.perspective(45.0, 1.5, 0.1, 200.0)
+    view = api.look_to_rh((0.5, 0.5, -1.0), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0))
+    a, b = api.Camera.new(proj, view), api.Camera.from_proj_view(proj, view)
+    assert a.descriptor() == b.buffer_descriptor() == b.to_bytes() and len(a.descriptor()) == 128
+    m = [api.Material.new([0.5, 0.5], [0.5, 0.5], [0.5, 0.5], [0, 0], k, None, None,
+                          api.EmissionScale(0.0, 0.0), None, api.MaterialType.Diffuse) for k in range(3)]
+    assert api.descriptor(m) == api.materials_buffer(m) and len(api.descriptor(m)) == 96
+    tl = api.TextureLoader()
+    tl.load_rgba8(api.solid_rgba8(1, 2, 3, 4, size=2))
+    assert len(tl.raw()) == 1 and tl.raw()[0].shape == (2, 2, 4)
+    assert api.TextureLoader.create_texture is api.TextureLoader.create_textures

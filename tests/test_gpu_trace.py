@@ -317,12 +317,11 @@ def _rot_y(deg, t):
     return np.array([[c, 0, s, t[0]], [0, 1, 0, t[1]], [-s, 0, c, t[2]]], np.float32)
 
 
-def test_instanced_scene_equals_baked_geometry():
-    """TLAS instances (main.rs:560-563, general.wesl:1059): a box instanced
-    twice with its own transforms and material tables, a quad once, and a
-    fourth instance whose mask hides it, render bit-exactly like one scene
-    holding the same triangles transformed ((m0 x + m1 y) + m2 z) + m3 in
-    float32, and like the oracle given those triangles on the same seeds."""
+def _instanced_room(mirror=False, aabbs=False):
+    """The Cornell room as an identity instance plus a box instanced twice
+    (rotated, optionally mirrored) and a quad, and a fourth instance whose
+    mask hides it; returns (instanced scene, flat spec of the same world-space
+    triangles, object-space data for the oracle, base spec)."""
     base = scenes.cornell_box(64, 64)
     box_v = np.array([[x, y, z, 0] for z in (-0.1, 0.1) for y in (0, 0.3) for x in (-0.1, 0.1)],
                      np.float32)
@@ -332,49 +331,113 @@ def test_instanced_scene_equals_baked_geometry():
                        [0.1, 0.6, 0.6, 0], [0.3, 0.8, 0.6, 0], [0.1, 0.8, 0.6, 0]], np.float32)
     nmat = len(base.materials)
     t1, t2 = _rot_y(15, (0.3, 0.0, 0.35)), _rot_y(-18, (0.7, 0.0, 0.6))
+    if mirror:
+        t2 = t2.copy()
+        t2[:, 0] = -t2[:, 0]   # x -> -x in object space: determinant < 0
     ident = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0]], np.float32)
     m_box1 = np.zeros(12, np.uint32)
     m_box2 = (np.arange(12) % nmat).astype(np.uint32)
     m_quad = np.full(2, nmat - 1, np.uint32)
+    room_v = base.vertices[base.indices.reshape(-1)]
+    room_m = base.material_idx
+    geometries = [(box_v, box_i), (quad_v, None), (room_v, None)]
     instances = [(0, t1, m_box1, 0xFF), (0, t2, m_box2, 0x01), (1, ident, m_quad, 0xFF),
-                 (0, ident, m_box1, 0x100)]
-    flat_v, flat_m = [], []
+                 (0, ident, m_box1, 0x100), (2, ident, room_m, 0xFF)]
+    if aabbs:
+        geometries.append(api.AabbGeometry(np.array([[0.2, 0.2, 0.2, 0.4, 0.5, 0.4],
+                                                     [0.0, 0.0, 0.0, 1.0, 1.0, 1.0]], np.float32)))
+        instances.append((3, t1, np.zeros(2, np.uint32), 0xFF))
+    flat_v, flat_m, obj, mir = [], [], [], []
     for g, tf, mi, mask in instances:
-        if mask & 0xFF == 0:
+        if mask & 0xFF == 0 or isinstance(geometries[g], api.AabbGeometry):
             continue
-        v, idx = (box_v, box_i) if g == 0 else (quad_v, None)
+        v, idx = geometries[g]
         p = v[idx] if idx is not None else v
         x, y, z = p[:, 0], p[:, 1], p[:, 2]
         w = [((tf[r, 0] * x + tf[r, 1] * y) + tf[r, 2] * z) + tf[r, 3] for r in range(3)]
         flat_v.append(np.stack(w + [np.zeros_like(x)], 1).astype(np.float32))
         flat_m.append(mi)
-    # the room without its boxes, as one more (identity) instance and in the flat scene
-    room_v = base.vertices[base.indices.reshape(-1)] if base.indices is not None else base.vertices
-    room_m = base.material_idx
+        obj.append(p[:, :3].reshape(-1, 9))
+        mir.append(np.full(len(p) // 3, np.linalg.det(tf[:, :3].astype(np.float64)) < 0, np.uint8))
     tex = base.texture_set()
-    inst = api.Scene.from_instances(
-        [(box_v, box_i), (quad_v, None), (room_v, None)],
-        instances + [(2, ident, room_m, 0xFF)], base.materials, tex)
-    flat = api.Scene(np.concatenate(flat_v + [room_v]), None,
-                     np.concatenate(flat_m + [room_m]), base.materials, tex)
-    assert inst.bvh_stats().num_triangles == flat.bvh_stats().num_triangles == 12 + 12 + 2 + len(room_m)
+    inst = api.Scene.from_instances(geometries, instances, base.materials, tex)
+    flat_spec = scenes.SceneSpec(np.concatenate(flat_v), None,
+                                 np.concatenate(flat_m).astype(np.uint32), base.materials,
+                                 base.diffuse, base.emission, base.attributes, base.recolour,
+                                 base.background, base.proj, base.view, 64, 64)
+    return inst, flat_spec, (np.concatenate(obj), np.concatenate(mir)), base
+
+
+def test_instanced_scene_against_the_oracle():
+    """TLAS instances (main.rs:560-563, general.wesl:1059): hits equal those
+    of one scene holding the same triangles transformed ((m0 x + m1 y) + m2 z)
+    + m3 in float32; shading takes the tangent and normal from the
+    object-space triangle, as the reference's vertex fetch returns object
+    space (bindings.wesl:102-110), so images are compared with the oracle
+    given those triangles, on the same seeds."""
+    inst, flat_spec, object_space, base = _instanced_room()
+    flat = flat_spec.build()
+    assert inst.bvh_stats().num_triangles == flat.bvh_stats().num_triangles
+    cam = base.camera()
+    p_i, t_i = inst.primary_hits(cam, 64, 64)
+    p_f, t_f = flat.primary_hits(cam, 64, 64)
+    np.testing.assert_array_equal(p_i, p_f)
+    np.testing.assert_array_equal(t_i.view(np.uint32), t_f.view(np.uint32))
+    pipe = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=4, bounces=5, guiding=api.Guiding.OFF))
     imgs = []
     for scene in (inst, flat):
-        pipe = api.RayTracer(api.Medium).create_pipeline(
-            api.PathTracerOpts(samples=4, bounces=5, guiding=api.Guiding.OFF))
         bufs = api.DataBuffers(64, 64, 1)
-        pipe.dispatch(scene, base.camera(), bufs, 3)
-        imgs.append((bufs.output(api.Output.COLOR), bufs.output(api.Output.ALBEDO)))
-    np.testing.assert_array_equal(imgs[0][0], imgs[1][0])
-    np.testing.assert_array_equal(imgs[0][1], imgs[1][1])
-    # and the instanced scene against the oracle, given the transformed triangles
-    flat_spec = scenes.SceneSpec(np.concatenate(flat_v + [room_v]), None,
-                                 np.concatenate(flat_m + [room_m]).astype(np.uint32),
-                                 base.materials, base.diffuse, base.emission, base.attributes,
-                                 base.recolour, base.background, base.proj, base.view, 64, 64)
-    c_o, _, a_o, _ = oracle.Scene(flat_spec).trace_plain(base.camera().to_bytes(), 64, 64,
-                                                         quality=1, samples=4, bounces=5,
-                                                         seed_offset=3)
-    np.testing.assert_array_equal(imgs[0][1], a_o)
+        pipe.dispatch(scene, cam, bufs, 3)
+        imgs.append((bufs.output(api.Output.COLOR), bufs.output(api.Output.NORMAL)))
+    assert not np.array_equal(imgs[0][1], imgs[1][1])   # rotated instances: object-space normals
+    osc = oracle.Scene(flat_spec, object_space=object_space)
+    c_o, n_o, a_o, _ = osc.trace_plain(cam.to_bytes(), 64, 64, quality=1, samples=4, bounces=5,
+                                       seed_offset=3)
+    np.testing.assert_allclose(imgs[0][1], n_o, atol=1e-6)
     close = np.all(np.abs(imgs[0][0] - c_o) <= 1e-4 * (1 + np.abs(c_o)), axis=-1)
     assert close.mean() > 0.97, close.mean()
+
+
+def test_identity_instances_equal_the_flat_scene():
+    base = scenes.cornell_box(48, 48)
+    room_v = base.vertices[base.indices.reshape(-1)]
+    ident = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0]], np.float32)
+    inst = api.Scene.from_instances([(room_v, None)], [(0, ident, base.material_idx, 0xFF)],
+                                    base.materials, base.texture_set())
+    flat = api.Scene(room_v, None, base.material_idx, base.materials, base.texture_set())
+    pipe = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=2, bounces=5, guiding=api.Guiding.OFF))
+    out = []
+    for scene in (inst, flat):
+        bufs = api.DataBuffers(48, 48, 1)
+        pipe.dispatch(scene, base.camera(), bufs, 9)
+        out.append(bufs.output(api.Output.COLOR))
+    np.testing.assert_array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("kind", [api.debug.FrontFace, api.debug.Tangent])
+def test_instanced_debug_shaders_use_object_space(kind):
+    """A mirrored instance: the tangent shader reads object-space vertices
+    (tangent.wesl:110-117) and the facing is decided in object space."""
+    inst, flat_spec, object_space, base = _instanced_room(mirror=True)
+    assert object_space[1].any()
+    pipe = api.RayTracer(kind).create_pipeline(api.PathTracerOpts())
+    bufs = api.DataBuffers(64, 64, 1)
+    pipe.dispatch(inst, base.camera(), bufs, 0)
+    osc = oracle.Scene(flat_spec, object_space=object_space)
+    ref = osc.trace_debug(base.camera().to_bytes(), 64, 64, int(kind) - int(api.debug.FrontFace))
+    np.testing.assert_array_equal(bufs.output(api.Output.COLOR), ref)
+
+
+def test_aabb_instances_never_hit_with_the_default_handler():
+    inst, _, _, base = _instanced_room()
+    with_boxes, _, _, _ = _instanced_room(aabbs=True)
+    pipe = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=2, bounces=5, guiding=api.Guiding.OFF))
+    out = []
+    for scene in (inst, with_boxes):
+        bufs = api.DataBuffers(64, 64, 1)
+        pipe.dispatch(scene, base.camera(), bufs, 4)
+        out.append(bufs.output(api.Output.COLOR))
+    np.testing.assert_array_equal(out[0], out[1])
