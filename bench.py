@@ -390,7 +390,7 @@ def guided_leg(api, scene, spec, frames: int) -> dict:
     }
 
 
-def batch_leg(api, scenes, frames: int) -> dict:
+def batch_leg(api, scenes, frames: int, eager_only: bool = False) -> dict:
     """64 uneven procedural scenes (1k-500k triangles log-uniform, 128^2 to
     1024^2 images) rendered as one batch: 4 spp, 6 bounces (no RR), Medium,
     plain.  Compared with rendering the same 64 images one lmb_trace
@@ -432,6 +432,9 @@ def batch_leg(api, scenes, frames: int) -> dict:
         return (time.perf_counter() - t0) / frames * 1e3
 
     eager_ms = timed(api.BatchConfig(use_graph=False))
+    if eager_only:   # the default launch shape without graphs or the sweep (kernel profiles)
+        log(f"batch leg (eager only): one-by-one {seq_ms:.1f} ms, batched {eager_ms:.1f} ms")
+        return {"one_by_one_ms_per_frame": seq_ms, "batched_eager_ms_per_frame": eager_ms}
     graph_ms = timed(api.BatchConfig())
     sweep = [api.BatchConfig(tile_w=tw, tile_h=th, blocks_per_cu=k, refill=16, pass_pixels=pp)
              for tw, th in ((8, 8), (64, 1)) for k in (4, 8, 16) for pp in (0, 1 << 22)]
@@ -490,6 +493,8 @@ def main() -> int:
     ap.add_argument("--no-guided-leg", action="store_true")
     ap.add_argument("--guided-frames", type=int, default=10)
     ap.add_argument("--batch-frames", type=int, default=3)
+    ap.add_argument("--batch-eager-only", action="store_true",
+                    help="batch leg with the default launch shape, no graph, no autotune")
     ap.add_argument("--ref-spp", type=int, default=4096)
     ap.add_argument("--cache-frames", type=int, default=96)
     ap.add_argument("--cache-warmup", type=int, default=4)
@@ -633,7 +638,7 @@ def main() -> int:
         out["spectral_restir"] = spectral_leg(api, scenes, args.triangles, args.width,
                                               args.height, args.spectral_frames)
     if not args.no_batch_leg:
-        out["batched_uneven"] = batch_leg(api, scenes, args.batch_frames)
+        out["batched_uneven"] = batch_leg(api, scenes, args.batch_frames, args.batch_eager_only)
     print(json.dumps(out), flush=True)
     return 0
 
