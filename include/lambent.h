@@ -171,11 +171,13 @@ lmb_status lmb_scene_create(const lmb_scene_desc *desc, lmb_scene *out);
  * mask), examples/cornell_box/src/main.rs:560-563): instance k draws
  * geometry `geometry` through `transform` (3x4 row-major object to world)
  * with materials materials[material_idx[prim]] (general.wesl:1059:
- * material_idx[instance_index].materials[primitive_index]).  The instances
- * are baked into one world-space BVH at creation (the ray query's closest
- * hit over instances, with single-level traversal); triangles are numbered
- * instance by instance in hits; an instance with (mask & 0xFF) == 0 is not
- * seen (the cull mask of the reference's ray queries is 0xFF).  As the
+ * material_idx[instance_index].materials[primitive_index]).  The scene is
+ * one BVH traversed in a single level: a top tree over the instances above
+ * one subtree per instance holding its triangles in world space.  Triangles
+ * are numbered instance by instance in hits, over the instances seen at
+ * creation; an instance with (mask & 0xFF) == 0 is not seen (the cull mask
+ * of the reference's ray queries is 0xFF).  Transforms must be finite and
+ * keep the geometry within 1e30 of the origin.  As the
  * reference's vertex fetch (get_vertices, bindings.wesl:102-110, or
  * getCommittedVertexPositions) returns object-space positions, the hit's
  * tangent and normal come from the object-space triangle, and its facing is
@@ -214,6 +216,22 @@ typedef struct lmb_instanced_desc {
     uint32_t flags;             /* LMB_SCENE_* */
 } lmb_instanced_desc;
 lmb_status lmb_scene_create_instanced(const lmb_instanced_desc *desc, lmb_scene *out);
+/* Moves instances: the counterpart of writing new TlasInstance entries and
+ * rebuilding the TLAS (wgpu build_acceleration_structures with the TLAS
+ * only, examples/cornell_box/src/main.rs:560-563).  `instances` lists every
+ * instance of the creation desc, in its order, with the same geometry; each
+ * one's transform and mask are taken (material_idx is ignored: an instance
+ * keeps the materials it was made with).  An instance hidden at creation
+ * (mask 0) cannot be shown later.  The top tree is rebuilt on the host; on
+ * the device every triangle is re-placed from its object-space copy and the
+ * subtrees are refit from the leaves up, keeping the topology built at
+ * creation (hits stay exact; boxes may grow looser than a fresh build's
+ * under rotation).  Synchronous: waits for the device's queued work first
+ * (which may be reading the scene) and returns when the scene is ready.  On
+ * an error reported before any device work (invalid argument, depth beyond
+ * the traversal stack) the scene is unchanged. */
+lmb_status lmb_scene_update_instances(lmb_scene scene, const lmb_instance *instances,
+                                      uint32_t num_instances);
 lmb_status lmb_scene_destroy(lmb_scene scene);
 /* BVH statistics: node count, depth, leaf triangle count, SAH cost, build ms */
 typedef struct lmb_bvh_stats {

@@ -200,6 +200,7 @@ _sig("lmb_textures_destroy", [_vp])
 _sig("lmb_scene_create", [ctypes.POINTER(_SceneDescC), ctypes.POINTER(_vp)])
 _sig("lmb_scene_destroy", [_vp])
 _sig("lmb_scene_create_instanced", [ctypes.POINTER(_InstancedDescC), ctypes.POINTER(_vp)])
+_sig("lmb_scene_update_instances", [_vp, ctypes.POINTER(_InstanceC), ctypes.c_uint32])
 _sig("lmb_scene_bvh_stats", [_vp, ctypes.POINTER(_BvhStatsC)])
 _sig("lmb_scene_intersect", [_vp, _f32p, ctypes.c_uint64, _vp])
 _sig("lmb_scene_intersect_device", [_vp, _vp, ctypes.c_uint64, _vp, _vp])
@@ -293,7 +294,8 @@ HEADER_FUNCTIONS = [
     "lmb_textures_set_background", "lmb_textures_destroy", "lmb_scene_create",
     "lmb_scene_destroy", "lmb_scene_bvh_stats", "lmb_scene_intersect",
     "lmb_scene_intersect_device", "lmb_scene_intersect_device_noalloc", "lmb_sync",
-    "lmb_primary_hits", "lmb_scene_create_instanced", *list(_OPTIONAL),
+    "lmb_primary_hits", "lmb_scene_create_instanced", "lmb_scene_update_instances",
+    *list(_OPTIONAL),
 ]
 
 
@@ -715,6 +717,19 @@ class Scene:
             for g, _, _, mask in instances
             if int(mask) & 0xFF and not isinstance(geometries[g], AabbGeometry))
         return self
+
+    def update_instances(self, instances) -> None:
+        """New transforms and masks for every instance of a scene made by
+        from_instances (the TLAS rebuild of main.rs:560-563): instances =
+        [(geometry, 3x4 transform, material_idx (ignored), mask)] in creation
+        order.  Returns when the scene is ready."""
+        ins = (_InstanceC * max(len(instances), 1))()
+        for k, (g, tf, _mi, mask) in enumerate(instances):
+            ins[k].geometry = int(g)
+            ins[k].transform[:] = [float(x) for x in np.asarray(tf, np.float32).reshape(12)]
+            ins[k].material_idx = None
+            ins[k].mask = int(mask)
+        _check(_lib.lmb_scene_update_instances(self._h, ins, len(instances)))
 
     @property
     def handle(self):

@@ -289,10 +289,7 @@ lmb_status lmb_textures_destroy(lmb_textures tex) {
 // ------------------------------------------------------------------ scenes
 namespace {
 
-// obj: NULL, or per triangle 12 floats (object-space v0, v1, v2 as vec4, the
-// first w = 1 when the instance transform mirrors).
-lmb_status scene_create(const lmb_scene_desc* desc, const std::vector<float>* obj,
-                        lmb_scene* out) {
+lmb_status scene_create(const lmb_scene_desc* desc, lmb_scene* out) {
     if (!desc || !out) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
     if (desc->num_vertices > 0 && !desc->vertices)
@@ -335,16 +332,6 @@ lmb_status scene_create(const lmb_scene_desc* desc, const std::vector<float>* ob
         return st;
     }
     s->dev.alpha_test = (desc->flags & LMB_SCENE_ALPHA_TEST) ? 1u : 0u;
-    if (obj) {
-        st = s->obj_tris.upload(obj->data(), obj->size() * sizeof(float));
-        if (st) {
-            scene_free(s);
-            delete s;
-            return st;
-        }
-        s->dev.obj_tris = s->obj_tris.as<float4>();
-        s->stats.device_bytes += obj->size() * sizeof(float);
-    }
     *out = s;
     return LMB_OK;
 }
@@ -352,7 +339,7 @@ lmb_status scene_create(const lmb_scene_desc* desc, const std::vector<float>* ob
 }  // namespace
 
 lmb_status lmb_scene_create(const lmb_scene_desc* desc, lmb_scene* out) {
-    return scene_create(desc, nullptr, out);
+    return scene_create(desc, out);
 }
 
 // TLAS instances (main.rs:560-563, general.wesl:1052-1059): each instance's
@@ -362,92 +349,6 @@ lmb_status lmb_scene_create(const lmb_scene_desc* desc, lmb_scene* out) {
 // instead), and traversal stays single-level.  Triangles are numbered
 // instance by instance; instances whose mask has no bit in 0xFF (the ray
 // query's cull mask) are left out.
-lmb_status lmb_scene_create_instanced(const lmb_instanced_desc* desc, lmb_scene* out) {
-    if (!desc || !out) return fail(LMB_ERR_INVALID_ARGUMENT, "NULL argument");
-    *out = nullptr;
-    if (desc->num_geometries > 0 && !desc->geometries)
-        return fail(LMB_ERR_INVALID_ARGUMENT, "geometries is NULL");
-    if (desc->num_instances > 0 && !desc->instances)
-        return fail(LMB_ERR_INVALID_ARGUMENT, "instances is NULL");
-    for (uint32_t gi = 0; gi < desc->num_geometries; ++gi) {
-        const lmb_geometry_desc& g = desc->geometries[gi];
-        if (g.num_aabbs == 0) continue;
-        if (!g.aabbs) return fail(LMB_ERR_INVALID_ARGUMENT, "geometry aabbs are NULL");
-        if (g.num_vertices > 0 || g.num_indices > 0)
-            return fail(LMB_ERR_INVALID_ARGUMENT, "geometry " + std::to_string(gi) +
-                                                      " holds both triangles and AABBs");
-        for (uint64_t b = 0; b < g.num_aabbs; ++b) {
-            const float* a = g.aabbs + 6 * b;
-            for (int c = 0; c < 3; ++c)
-                if (!(a[c] <= a[c + 3]) || !std::isfinite(a[c]) || !std::isfinite(a[c + 3]))
-                    return fail(LMB_ERR_INVALID_ARGUMENT,
-                                "AABB " + std::to_string(b) + " of geometry " +
-                                    std::to_string(gi) + " is not finite with min <= max");
-        }
-    }
-    std::vector<float> verts, obj;
-    std::vector<uint32_t> mat_idx;
-    bool transformed = false;
-    for (uint32_t k = 0; k < desc->num_instances; ++k) {
-        const lmb_instance& in = desc->instances[k];
-        if (in.geometry >= desc->num_geometries)
-            return fail(LMB_ERR_INVALID_ARGUMENT,
-                        "instance " + std::to_string(k) + " names a missing geometry");
-        if ((in.mask & 0xFFu) == 0) continue;
-        const lmb_geometry_desc& g = desc->geometries[in.geometry];
-        if (g.num_aabbs > 0) {
-            // Procedural (AABB) primitives: the reference hands each candidate
-            // to its intersection handler (general.wesl:1068-1076), and the
-            // default handler (default_intersection_handler.wesl) reports no
-            // hit, so they never take part in a closest hit.
-            if (!in.material_idx)
-                return fail(LMB_ERR_INVALID_ARGUMENT,
-                            "instance " + std::to_string(k) + " has no material_idx");
-            continue;
-        }
-        if (g.num_vertices > 0 && !g.vertices)
-            return fail(LMB_ERR_INVALID_ARGUMENT, "geometry vertices are NULL");
-        if (g.indices && g.num_indices % 3 != 0)
-            return fail(LMB_ERR_INVALID_ARGUMENT, "index count is not a multiple of 3");
-        const uint64_t ntri = g.indices ? g.num_indices / 3 : g.num_vertices / 3;
-        if (ntri > 0 && !in.material_idx)
-            return fail(LMB_ERR_INVALID_ARGUMENT,
-                        "instance " + std::to_string(k) + " has no material_idx");
-        const float* m = in.transform;
-        static const float kIdentity[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
-        transformed = transformed || std::memcmp(m, kIdentity, sizeof kIdentity) != 0;
-        const double det = (double)m[0] * ((double)m[5] * m[10] - (double)m[6] * m[9]) -
-                           (double)m[1] * ((double)m[4] * m[10] - (double)m[6] * m[8]) +
-                           (double)m[2] * ((double)m[4] * m[9] - (double)m[5] * m[8]);
-        for (uint64_t t = 0; t < 3 * ntri; ++t) {
-            const uint64_t vi = g.indices ? g.indices[t] : t;
-            if (vi >= g.num_vertices)
-                return fail(LMB_ERR_INVALID_ARGUMENT, "index out of range");
-            const float x = g.vertices[4 * vi], y = g.vertices[4 * vi + 1], z = g.vertices[4 * vi + 2];
-            // 3x4 row-major object-to-world, summed left to right
-            verts.push_back(((m[0] * x + m[1] * y) + m[2] * z) + m[3]);
-            verts.push_back(((m[4] * x + m[5] * y) + m[6] * z) + m[7]);
-            verts.push_back(((m[8] * x + m[9] * y) + m[10] * z) + m[11]);
-            verts.push_back(0.0f);
-            obj.insert(obj.end(), {x, y, z, (t % 3 == 0 && det < 0.0) ? 1.0f : 0.0f});
-        }
-        mat_idx.insert(mat_idx.end(), in.material_idx, in.material_idx + ntri);
-    }
-    lmb_scene_desc flat{};
-    flat.vertices = verts.data();
-    flat.num_vertices = verts.size() / 4;
-    flat.indices = nullptr;
-    flat.num_indices = 0;
-    flat.material_idx = mat_idx.data();
-    flat.materials = desc->materials;
-    flat.num_materials = desc->num_materials;
-    flat.textures = desc->textures;
-    flat.flags = desc->flags;
-    // Identity transforms only: the world-space triangles are the object-
-    // space ones and the shading needs no second copy.
-    return scene_create(&flat, transformed ? &obj : nullptr, out);
-}
-
 lmb_status lmb_scene_destroy(lmb_scene scene) {
     if (!scene) return LMB_OK;
     scene_free(scene);

@@ -203,6 +203,40 @@ int pick_exponent(float lo, float hi) {
 
 }  // namespace
 
+void assign_octant_slots(const float lo[3], const float hi[3], const float (*child_lo)[3],
+                         const float (*child_hi)[3], int n, int slot_of[8]) {
+    float center[3];
+    for (int a = 0; a < 3; ++a) center[a] = 0.5f * (lo[a] + hi[a]);
+    struct Cand {
+        float score;
+        int child, slot;
+    };
+    Cand cands[64];
+    int nc = 0;
+    for (int i = 0; i < n; ++i) {
+        float d[3];
+        for (int a = 0; a < 3; ++a) d[a] = 0.5f * (child_lo[i][a] + child_hi[i][a]) - center[a];
+        for (int s = 0; s < 8; ++s) {
+            float sc = ((s & 1) ? d[0] : -d[0]) + ((s & 2) ? d[1] : -d[1]) +
+                       ((s & 4) ? d[2] : -d[2]);
+            cands[nc++] = {sc, i, s};
+        }
+    }
+    std::stable_sort(cands, cands + nc,
+                     [](const Cand& x, const Cand& y) { return x.score > y.score; });
+    int child_in_slot[8];
+    for (int i = 0; i < 8; ++i) {
+        slot_of[i] = -1;
+        child_in_slot[i] = -1;
+    }
+    for (int k = 0; k < nc; ++k) {
+        const Cand& c = cands[k];
+        if (slot_of[c.child] >= 0 || child_in_slot[c.slot] >= 0) continue;
+        slot_of[c.child] = c.slot;
+        child_in_slot[c.slot] = c.child;
+    }
+}
+
 int build_bvh8(const float* vertices, uint64_t num_vertices, const uint32_t* indices,
                uint64_t num_triangles, BvhBuildResult* out, std::string* err) {
     auto t0 = std::chrono::steady_clock::now();
@@ -299,37 +333,17 @@ int build_bvh8(const float* vertices, uint64_t num_vertices, const uint32_t* ind
         }
 
         // Octant slot assignment.
-        float center[3];
-        for (int a = 0; a < 3; ++a) center[a] = 0.5f * (b.lo[a] + b.hi[a]);
-        struct Cand {
-            float score;
-            int child, slot;
-        };
-        std::vector<Cand> cands;
-        cands.reserve(64);
-        for (int i = 0; i < nch; ++i) {
-            const BNode& c = bn[ch[i]];
-            float d[3];
-            for (int a = 0; a < 3; ++a) d[a] = 0.5f * (c.lo[a] + c.hi[a]) - center[a];
-            for (int s = 0; s < 8; ++s) {
-                float sc = ((s & 1) ? d[0] : -d[0]) + ((s & 2) ? d[1] : -d[1]) +
-                           ((s & 4) ? d[2] : -d[2]);
-                cands.push_back({sc, i, s});
+        float clo[8][3], chi[8][3];
+        for (int i = 0; i < nch; ++i)
+            for (int a = 0; a < 3; ++a) {
+                clo[i][a] = bn[ch[i]].lo[a];
+                chi[i][a] = bn[ch[i]].hi[a];
             }
-        }
-        std::stable_sort(cands.begin(), cands.end(),
-                         [](const Cand& x, const Cand& y) { return x.score > y.score; });
         int slot_of[8];
         int child_in_slot[8];
-        for (int i = 0; i < 8; ++i) {
-            slot_of[i] = -1;
-            child_in_slot[i] = -1;
-        }
-        for (const Cand& c : cands) {
-            if (slot_of[c.child] >= 0 || child_in_slot[c.slot] >= 0) continue;
-            slot_of[c.child] = c.slot;
-            child_in_slot[c.slot] = c.child;
-        }
+        assign_octant_slots(b.lo, b.hi, clo, chi, nch, slot_of);
+        for (int i = 0; i < 8; ++i) child_in_slot[i] = -1;
+        for (int i = 0; i < nch; ++i) child_in_slot[slot_of[i]] = i;
 
         Bvh8Node node;
         std::memset(&node, 0, sizeof node);

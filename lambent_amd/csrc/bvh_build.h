@@ -23,4 +23,11 @@ struct BvhBuildResult {
 int build_bvh8(const float* vertices, uint64_t num_vertices, const uint32_t* indices,
                uint64_t num_triangles, BvhBuildResult* out, std::string* err);
 
+// Slot (0-7) of each of n <= 8 children of a node covering [lo, hi]: greedy
+// maximum alignment of each child's centroid offset from the node centre with
+// the slot's octant direction (bit a of the slot set = +axis a), so that
+// traversal in `slot ^ octant(ray)` order meets near children first.
+void assign_octant_slots(const float lo[3], const float hi[3], const float (*child_lo)[3],
+                         const float (*child_hi)[3], int n, int slot_of[8]);
+
 }  // namespace lmb

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -58,11 +59,46 @@ struct lmb_textures_s {
     bool has_background = false;
 };
 
+namespace lmb {
+
+// One instance of an instanced scene with triangles in the BVH: an
+// instance-pure subtree whose root's node position the top tree decides.
+struct InstancePart {
+    uint32_t instance = 0;             // index in the creation desc
+    uint32_t first_tri = 0, num_tris = 0;  // leaf-order triangle range
+    uint32_t height = 0;               // node height of its root (0: no internal children)
+    uint32_t levels = 0;               // node levels of the subtree
+    uint32_t num_nodes = 0;
+    double sah = 0.0;                  // node areas relative to its root, at creation
+    float obj_lo[3] = {0.f, 0.f, 0.f}, obj_hi[3] = {0.f, 0.f, 0.f};  // object-space box
+};
+
+// What moving the instances of an instanced scene needs (api_instances.cpp).
+struct InstanceState {
+    uint32_t num_instances = 0;
+    std::vector<uint32_t> geometry;    // per desc instance
+    std::vector<int32_t> part_of;      // per desc instance: part index or -1
+    std::vector<uint8_t> hidden_at_creation;  // per desc instance: triangles but mask 0
+    std::vector<InstancePart> parts;
+    std::vector<uint32_t> level_off;   // static refit entries of height h: [off[h], off[h + 1])
+    uint32_t region = 0;               // node positions of the top tree and the part roots
+    DeviceBuffer entries;              // static RefitEntry, by height
+    DeviceBuffer top;                  // RefitEntry of the top tree, by height
+    DeviceBuffer root_pos;             // u32 per part
+    DeviceBuffer first;                // u32 per part + 1: leaf-order triangle ranges
+    DeviceBuffer xf;                   // PartTransform per part
+    DeviceBuffer box;                  // 2 float4 per node position
+    DeviceBuffer error;                // u32
+};
+
+}  // namespace lmb
+
 struct lmb_scene_s {
     int device = 0;
     lmb::SceneDev dev{};
     lmb::DeviceBuffer nodes, tris, prim_material, materials, texels, tex;
     lmb::DeviceBuffer obj_tris;  // instanced scenes: object-space triangles (or empty)
+    std::unique_ptr<lmb::InstanceState> inst;  // instanced scenes only
     lmb::DeviceBuffer fetch;  // ray counter of the query kernels
     lmb::SpillBuffer spill;
     lmb_bvh_stats stats{};

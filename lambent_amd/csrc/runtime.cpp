@@ -161,6 +161,8 @@ void scene_free(lmb_scene s) {
     s->materials.release();
     s->texels.release();
     s->tex.release();
+    s->obj_tris.release();
+    s->inst.reset();
     s->spill.buf.release();
     s->spill.ptr = nullptr;
     s->spill.threads = 0;
