@@ -213,10 +213,13 @@ struct Traversal {
         // The |O| part is folded into the per-axis offsets (near planes use
         // O - 6u|O|, far planes O + 6u|O|, covering the rounding of that fma
         // too), so it never couples axes: a near-parallel axis with a huge
-        // |O| widens only its own planes.  The 2u |t| part is covered per
-        // child by tn - 4u|tn| and tf + 4u|tf| (one fma each, whose own
-        // rounding leaves at least 3u|t| of margin).  Every box the exact slab
-        // test hits is therefore hit here.
+        // |O| widens only its own planes.  The 2u |t| parts of tn and tf are
+        // covered on the far side only: if the exact slab test hits, then
+        // tn - 2u|tn| <= tf + 2u|tf|, which (by the signs of tn and tf) gives
+        // tn <= tf + 4.0001u|tf|, and tf_hi = fl(tf + 6u|tf|) >= tf + 4.99u|tf|;
+        // likewise tn <= best_hi = fl(best + 6u|best|), once per node, and
+        // tmin <= tf + 2u|tf| <= tf_hi.  Every box the exact slab test hits
+        // is therefore hit here.
         const float Sx = fminf(fmaxf(sx * ix, -1e38f), 1e38f);
         const float Sy = fminf(fmaxf(sy * iy, -1e38f), 1e38f);
         const float Sz = fminf(fmaxf(sz * iz, -1e38f), 1e38f);
@@ -227,7 +230,8 @@ struct Traversal {
         const float Onx = fmaf(-fabsf(Ox), kOffsetSlack, Ox), Ofx = fmaf(fabsf(Ox), kOffsetSlack, Ox);
         const float Ony = fmaf(-fabsf(Oy), kOffsetSlack, Oy), Ofy = fmaf(fabsf(Oy), kOffsetSlack, Oy);
         const float Onz = fmaf(-fabsf(Oz), kOffsetSlack, Oz), Ofz = fmaf(fabsf(Oz), kOffsetSlack, Oz);
-        const float best = h.t;
+        constexpr float kChildSlack = 6.0f * 5.9604645e-8f;
+        const float best_hi = fmaf(fabsf(h.t), kChildSlack, h.t);
         uint32_t node_hits = 0, tri_hits = 0;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
@@ -240,14 +244,12 @@ struct Traversal {
             const float tfz = fmaf((float)byte_of(fz0, fz1, s), Sz, Ofz);
             const float tn = fmaxf(fmaxf(tnx, tny), tnz);
             const float tf = fminf(fminf(tfx, tfy), tfz);
-            constexpr float kChildSlack = 4.0f * 5.9604645e-8f;
-            const float tn_lo = fmaf(-fabsf(tn), kChildSlack, tn);
             const float tf_hi = fmaf(fabsf(tf), kChildSlack, tf);
             // Branch-free: an empty slot's meta byte is 0, so a hit on it adds
             // no internal bit and an empty triangle run (a branch per slot
             // cost about ten scalar instructions of exec-mask bookkeeping).
             static_assert(kMetaEmpty == 0, "empty slots must carry no triangles");
-            const bool hit = tn_lo <= fminf(tf_hi, best) && r.tmin <= tf_hi;
+            const bool hit = tn <= fminf(tf_hi, best_hi) && r.tmin <= tf_hi;
             const bool internal = meta == kMetaInternal;
             node_hits |= (hit && internal) ? (1u << ((uint32_t)s ^ oct)) : 0u;
             tri_hits |= (hit && !internal) ? ((meta >> 5) << (meta & 31u)) : 0u;
