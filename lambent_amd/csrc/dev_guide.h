@@ -154,16 +154,23 @@ __device__ __forceinline__ Chain world_chain_read(const WorldMarkovChain* w) {
     return c;
 }
 
-// general.wesl:192-226 grid_load
+// general.wesl:192-226 grid_load.  The reference reads the current cell's
+// chain before testing its key, and a forced or missed key then replaces it
+// with last frame's chain or zero: the current chain is read here only when
+// it is returned (the world candidates force last frame's table 70% of the
+// time, and each skipped read is one or two 128-B lines).  The relaxed loads
+// of the key and the chain are then made in the other order, which is one
+// of the interleavings the reference's racy grid allows.
 __device__ __forceinline__ Chain grid_load(WorldMarkovChain* cur, const WorldMarkovChain* old,
                                            const GridKey& k, bool force_old) {
-    Chain s = world_chain_read(cur + k.idx);
-    if (aload(&cur[k.idx].secondary_hash) != k.secondary || force_old) {
+    Chain s;
+    if (force_old || aload(&cur[k.idx].secondary_hash) != k.secondary) {
         if (aload(&old[k.idx].secondary_hash) == k.secondary)
             s = world_chain_read(old + k.idx);
         else
             s = chain_zero();
     } else {
+        s = world_chain_read(cur + k.idx);
         cur[k.idx].timeout = kCacheTimeout;
     }
     s.ls = strip_nan3(s.ls);
