@@ -3,6 +3,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -92,6 +93,11 @@ struct lmb_data_buffers_s {
     DeviceBuffer own_seed, pix_sum, ps_seed, ps_through, ps_accum;
     DeviceBuffer queue0, queue1, hits, counters;
     DeviceBuffer hits_primary;  // allocated by the first trace of several samples
+    // Plain traces of several samples on a small image trace samples
+    // together: per-path state for grp_paths paths (allocated on first use).
+    DeviceBuffer grp_seed, grp_through, grp_accum, grp_sum, grp_queue0, grp_queue1, grp_hits;
+    SpillBuffer grp_spill;
+    uint64_t grp_paths = 0;
     DeviceBuffer gi, gi_old, info, mc, mc_old, wmc, wmc_old;
     SpillBuffer spill;
     uint64_t gi_count = 0, mc_count = 0, wmc_count = 0;
@@ -214,6 +220,10 @@ lmb_status alloc_window(lmb_data_buffers b, uint32_t width, uint32_t height) {
     std::swap(b->spill.threads, spill.threads);
     std::swap(b->spill.ptr, spill.ptr);
     b->hits_primary.release();
+    for (DeviceBuffer* d : {&b->grp_seed, &b->grp_through, &b->grp_accum, &b->grp_sum,
+                            &b->grp_queue0, &b->grp_queue1, &b->grp_hits})
+        d->release();
+    b->grp_paths = 0;
     b->gi_count = gi_count;
     b->mc_count = mc_count;
     b->width = width;
@@ -451,6 +461,34 @@ lmb_status lmb_data_buffers_kernel_stats(lmb_data_buffers b, lmb_kernel_stats* s
     return LMB_OK;
 }
 
+// Samples the plain tracer traces together on a frame of npix pixels: as
+// many as keep the paths of a group at or under a target (2^21 by default;
+// the environment variable LMB_SAMPLE_GROUP_PATHS overrides it, 0 = one
+// sample at a time).  A 256 x 256 frame fills the GPU only with 16 samples
+// in flight; at 1080p one sample already gives 2M paths.
+static uint32_t sample_group(uint64_t npix, uint32_t samples) {
+    uint64_t target = 1ull << 21;
+    if (const char* env = std::getenv("LMB_SAMPLE_GROUP_PATHS")) target = std::strtoull(env, nullptr, 10);
+    if (npix == 0 || samples < 2) return 1;
+    const uint64_t g = std::min<uint64_t>(samples, target / npix);
+    return g >= 2 ? (uint32_t)g : 1u;
+}
+
+// Per-path state of `paths` grouped sample paths.
+static lmb_status ensure_group(lmb_data_buffers b, uint64_t paths) {
+    if (b->grp_paths >= paths) return LMB_OK;
+    b->grp_paths = 0;
+    lmb_status st;
+    if ((st = b->grp_seed.alloc(paths * 4)) || (st = b->grp_through.alloc(paths * 16)) ||
+        (st = b->grp_accum.alloc(paths * 16)) || (st = b->grp_sum.alloc(paths * 16)) ||
+        (st = b->grp_queue0.alloc(paths * 32)) || (st = b->grp_queue1.alloc(paths * 32)) ||
+        (st = b->grp_hits.alloc(paths * 16)) ||
+        (st = ensure_spill(&b->grp_spill, extend_threads(paths))))
+        return st;
+    b->grp_paths = paths;
+    return LMB_OK;
+}
+
 // The bounce-0 hit buffer that a trace of several samples reuses.
 static lmb_status ensure_primary_hits(lmb_data_buffers b, uint32_t samples) {
     const uint64_t bytes = (uint64_t)b->width * b->height * 16;
@@ -489,7 +527,27 @@ lmb_status lmb_trace(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
         e = launch_guided(scene->dev, cd, bufs->dev, bufs->guided, p, (hipStream_t)stream,
                           bufs->timer.get());
     } else {
-        e = launch_trace(scene->dev, cd, bufs->dev, p, (hipStream_t)stream, bufs->timer.get());
+        const uint32_t group = sample_group(bufs->dev.npix, o.samples);
+        if (group > 1) {
+            lmb_status st = ensure_group(bufs, bufs->dev.npix * group);
+            if (st) return st;
+            FrameDev fg = bufs->dev;
+            fg.ps_seed = bufs->grp_seed.as<uint32_t>();
+            fg.ps_through = bufs->grp_through.as<float4>();
+            fg.ps_accum = bufs->grp_accum.as<float4>();
+            fg.pix_sum = bufs->grp_sum.as<float4>();  // each path's sample colour
+            fg.queue[0] = bufs->grp_queue0.as<float4>();
+            fg.queue[1] = bufs->grp_queue1.as<float4>();
+            fg.hits = bufs->grp_hits.as<uint4>();
+            fg.hits_primary = nullptr;
+            fg.spill = bufs->grp_spill.ptr;
+            fg.own_seed = nullptr;
+            fg.out_color = fg.out_normal = fg.out_albedo = nullptr;
+            e = launch_trace_grouped(scene->dev, cd, bufs->dev, fg, group, p, (hipStream_t)stream,
+                                     bufs->timer.get());
+        } else {
+            e = launch_trace(scene->dev, cd, bufs->dev, p, (hipStream_t)stream, bufs->timer.get());
+        }
     }
     if (e != hipSuccess) return fail_hip(e, "trace launch");
     return LMB_OK;

@@ -65,6 +65,9 @@ struct FrameDev {
     // Paths that size the persistent grids of this frame's launches (0: npix);
     // a pass over a small subset of the pixels runs on a smaller grid.
     uint64_t grid_paths;
+    // Samples of the plain tracer traced together (path id pixel * group +
+    // j); 0 or 1: one sample at a time, path id = pixel.
+    uint32_t group;
 };
 
 inline uint64_t grid_paths(const FrameDev& f) { return f.grid_paths ? f.grid_paths : f.npix; }
@@ -136,5 +139,13 @@ hipError_t launch_debug(const SceneDev& s, const CameraDev& cam, const FrameDev&
                         const TraceParams& p, hipStream_t stream, KernelTimer* timer);
 hipError_t launch_trace(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
                         const TraceParams& p, hipStream_t stream, KernelTimer* timer);
+// The plain tracer with `group` samples traced together: fg is the frame
+// with the group's per-path state (ps_seed, ps_through, ps_accum, queues,
+// hits and spill for npix * group paths) and pix_sum = each path's sample
+// colour; the image, the camera queue and hits_primary are f's.  Images are
+// those of launch_trace: a pixel's sample colours are summed in sample order.
+hipError_t launch_trace_grouped(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
+                                const FrameDev& fg, uint32_t group, const TraceParams& p,
+                                hipStream_t stream, KernelTimer* timer);
 
 }  // namespace lmb

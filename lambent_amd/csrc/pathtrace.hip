@@ -174,14 +174,20 @@ static void launch_extend_kernel(const SceneDev& s, const FrameDev& f, const Tra
 
 // kTable: the queue's rays carry a scene index (direction .w) into `table`
 // (the batched tracer); otherwise every ray belongs to `s1`.
-template <uint32_t Q, bool kTable>
+// kGroup: f.group samples traced together (path id pixel * group + j).  At
+// bounce 0 queue slot w is sample j = w % group of camera slot w / group
+// (the camera queue and its hits are shared by the group's samples); a path
+// that ends stores its sample colour in pix_sum[path id].
+template <uint32_t Q, bool kTable, bool kGroup = false>
 __global__ __launch_bounds__(kPtBlock) void k_shade_plain_t(SceneDev s1, const SceneDev* table,
                                                             FrameDev f, TraceParams p,
                                                             uint32_t bounce) {
     const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
-    const uint32_t count = f.counters[bounce];
-    const uint64_t count_r = block_uniform_count<kPtBlock>(count);
+    const uint32_t group = kGroup ? f.group : 1u;
+    const bool group_camera = kGroup && bounce == 0;
+    const uint64_t count = group_camera ? f.npix * group : (uint64_t)f.counters[bounce];
+    const uint64_t count_r = ((uint64_t)count + kPtBlock - 1) / kPtBlock * kPtBlock;
     __shared__ ReserveLds<kPtBlock, 1> rl;
     const float4* __restrict__ qin = f.queue[bounce & 1];
     float4* __restrict__ qout = f.queue[(bounce + 1) & 1];
@@ -189,10 +195,13 @@ __global__ __launch_bounds__(kPtBlock) void k_shade_plain_t(SceneDev s1, const S
         bool cont = false;
         float4 out_o = make_float4(0.f, 0.f, 0.f, 0.f), out_d = out_o;
         if (slot < count) {
-            const float4 o4 = qin[2 * slot];
-            const float4 d4 = qin[2 * slot + 1];
-            const uint32_t pid = __float_as_uint(o4.w);
-            const uint4 h = f.hits[slot];
+            const uint64_t qs = group_camera ? slot / group : slot;
+            const uint32_t j = group_camera ? (uint32_t)(slot - qs * group) : 0u;
+            const float4 o4 = qin[2 * qs];
+            const float4 d4 = qin[2 * qs + 1];
+            const uint32_t pix = __float_as_uint(o4.w);  // path id (camera queue: pixel)
+            const uint32_t pid = group_camera ? pix * group + j : pix;
+            const uint4 h = f.hits[qs];
             uint32_t seed = f.ps_seed[pid];
             const float4 th = f.ps_through[pid];
             const float4 ac = f.ps_accum[pid];
@@ -208,9 +217,9 @@ __global__ __launch_bounds__(kPtBlock) void k_shade_plain_t(SceneDev s1, const S
                 const Isect it = make_isect(s, h.x, t, __uint_as_float(h.z),
                                             __uint_as_float(h.w), dir);
                 const V3 new_pos = fma3(dir, v3s(t), org);
-                if (bounce == 0) {
-                    f.out_normal[pid] = make_float4(it.normal.x, it.normal.y, it.normal.z, 1.f);
-                    f.out_albedo[pid] = make_float4(it.color.x, it.color.y, it.color.z, 1.f);
+                if (bounce == 0 && j == 0) {  // every sample's camera hit is the same
+                    f.out_normal[pix] = make_float4(it.normal.x, it.normal.y, it.normal.z, 1.f);
+                    f.out_albedo[pix] = make_float4(it.color.x, it.color.y, it.color.z, 1.f);
                 }
                 // general.wesl:714-901
                 scol = fma3(color, it.emission, scol);
@@ -232,20 +241,26 @@ __global__ __launch_bounds__(kPtBlock) void k_shade_plain_t(SceneDev s1, const S
                     f.ps_seed[pid] = seed;
                     f.ps_through[pid] = make_float4(color.x, color.y, color.z, picked);
                     f.ps_accum[pid] = make_float4(scol.x, scol.y, scol.z, 0.f);
+                } else if (kGroup) {
+                    f.pix_sum[pid] = make_float4(scol.x, scol.y, scol.z, 0.f);
                 } else {
                     float4 ps = f.pix_sum[pid];
                     f.pix_sum[pid] = make_float4(ps.x + scol.x, ps.y + scol.y, ps.z + scol.z, 0.f);
                 }
             } else {
-                if (bounce == 0) {
-                    f.out_normal[pid] = make_float4(0.f, 0.f, 0.f, 1.f);
-                    f.out_albedo[pid] = make_float4(0.f, 0.f, 0.f, 1.f);
+                if (bounce == 0 && j == 0) {
+                    f.out_normal[pix] = make_float4(0.f, 0.f, 0.f, 1.f);
+                    f.out_albedo[pix] = make_float4(0.f, 0.f, 0.f, 1.f);
                 }
                 // general.wesl:915-938
                 const V3 bg = background(s, dir);
                 scol = fma3(color, bg, scol);
-                float4 ps = f.pix_sum[pid];
-                f.pix_sum[pid] = make_float4(ps.x + scol.x, ps.y + scol.y, ps.z + scol.z, 0.f);
+                if (kGroup) {
+                    f.pix_sum[pid] = make_float4(scol.x, scol.y, scol.z, 0.f);
+                } else {
+                    float4 ps = f.pix_sum[pid];
+                    f.pix_sum[pid] = make_float4(ps.x + scol.x, ps.y + scol.y, ps.z + scol.z, 0.f);
+                }
             }
         }
         // Compact the survivors into the next queue: one atomic per block.
@@ -318,6 +333,124 @@ static hipError_t launch_plain(const SceneDev& s, const CameraDev& cam, const Fr
     }
     hipLaunchKernelGGL(k_resolve, grid, block, 0, stream, f, p);
     return hipGetLastError();
+}
+
+// ------------------------------------------------------- grouped samples
+// Samples s0 .. s0 + group - 1 of every pixel start together.  A sample's
+// seeds depend only on the previous sample's start seed (general.wesl:516,
+// 544), so they are drawn here for the whole group: path pixel * group + j
+// gets the start seed of sample s0 + j.  The first group also writes the
+// camera rays (traced once for all samples: no jitter) and clears pix_sum.
+__global__ __launch_bounds__(kPtBlock) void k_generate_group(CameraDev cam, FrameDev f,
+                                                             FrameDev fg, TraceParams p,
+                                                             uint32_t s0) {
+    const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
+    const uint32_t group = fg.group;
+    if (gtid == 0) {
+        f.counters[0] = (uint32_t)f.npix;
+        for (uint32_t i = 1; i <= kMaxBounces; ++i) f.counters[i] = 0;
+        for (uint32_t k = 0; k < 2 * kFetchSegments; ++k) f.counters[kCounterFetch + k * kFetchStride] = 0;
+        if (s0 == 0) *vertex_counter(f) = 0;
+    }
+    for (uint64_t slot = gtid; slot < f.npix; slot += stride) {
+        uint32_t x, y;
+        camera_slot_pixel(slot, f.width, f.height, x, y);
+        const uint64_t pix = (uint64_t)y * f.width + x;
+        uint32_t seed = s0 == 0 ? rand_u32(x * y) + p.seed_offset + y : rand_u32(f.own_seed[pix]);
+        for (uint32_t j = 0; j < group; ++j) {
+            if (j) seed = rand_u32(seed);
+            const uint64_t pid = pix * group + j;
+            fg.ps_seed[pid] = seed;
+            fg.ps_through[pid] = make_float4(1.f, 1.f, 1.f, -1.f);
+            fg.ps_accum[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        f.own_seed[pix] = seed;
+        if (s0 == 0) {
+            f.pix_sum[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+            float o[3], d[3];
+            create_ray(cam, x, y, f.width, f.height, o, d);
+            f.queue[0][2 * slot] = make_float4(o[0], o[1], o[2], __uint_as_float((uint32_t)pix));
+            f.queue[0][2 * slot + 1] = make_float4(d[0], d[1], d[2], 0.f);
+        }
+    }
+}
+
+// pix_sum += the group's sample colours, in sample order (as the samples
+// add them one after another when traced one at a time).
+__global__ __launch_bounds__(kPtBlock) void k_group_sum(FrameDev f, const float4* __restrict__ colour,
+                                                        uint32_t group) {
+    const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
+    for (uint64_t pix = gtid; pix < f.npix; pix += stride) {
+        float4 ps = f.pix_sum[pix];
+        for (uint32_t j = 0; j < group; ++j) {
+            const float4 c = colour[pix * group + j];
+            ps = make_float4(ps.x + c.x, ps.y + c.y, ps.z + c.z, 0.f);
+        }
+        f.pix_sum[pix] = ps;
+    }
+}
+
+template <uint32_t Q>
+static hipError_t launch_plain_grouped(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
+                                       const FrameDev& fg, uint32_t group, const TraceParams& p,
+                                       hipStream_t stream, KernelTimer* tm) {
+    const dim3 block(kPtBlock);
+    const dim3 pgrid((uint32_t)(frame_threads(f.npix) / kPtBlock));
+    const FrameDev f0 = primary_view(f, p);  // the camera rays' hits: hits_primary
+    for (uint32_t s0 = 0; s0 < p.samples; s0 += group) {
+        FrameDev g = fg;
+        g.group = std::min(group, p.samples - s0);
+        const uint64_t paths = f.npix * g.group;
+        g.grid_paths = paths;
+        const dim3 grid((uint32_t)(frame_threads(paths) / kPtBlock));
+        const dim3 xgrid((uint32_t)(extend_threads(paths) / kPtBlock));
+        hipLaunchKernelGGL(k_generate_group, pgrid, block, 0, stream, cam, f, g, p, s0);
+        if (s0 == 0) {
+            if (tm) tm->begin(kKernelExtend, stream);
+            launch_extend_kernel(s, f0, p, 0, dim3((uint32_t)(extend_threads(f.npix) / kPtBlock)),
+                                 stream);
+            if (tm) tm->end(kKernelExtend, stream);
+        }
+        // bounce 0 reads the camera queue and hits; later bounces the group's
+        FrameDev g0 = g;
+        g0.queue[0] = f0.queue[0];
+        g0.hits = f0.hits;
+        g0.out_normal = f.out_normal;
+        g0.out_albedo = f.out_albedo;
+        for (uint32_t b = 0; b < p.bounces; ++b) {
+            if (b > 0) {
+                if (tm) tm->begin(kKernelExtend, stream);
+                launch_extend_kernel(s, g, p, b, xgrid, stream);
+                if (tm) tm->end(kKernelExtend, stream);
+            }
+            if (tm) tm->begin(kKernelShade, stream);
+            hipLaunchKernelGGL((k_shade_plain_t<Q, false, true>), grid, block, 0, stream, s,
+                               (const SceneDev*)nullptr, b == 0 ? g0 : g, p, b);
+            if (tm) tm->end(kKernelShade, stream);
+        }
+        hipLaunchKernelGGL(k_group_sum, pgrid, block, 0, stream, f, (const float4*)g.pix_sum,
+                           g.group);
+    }
+    hipLaunchKernelGGL(k_resolve, pgrid, block, 0, stream, f, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_grouped(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
+                                const FrameDev& fg, uint32_t group, const TraceParams& p,
+                                hipStream_t stream, KernelTimer* timer) {
+    if (f.npix == 0) return hipSuccess;
+    switch (p.quality) {
+        case kQualityLow:
+            return launch_plain_grouped<kQualityLow>(s, cam, f, fg, group, p, stream, timer);
+        case kQualityHigh:
+            return launch_plain_grouped<kQualityHigh>(s, cam, f, fg, group, p, stream, timer);
+        case kQualitySpectral:
+            return launch_plain_grouped<kQualitySpectral>(s, cam, f, fg, group, p, stream, timer);
+        default:
+            return launch_plain_grouped<kQualityMedium>(s, cam, f, fg, group, p, stream, timer);
+    }
 }
 
 // ---------------------------------------------------------------- batches

@@ -441,3 +441,28 @@ def test_aabb_instances_never_hit_with_the_default_handler():
         pipe.dispatch(scene, base.camera(), bufs, 4)
         out.append(bufs.output(api.Output.COLOR))
     np.testing.assert_array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("quality", [api.Low, api.Medium, api.Spectral])
+@pytest.mark.parametrize("rr_start", [3, 99])
+def test_grouped_samples_equal_one_at_a_time(quality, rr_start, monkeypatch):
+    """Samples traced together (path id pixel * group + j, small frames) give
+    the image, normals, albedo and vertex count of the samples traced one
+    after another: the start seeds chain per pixel (general.wesl:516, 544)
+    and a pixel's sample colours are added in sample order.  Groups of 16
+    and of 6 (a short last group for 16 samples)."""
+    spec = scenes.cornell_box(64, 48)
+    scene = spec.build()
+    pipe = api.RayTracer(quality).create_pipeline(
+        api.PathTracerOpts(samples=16, bounces=8, rr_start=rr_start, guiding=api.Guiding.OFF))
+    out = []
+    for paths in ("0", str(64 * 48 * 16), str(64 * 48 * 6)):
+        monkeypatch.setenv("LMB_SAMPLE_GROUP_PATHS", paths)
+        bufs = api.DataBuffers(64, 48, 1)
+        pipe.dispatch(scene, spec.camera(), bufs, 7)
+        out.append([bufs.output(o) for o in (api.Output.COLOR, api.Output.NORMAL,
+                                             api.Output.ALBEDO)] + [bufs.path_vertices()])
+    for other in out[1:]:
+        for a, b in zip(out[0][:3], other[:3]):
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+        assert out[0][3] == other[3]
