@@ -486,8 +486,12 @@ static lmb_status ensure_group(lmb_data_buffers b, uint64_t paths) {
         (st = b->grp_accum.alloc(paths * 16)) || (st = b->grp_sum.alloc(paths * 16)) ||
         (st = b->grp_queue0.alloc(paths * 32)) || (st = b->grp_queue1.alloc(paths * 32)) ||
         (st = b->grp_hits.alloc(paths * 16)) ||
-        (st = ensure_spill(&b->grp_spill, extend_threads(paths))))
+        (st = ensure_spill(&b->grp_spill, extend_threads(paths)))) {
+        for (DeviceBuffer* d : {&b->grp_seed, &b->grp_through, &b->grp_accum, &b->grp_sum,
+                                &b->grp_queue0, &b->grp_queue1, &b->grp_hits})
+            d->release();
         return st;
+    }
     b->grp_paths = paths;
     return LMB_OK;
 }
@@ -530,10 +534,13 @@ lmb_status lmb_trace(lmb_tracer tracer, lmb_scene scene, const lmb_camera* cam,
         e = launch_guided(scene->dev, cd, bufs->dev, bufs->guided, p, (hipStream_t)stream,
                           bufs->timer.get());
     } else {
-        const uint32_t group = sample_group(bufs->dev.npix, o.samples);
+        uint32_t group = sample_group(bufs->dev.npix, o.samples);
+        // Without the memory for a group, the samples go one at a time.
+        if (group > 1 && ensure_group(bufs, bufs->dev.npix * group) != LMB_OK) {
+            group = 1;
+            clear_error();
+        }
         if (group > 1) {
-            lmb_status st = ensure_group(bufs, bufs->dev.npix * group);
-            if (st) return st;
             FrameDev fg = bufs->dev;
             fg.ps_seed = bufs->grp_seed.as<uint32_t>();
             fg.ps_through = bufs->grp_through.as<float4>();

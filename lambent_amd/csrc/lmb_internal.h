@@ -17,6 +17,7 @@
 namespace lmb {
 
 lmb_status fail(lmb_status code, const std::string& msg);
+void clear_error();  // forgets the last error message (a failure handled internally)
 lmb_status fail_hip(hipError_t e, const char* what);
 
 // Owning device allocation.
