@@ -357,3 +357,41 @@ def test_moving_a_million_triangles_costs_milliseconds():
         w = [((tf[r, 0] * x + tf[r, 1] * y) + tf[r, 2] * z) + tf[r, 3] for r in range(3)]
         flat.append(np.stack(w + [np.zeros_like(x)], 1).astype(np.float32))
     _same_hits(scene.intersect(rays), oracle.Bvh(np.concatenate(flat), None).intersect(rays))
+
+
+def test_updates_reach_batches_and_the_other_tracers():
+    """A batch made before an update renders the moved instances (its scene
+    table holds the same buffers), equal to per-image traces of the updated
+    scene; guided and cached traces of the updated scene stay finite."""
+    room = Room(n_soups=3)
+    scene = room.scene(room.instances(0.0))
+    cam = room.base.camera()
+    opts = api.PathTracerOpts(samples=2, bounces=4, guiding=api.Guiding.OFF,
+                              restir_temporal=api.RestirTemporalMode.Disabled)
+    batch = api.Batch([(scene, cam, 64, 64)], opts, api.Medium)
+    batch.trace(3)
+    before = batch.output(0)
+    scene.update_instances(room.instances(1.3))
+    batch.trace(3)
+    after = batch.output(0)
+    assert not np.array_equal(before, after)
+    pipe = api.RayTracer(api.Medium).create_pipeline(opts)
+    bufs = api.DataBuffers(64, 64, 1, api.BufferType(0))
+    pipe.dispatch(scene, cam, bufs, 3)
+    np.testing.assert_array_equal(after, bufs.output(api.Output.COLOR))
+    guided = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=1, bounces=5, guiding=api.Guiding.MARKOV_CHAIN))
+    gb = api.DataBuffers(64, 64, 10_000, api.BufferType.all())
+    for f in range(3):
+        guided.dispatch(scene, cam, gb, f)
+        gb.advance_frame(api.BufferType.all(), api.AdvanceOptions(20))
+    assert np.isfinite(gb.output(api.Output.COLOR)).all()
+    cache = api.NeuralCache(api.NeuralCacheOpts(train_paths=4096, train_steps=4))
+    cached = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=2, bounces=5, guiding=api.Guiding.OFF))
+    cb = api.DataBuffers(64, 64, 1, api.BufferType(0))
+    for f in range(3):
+        scene.update_instances(room.instances(1.3 + 0.2 * f))
+        cached.dispatch_cached(scene, cam, cb, cache, f)
+    img = cb.output(api.Output.COLOR)
+    assert np.isfinite(img).all() and img[..., :3].mean() > 0
