@@ -338,9 +338,17 @@ static hipError_t launch_plain(const SceneDev& s, const CameraDev& cam, const Fr
 // ------------------------------------------------------- grouped samples
 // Samples s0 .. s0 + group - 1 of every pixel start together.  A sample's
 // seeds depend only on the previous sample's start seed (general.wesl:516,
-// 544), so they are drawn here for the whole group: path pixel * group + j
-// gets the start seed of sample s0 + j.  The first group also writes the
-// camera rays (traced once for all samples: no jitter) and clears pix_sum.
+// 544), so they are drawn here for the whole group, one thread per path:
+// path pixel * group + j gets the start seed of sample s0 + j (own_seed is
+// advanced past the group by k_group_sum).  The first group also writes
+// the camera rays (traced once for all samples: no jitter) and clears
+// pix_sum.
+__device__ __forceinline__ uint32_t first_start_seed(const FrameDev& f, const TraceParams& p,
+                                                     uint32_t s0, uint64_t pix) {
+    const uint32_t x = (uint32_t)(pix % f.width), y = (uint32_t)(pix / f.width);
+    return s0 == 0 ? rand_u32(x * y) + p.seed_offset + y : rand_u32(f.own_seed[pix]);
+}
+
 __global__ __launch_bounds__(kPtBlock) void k_generate_group(CameraDev cam, FrameDev f,
                                                              FrameDev fg, TraceParams p,
                                                              uint32_t s0) {
@@ -353,32 +361,34 @@ __global__ __launch_bounds__(kPtBlock) void k_generate_group(CameraDev cam, Fram
         for (uint32_t k = 0; k < 2 * kFetchSegments; ++k) f.counters[kCounterFetch + k * kFetchStride] = 0;
         if (s0 == 0) *vertex_counter(f) = 0;
     }
+    const uint64_t paths = f.npix * group;
+    for (uint64_t pid = gtid; pid < paths; pid += stride) {
+        const uint64_t pix = pid / group;
+        const uint32_t j = (uint32_t)(pid - pix * group);
+        uint32_t seed = first_start_seed(f, p, s0, pix);
+        for (uint32_t k = 0; k < j; ++k) seed = rand_u32(seed);
+        fg.ps_seed[pid] = seed;
+        fg.ps_through[pid] = make_float4(1.f, 1.f, 1.f, -1.f);
+        fg.ps_accum[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (s0 != 0) return;
     for (uint64_t slot = gtid; slot < f.npix; slot += stride) {
         uint32_t x, y;
         camera_slot_pixel(slot, f.width, f.height, x, y);
         const uint64_t pix = (uint64_t)y * f.width + x;
-        uint32_t seed = s0 == 0 ? rand_u32(x * y) + p.seed_offset + y : rand_u32(f.own_seed[pix]);
-        for (uint32_t j = 0; j < group; ++j) {
-            if (j) seed = rand_u32(seed);
-            const uint64_t pid = pix * group + j;
-            fg.ps_seed[pid] = seed;
-            fg.ps_through[pid] = make_float4(1.f, 1.f, 1.f, -1.f);
-            fg.ps_accum[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        f.own_seed[pix] = seed;
-        if (s0 == 0) {
-            f.pix_sum[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
-            float o[3], d[3];
-            create_ray(cam, x, y, f.width, f.height, o, d);
-            f.queue[0][2 * slot] = make_float4(o[0], o[1], o[2], __uint_as_float((uint32_t)pix));
-            f.queue[0][2 * slot + 1] = make_float4(d[0], d[1], d[2], 0.f);
-        }
+        f.pix_sum[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+        float o[3], d[3];
+        create_ray(cam, x, y, f.width, f.height, o, d);
+        f.queue[0][2 * slot] = make_float4(o[0], o[1], o[2], __uint_as_float((uint32_t)pix));
+        f.queue[0][2 * slot + 1] = make_float4(d[0], d[1], d[2], 0.f);
     }
 }
 
 // pix_sum += the group's sample colours, in sample order (as the samples
-// add them one after another when traced one at a time).
+// add them one after another when traced one at a time), and own_seed =
+// the start seed of the group's last sample.
 __global__ __launch_bounds__(kPtBlock) void k_group_sum(FrameDev f, const float4* __restrict__ colour,
+                                                        TraceParams p, uint32_t s0,
                                                         uint32_t group) {
     const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
@@ -389,6 +399,9 @@ __global__ __launch_bounds__(kPtBlock) void k_group_sum(FrameDev f, const float4
             ps = make_float4(ps.x + c.x, ps.y + c.y, ps.z + c.z, 0.f);
         }
         f.pix_sum[pix] = ps;
+        uint32_t seed = first_start_seed(f, p, s0, pix);
+        for (uint32_t j = 1; j < group; ++j) seed = rand_u32(seed);
+        f.own_seed[pix] = seed;
     }
 }
 
@@ -406,7 +419,7 @@ static hipError_t launch_plain_grouped(const SceneDev& s, const CameraDev& cam, 
         g.grid_paths = paths;
         const dim3 grid((uint32_t)(frame_threads(paths) / kPtBlock));
         const dim3 xgrid((uint32_t)(extend_threads(paths) / kPtBlock));
-        hipLaunchKernelGGL(k_generate_group, pgrid, block, 0, stream, cam, f, g, p, s0);
+        hipLaunchKernelGGL(k_generate_group, grid, block, 0, stream, cam, f, g, p, s0);
         if (s0 == 0) {
             if (tm) tm->begin(kKernelExtend, stream);
             launch_extend_kernel(s, f0, p, 0, dim3((uint32_t)(extend_threads(f.npix) / kPtBlock)),
@@ -430,8 +443,8 @@ static hipError_t launch_plain_grouped(const SceneDev& s, const CameraDev& cam, 
                                (const SceneDev*)nullptr, b == 0 ? g0 : g, p, b);
             if (tm) tm->end(kKernelShade, stream);
         }
-        hipLaunchKernelGGL(k_group_sum, pgrid, block, 0, stream, f, (const float4*)g.pix_sum,
-                           g.group);
+        hipLaunchKernelGGL(k_group_sum, pgrid, block, 0, stream, f, (const float4*)g.pix_sum, p,
+                           s0, g.group);
     }
     hipLaunchKernelGGL(k_resolve, pgrid, block, 0, stream, f, p);
     return hipGetLastError();
