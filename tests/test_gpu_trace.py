@@ -466,3 +466,25 @@ def test_grouped_samples_equal_one_at_a_time(quality, rr_start, monkeypatch):
         for a, b in zip(out[0][:3], other[:3]):
             np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
         assert out[0][3] == other[3]
+
+
+def test_grouped_samples_on_an_odd_frame(monkeypatch):
+    """A 37 x 29 frame (a partial last band of camera slots), 7 samples in
+    groups of 3 (3 + 3 + 1), against one sample at a time and the oracle."""
+    spec = scenes.cornell_box(37, 29)
+    scene = spec.build()
+    pipe = api.RayTracer(api.Medium).create_pipeline(
+        api.PathTracerOpts(samples=7, bounces=6, rr_start=2, guiding=api.Guiding.OFF))
+    out = []
+    for paths in ("0", str(37 * 29 * 3)):
+        monkeypatch.setenv("LMB_SAMPLE_GROUP_PATHS", paths)
+        bufs = api.DataBuffers(37, 29, 1)
+        pipe.dispatch(scene, spec.camera(), bufs, 11)
+        out.append(bufs.output(api.Output.COLOR))
+    np.testing.assert_array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
+    c_o, _, _, _ = oracle.Scene(spec).trace_plain(spec.camera().to_bytes(), 37, 29, quality=1,
+                                                   samples=7, bounces=6, rr_start=2,
+                                                   seed_offset=11)
+    close = np.all(np.abs(out[1][..., :3] - c_o[..., :3]) <= 1e-4 * (1 + np.abs(c_o[..., :3])),
+                   axis=-1)
+    assert close.mean() > 0.97, close.mean()
