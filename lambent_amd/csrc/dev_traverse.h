@@ -271,6 +271,15 @@ struct Traversal {
             ghits = e.y & 0xFFu;
             gimask = e.y >> 8;
         }
+        descend(gbase, ghits, gimask, st);
+    }
+
+    // Goes to the nearest child of a hit set (ghits: hit children in the
+    // ray's octant order, never empty; gimask: the node's internal-child
+    // mask, whose children lie contiguously from gbase) and pushes the rest.
+    template <int BLOCK>
+    __device__ __forceinline__ void descend(uint32_t gbase, uint32_t ghits, uint32_t gimask,
+                                            ShortStack<BLOCK>& st) {
         const uint32_t key = __builtin_ctz(ghits);
         ghits &= ghits - 1u;
         if (ghits) st.push(make_uint2(gbase, ghits | (gimask << 8)));
@@ -325,6 +334,7 @@ struct TriShare {
     unsigned long long* key;  // (orderable t << 32) | primitive id of the best hit
     float2* uv;
     uint32_t* leaf;
+    uint32_t* busy;  // stamp of the last pass in which a helper of this lane's ray ran
 };
 
 // Order-preserving map of floats to unsigned integers (NaN excluded).
@@ -352,20 +362,41 @@ __device__ __forceinline__ uint32_t nth_set_bit(uint32_t m, uint32_t j) {
     return b;
 }
 
+// Position of the j-th set bit (j from 0) of a 64-bit mask with more than j
+// bits set.
+__device__ __forceinline__ uint32_t nth_set_bit64(uint64_t m, uint32_t j) {
+    uint32_t w = (uint32_t)m, b = 0, c = __popc(w);
+    if (j >= c) { j -= c; w = (uint32_t)(m >> 32); b = 32; }
+    c = __popc(w & 0xFFFFu);
+    if (j >= c) { j -= c; w >>= 16; b += 16; }
+    c = __popc(w & 0xFFu);
+    if (j >= c) { j -= c; w >>= 8; b += 8; }
+    c = __popc(w & 0xFu);
+    if (j >= c) { j -= c; w >>= 4; b += 4; }
+    c = __popc(w & 0x3u);
+    if (j >= c) { j -= c; w >>= 2; b += 2; }
+    if (j >= (w & 1u)) b += 1;
+    return b;
+}
+
 // Shared triangle phase, called by all 64 lanes of a wave.  Lane l owns the
 // triangles tbase_l + bit of tmask_l; the wave's (owner, triangle) pairs are
 // spread over all lanes, 64 per round, each tested against its owner's ray
-// (fetched by lane shuffles).  Per owner the closest hit is the minimum of
+// (fetched by lane shuffles).  Per ray the closest hit is the minimum of
 // (t, primitive id) - the order the one-lane loop applies (nearer t first,
-// then the lower primitive id) - taken with a 64-bit LDS atomicMin; the lane
-// holding the winning triangle then writes its (u, v) and leaf index.
+// then the lower primitive id) - taken with a 64-bit LDS atomicMin on the
+// hit record of lane key_lane_l (the lane itself, or for a helper the lane
+// whose ray it traverses); the lane holding the winning triangle then writes
+// its (u, v) and leaf index there.  any_helper: some lane of the wave is a
+// helper (wave-uniform).
 template <class Scenes>
 __device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t tbase, uint32_t tmask,
                                                  const Scenes& sc, const TriShare& ts,
-                                                 uint32_t lane) {
+                                                 uint32_t lane, uint32_t key_lane, bool helper,
+                                                 bool any_helper) {
     const uint32_t c = __popc(tmask);
     if (!__ballot(c != 0u)) return;
-    if (!__ballot(c > 1u)) {
+    if (!any_helper && !__ballot(c > 1u)) {
         // at most one triangle per lane: each lane tests its own in place
         if (c) {
             const uint32_t leaf = tbase + (uint32_t)__builtin_ctz(tmask);
@@ -419,6 +450,7 @@ __device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t tbase, 
         sh.sy = __shfl(tv.sh.sy, (int)owner);
         sh.sz = __shfl(tv.sh.sz, (int)owner);
         const uint32_t ok = __shfl(kxyz, (int)owner);
+        const uint32_t o_key = __shfl(key_lane, (int)owner);
         const LeafTri* __restrict__ tris = sc.tris_of(owner);
         sh.kx = (int)(ok & 3u);
         sh.ky = (int)((ok >> 2) & 3u);
@@ -431,20 +463,20 @@ __device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t tbase, 
             leaf = o_base + nth_set_bit(o_mask, i - o_off);
             const float4* tp = reinterpret_cast<const float4*>(tris + leaf);
             const float4 a = tp[0], b = tp[1], cc = tp[2];
-            const float tmax = from_orderable((uint32_t)(ts.key[owner] >> 32));
+            const float tmax = from_orderable((uint32_t)(ts.key[o_key] >> 32));
             bool found = intersect_triangle(sh, ox, oy, oz, a, b, cc, otmin, tmax, &t, &u, &v);
             if constexpr (Scenes::kAlphaTest) found = found && sc.accept(b, u, v);
             if (found) {
                 hit = true;
                 mykey = ((unsigned long long)orderable(t) << 32) | __float_as_uint(a.w);
-                atomicMin(ts.key + owner, mykey);
+                atomicMin(ts.key + o_key, mykey);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        if (hit && ts.key[owner] == mykey) {
-            ts.uv[owner] = make_float2(u, v);
-            ts.leaf[owner] = leaf;
+        if (hit && ts.key[o_key] == mykey) {
+            ts.uv[o_key] = make_float2(u, v);
+            ts.leaf[o_key] = leaf;
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -458,6 +490,8 @@ __device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t tbase, 
         tv.h.v = uv.y;
         tv.h.leaf = ts.leaf[lane];
     }
+    // a helper prunes with the best distance found so far for its ray
+    if (helper) tv.h.t = from_orderable((uint32_t)(ts.key[key_lane] >> 32));
 }
 
 // Persistent traversal over a ray queue with wave-level refill: whenever at
@@ -466,6 +500,17 @@ __device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t tbase, 
 // not wait for the longest ray of the wave.  `fetch`: kFetchSegments zeroed
 // counters kFetchStride words apart, one per queue segment, counting the rays
 // handed out there.
+//
+// Once the queue is drained the wave's idle lanes help the lanes still
+// traversing: each pass, the k-th idle lane takes the top stack entry (a set
+// of hit children of one node) of the k-th lane with a non-empty stack and
+// traverses those subtrees for the same ray, as a helper.  A helper's
+// triangles update its ray's hit record (the LDS key of the owning lane), it
+// prunes with that record's distance, and it can itself donate.  A ray's
+// result is stored once its own lane has finished and no helper of it is
+// still traversing.  The closest hit (and the tie rule) does not depend on
+// which lane visits which subtree, so the result is unchanged; the wave's
+// tail - lanes waiting for the longest ray - gets shorter.
 // Source must provide: Ray load(uint64_t slot, uint32_t* scene),
 // void store(uint64_t slot, Hit).
 template <int BLOCK, class Scenes, class Source>
@@ -484,7 +529,13 @@ __device__ __forceinline__ void traverse_queue_t(Scenes& sc, uint64_t count, uin
     Traversal tv;
     tv.active = false;
     uint64_t slot = 0;
-    bool exhausted = false;  // wave-uniform
+    uint32_t item = 0;
+    uint32_t key_lane = lane;  // lane holding the hit record of this lane's ray
+    bool helper = false;       // traversing subtrees of key_lane's ray
+    bool pending = false;      // own traversal done, helpers of this ray still running
+    bool exhausted = false;    // wave-uniform
+    uint32_t stamp = 0;        // wave-uniform
+    ts.busy[lane] = ~0u;
     while (true) {
         if (!exhausted) {
             const uint64_t idle = __ballot(!tv.active);
@@ -506,7 +557,6 @@ __device__ __forceinline__ void traverse_queue_t(Scenes& sc, uint64_t count, uin
                     const uint64_t mine = lo + base + rank;
                     if (mine < hi) {
                         slot = mine;
-                        uint32_t item = 0;
                         const Ray r = src.load(slot, &item);
                         sc.bind(item);
                         tv.init(r, sc.num_nodes());
@@ -515,13 +565,81 @@ __device__ __forceinline__ void traverse_queue_t(Scenes& sc, uint64_t count, uin
                     }
                 }
             }
+        } else {
+            const uint64_t idle = __ballot(!tv.active && !pending);
+            const uint64_t donors = __ballot(tv.active && st.sp > 0);
+            if (idle && donors) {
+                const uint32_t n = (uint32_t)min(__popcll(idle), __popcll(donors));
+                const uint64_t below = (1ull << lane) - 1ull;
+                uint2 e = make_uint2(0u, 0u);
+                if (((donors >> lane) & 1ull) && (uint32_t)__popcll(donors & below) < n) e = st.pop();
+                const uint32_t idle_rank = (uint32_t)__popcll(idle & below);
+                const bool takes = ((idle >> lane) & 1ull) && idle_rank < n;
+                const int from = takes ? (int)nth_set_bit64(donors, idle_rank) : (int)lane;
+                const uint32_t kxyz = (uint32_t)tv.sh.kx | ((uint32_t)tv.sh.ky << 2) |
+                                      ((uint32_t)tv.sh.kz << 4) | (tv.oct << 6) | (key_lane << 9);
+                const uint32_t ex = __shfl(e.x, from), ey = __shfl(e.y, from);
+                const float ox = __shfl(tv.r.ox, from), oy = __shfl(tv.r.oy, from);
+                const float oz = __shfl(tv.r.oz, from), tmin = __shfl(tv.r.tmin, from);
+                const float ix = __shfl(tv.ix, from), iy = __shfl(tv.iy, from);
+                const float iz = __shfl(tv.iz, from), best = __shfl(tv.h.t, from);
+                const float sx = __shfl(tv.sh.sx, from), sy = __shfl(tv.sh.sy, from);
+                const float sz = __shfl(tv.sh.sz, from);
+                const uint32_t k = __shfl(kxyz, from);
+                const uint32_t it = __shfl(item, from);
+                if (takes) {
+                    tv.r.ox = ox;
+                    tv.r.oy = oy;
+                    tv.r.oz = oz;
+                    tv.r.tmin = tmin;
+                    tv.ix = ix;
+                    tv.iy = iy;
+                    tv.iz = iz;
+                    tv.h.t = best;
+                    tv.sh.sx = sx;
+                    tv.sh.sy = sy;
+                    tv.sh.sz = sz;
+                    tv.sh.kx = (int)(k & 3u);
+                    tv.sh.ky = (int)((k >> 2) & 3u);
+                    tv.sh.kz = (int)((k >> 4) & 3u);
+                    tv.oct = (k >> 6) & 7u;
+                    key_lane = k >> 9;
+                    item = it;
+                    sc.bind(item);
+                    helper = true;
+                    tv.active = true;
+                    st.sp = 0;
+                    tv.descend(ex, ey & 0xFFu, ey >> 8, st);
+                }
+            }
         }
         const bool was_active = tv.active;
         uint32_t tbase = 0, tmask = 0;
         if (was_active) tv.step<BLOCK>(sc.nodes(), st, &tbase, &tmask);
-        shared_triangles(tv, tbase, tmask, sc, ts, lane);
-        if (was_active && !tv.active) src.store(slot, tv.h);
-        if (exhausted && !__any(tv.active)) break;
+        const bool any_helper = exhausted && __ballot(helper) != 0ull;
+        shared_triangles(tv, tbase, tmask, sc, ts, lane, key_lane, helper, any_helper);
+        if (was_active && !tv.active) {
+            if (helper) helper = false;
+            else pending = true;
+        }
+        if (__ballot(pending)) {
+            bool waits = false;
+            if (__ballot(helper)) {
+                // helpers still traversing mark their rays' lanes
+                ++stamp;
+                if (helper) ts.busy[key_lane] = stamp;
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                waits = ts.busy[lane] == stamp;
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (pending && !waits) {
+                src.store(slot, tv.h);
+                pending = false;
+            }
+        }
+        if (exhausted && !__any(tv.active || pending)) break;
     }
 }
 
@@ -553,7 +671,8 @@ __device__ __forceinline__ void traverse_queue(const Bvh8Node* __restrict__ node
     __shared__ unsigned long long name##_key[BLOCK];                                 \
     __shared__ float2 name##_uv[BLOCK];                                              \
     __shared__ uint32_t name##_leaf[BLOCK];                                          \
+    __shared__ uint32_t name##_busy[BLOCK];                                          \
     const TriShare name{name##_key + (threadIdx.x & ~63u), name##_uv + (threadIdx.x & ~63u), \
-                        name##_leaf + (threadIdx.x & ~63u)}
+                        name##_leaf + (threadIdx.x & ~63u), name##_busy + (threadIdx.x & ~63u)}
 
 }  // namespace lmb
