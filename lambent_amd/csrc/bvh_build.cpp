@@ -2,11 +2,14 @@
 // driver-built BLAS of the reference (examples/cornell_box/src/main.rs:464-563;
 // the reference's ray queries traverse it at general.wesl:1052-1078).
 //
-// 1. Binned-SAH binary BVH (16 bins per axis, leaves of <= 3 triangles),
+// 1. Binned-SAH binary BVH (16 bins per axis) down to single triangles,
 //    subtrees above 64k primitives built on separate threads.
-// 2. Collapse to 8-wide: repeatedly open the child of largest surface area
-//    (weighted by 1 + the triangles of its leaf children) until the node has
-//    8 children or only leaves remain.
+// 2. Collapse to 8-wide by dynamic programming over the binary tree
+//    (the cost-optimal collapse of Ylitie, Karras and Laine 2017, "Efficient
+//    Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs"): every
+//    binary subtree is covered by at most i child slots, i = 1..8, at least
+//    surface-area cost, where a visited 8-wide node costs kNodeCost and a
+//    triangle of a hit leaf slot kTriCost.
 // 3. Slot assignment by direction octant (greedy maximum alignment of the
 //    child centroid offset with the slot's octant direction).
 // 4. 8-bit quantization of the child boxes against a per-node origin that is
@@ -15,6 +18,7 @@
 #include "bvh_build.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -44,7 +48,28 @@ struct BNode {
 };
 
 constexpr int kBins = 16;
+// Triangles of one leaf slot (the node format's limit).
 constexpr uint32_t kMaxLeaf = 3;
+// The binary tree is built down to leaves of this many triangles; the
+// collapse chooses the leaf slots (subtrees of at most kMaxLeaf triangles).
+#ifndef LMB_BVH_BINARY_LEAF
+#define LMB_BVH_BINARY_LEAF 1
+#endif
+constexpr uint32_t kBinaryLeaf = LMB_BVH_BINARY_LEAF;
+static_assert(kBinaryLeaf >= 1 && kBinaryLeaf <= kMaxLeaf, "binary leaves must fit a leaf slot");
+
+// Relative costs of the collapse: one visit of an 8-wide node (its eight
+// child-box tests and the choice of the next node) and one triangle test of a
+// hit leaf slot.  The traversal tests triangles in rounds shared by the 64
+// lanes of a wave, and a round runs whenever any lane has a triangle whether
+// or not it is full, so one more triangle costs far less than a node visit:
+// on the 1M-triangle soup at 1080p the traversal time is least for 0.15 to
+// 0.25 (0.2: 4% below the greedy area-based collapse; 1.0 is 2% above it).
+#ifndef LMB_BVH_TRI_COST
+#define LMB_BVH_TRI_COST 0.2
+#endif
+constexpr float kNodeCost = 1.0f;
+constexpr float kTriCost = (float)(LMB_BVH_TRI_COST);
 constexpr uint32_t kParallelThreshold = 1u << 16;
 
 inline float half_area(const float lo[3], const float hi[3]) {
@@ -93,7 +118,7 @@ class BinaryBuilder {
         BNode& n = nodes_[ni];
         std::memcpy(n.lo, b.lo, sizeof n.lo);
         std::memcpy(n.hi, b.hi, sizeof n.hi);
-        if (count <= kMaxLeaf) {
+        if (count <= kBinaryLeaf) {
             n.first = first;
             n.count = count;
             n.left = 0;
@@ -178,6 +203,102 @@ class BinaryBuilder {
     std::vector<PrimRef>& refs_;
     std::vector<BNode> nodes_;
     std::atomic<uint32_t> next_;
+};
+
+// Cost-optimal covering of every binary subtree by at most i slots of an
+// 8-wide node.  A slot is either a leaf (a subtree of at most kMaxLeaf
+// triangles, tested when the slot's box is hit) or an 8-wide node of its
+// own.  Children of a binary node have larger indices than the node, and a
+// subtree's primitives are contiguous, left child first.
+class CollapsePlan {
+  public:
+    CollapsePlan(const std::vector<BNode>& bn, uint32_t count)
+        : bn_(bn), cost_(count), split_(count), wide_left_(count), leaf_(count), prims_(count),
+          first_(count) {
+        for (int64_t n = (int64_t)count - 1; n >= 0; --n) {
+            const BNode& b = bn[n];
+            const float area = half_area(b.lo, b.hi);
+            if (b.count > 0) {
+                prims_[n] = b.count;
+                first_[n] = b.first;
+                leaf_[n] = 1;
+                cost_[n].fill(area * (float)b.count * kTriCost);
+                split_[n].fill(0);
+                wide_left_[n] = 0;
+                continue;
+            }
+            const uint32_t l = b.left, r = b.left + 1;
+            prims_[n] = prims_[l] + prims_[r];
+            first_[n] = first_[l];
+            float best = std::numeric_limits<float>::infinity();
+            uint8_t best_k = 1;
+            for (int k = 1; k < 8; ++k) {
+                const float c = cost_[l][k - 1] + cost_[r][8 - k - 1];
+                if (c < best) {
+                    best = c;
+                    best_k = (uint8_t)k;
+                }
+            }
+            wide_left_[n] = best_k;
+            float single = area * kNodeCost + best;
+            leaf_[n] = 0;
+            if (prims_[n] <= kMaxLeaf) {
+                const float as_leaf = area * (float)prims_[n] * kTriCost;
+                if (as_leaf <= single) {
+                    single = as_leaf;
+                    leaf_[n] = 1;
+                }
+            }
+            cost_[n][0] = single;
+            split_[n][0] = 0;
+            for (int i = 2; i <= 8; ++i) {
+                float c = single;
+                uint8_t sk = 0;
+                for (int k = 1; k < i; ++k) {
+                    const float d = cost_[l][k - 1] + cost_[r][i - k - 1];
+                    if (d < c) {
+                        c = d;
+                        sk = (uint8_t)k;
+                    }
+                }
+                cost_[n][i - 1] = c;
+                split_[n][i - 1] = sk;
+            }
+        }
+    }
+
+    // The child slots (binary node indices) of the 8-wide node made from the
+    // internal binary node n.
+    int children(uint32_t n, uint32_t out[8]) const {
+        int m = 0;
+        const uint32_t l = bn_[n].left;
+        expand(l, wide_left_[n], out, &m);
+        expand(l + 1, 8 - wide_left_[n], out, &m);
+        return m;
+    }
+    // Slot n is a leaf of prims(n) primitives starting at first(n).
+    bool leaf(uint32_t n) const { return leaf_[n] != 0; }
+    uint32_t prims(uint32_t n) const { return prims_[n]; }
+    uint32_t first(uint32_t n) const { return first_[n]; }
+
+  private:
+    void expand(uint32_t n, int i, uint32_t out[8], int* m) const {
+        const uint8_t k = split_[n][i - 1];
+        if (k == 0) {
+            out[(*m)++] = n;
+            return;
+        }
+        expand(bn_[n].left, k, out, m);
+        expand(bn_[n].left + 1, i - k, out, m);
+    }
+
+    const std::vector<BNode>& bn_;
+    std::vector<std::array<float, 8>> cost_;    // [n][i - 1]: least cost with at most i slots
+    std::vector<std::array<uint8_t, 8>> split_;  // [n][i - 1]: 0 = one slot, else slots of the left child
+    std::vector<uint8_t> wide_left_;             // slots of the left child when n is an 8-wide node
+    std::vector<uint8_t> leaf_;                  // n as one slot is a leaf
+    std::vector<uint32_t> prims_;                // primitives of the subtree
+    std::vector<uint32_t> first_;                // its first primitive
 };
 
 inline bool finite3(const float* v) {
@@ -280,6 +401,7 @@ int build_bvh8(const float* vertices, uint64_t num_vertices, const uint32_t* ind
     BinaryBuilder bb(refs);
     bb.build();
     std::vector<BNode>& bn = bb.nodes();
+    const CollapsePlan plan(bn, bb.node_count());
 
     // Collapse into 8-wide nodes, breadth first so that the internal children
     // of every node are contiguous.
@@ -305,31 +427,7 @@ int build_bvh8(const float* vertices, uint64_t num_vertices, const uint32_t* ind
         if (b.count > 0) {
             ch[nch++] = pd.bnode;  // only a leaf root gets here
         } else {
-            ch[nch++] = b.left;
-            ch[nch++] = b.left + 1;
-            while (nch < 8) {
-                int pick = -1;
-                float best = -1.f;
-                for (int i = 0; i < nch; ++i) {
-                    const BNode& c = bn[ch[i]];
-                    if (c.count > 0) continue;
-                    // Opening a node whose children are leaves lifts their
-                    // triangles into this node (no separate visit): weight its
-                    // area by 1 + the triangles of its leaf children.  Fuller
-                    // nodes (about a third fewer on triangle soups) and fewer
-                    // visits per ray than opening by area alone.
-                    const float a = half_area(c.lo, c.hi) *
-                                    (float)(1u + bn[c.left].count + bn[c.left + 1].count);
-                    if (a > best) {
-                        best = a;
-                        pick = i;
-                    }
-                }
-                if (pick < 0) break;
-                uint32_t l = bn[ch[pick]].left;
-                ch[pick] = l;
-                ch[nch++] = l + 1;
-            }
+            nch = plan.children(pd.bnode, ch);
         }
 
         // Octant slot assignment.
@@ -401,15 +499,16 @@ int build_bvh8(const float* vertices, uint64_t num_vertices, const uint32_t* ind
                 node.qlo[a][s] = (uint8_t)ql;
                 node.qhi[a][s] = (uint8_t)qh;
             }
-            if (c.count == 0) {
+            if (!plan.leaf(ch[ci])) {
                 node.meta[s] = kMetaInternal;
                 node.imask |= (uint8_t)(1u << s);
                 n_internal++;
             } else {
-                uint32_t unary = (1u << c.count) - 1u;
+                const uint32_t count = plan.prims(ch[ci]), first = plan.first(ch[ci]);
+                uint32_t unary = (1u << count) - 1u;
                 node.meta[s] = (uint8_t)((unary << 5) | tri_off);
-                for (uint32_t k = 0; k < c.count; ++k) {
-                    const PrimRef& r = refs[c.first + k];
+                for (uint32_t k = 0; k < count; ++k) {
+                    const PrimRef& r = refs[first + k];
                     uint64_t t = r.id;
                     uint64_t i0 = indices ? indices[3 * t + 0] : 3 * t + 0;
                     uint64_t i1 = indices ? indices[3 * t + 1] : 3 * t + 1;
@@ -426,7 +525,7 @@ int build_bvh8(const float* vertices, uint64_t num_vertices, const uint32_t* ind
                     lt.v2[3] = 0.f;
                     out->tris.push_back(lt);
                 }
-                tri_off += c.count;
+                tri_off += count;
             }
         }
         if (tri_off > 24) {

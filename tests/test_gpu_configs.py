@@ -230,7 +230,7 @@ def test_cfg4_full_batch_bit_exact_against_per_image_traces():
 def test_deep_bvh_frame_same_seeds_as_oracle():
     # triangles at geometric scales: a BVH of more node levels than the 8
     # traversal stack entries in LDS, so deep paths use the global spill
-    n, ratio = 1000, 1.018
+    n, ratio = 1500, 1.018
     x = ratio ** -np.arange(n, dtype=np.float64)
     c = np.stack([x, 0.3 * x, -0.2 * x], axis=1)
     s = (0.05 * x)[:, None]

@@ -47,7 +47,7 @@ class Hip:
 HIP = Hip()
 
 
-def deep_cluster_scene(n=1000, ratio=1.018, soup=20000, seed=3):
+def deep_cluster_scene(n=1500, ratio=1.018, soup=20000, seed=3):
     """Triangles at geometric scales ratio^-k along one direction (a BVH of
     more node levels than the LDS stack holds) plus a random soup apart from
     them."""
