@@ -6,12 +6,16 @@ Workload: procedural scene of 1,000,000 triangles (centres uniform in
 from bounce 3, Medium quality, guiding off.  One step = one lmb_trace of one
 frame (seed_offset = step index) with the scene resident in HBM.
 
-value = path vertices (closest-hit queries) of the K timed frames / wall time.
+value = path vertices (closest-hit queries) of the K timed frames / wall time;
+the timed frames record no per-kernel events.
 roofline: the traversal ("extend") kernel, algorithmic bytes per path vertex
 184 + 80 * L = 664 (48 B ray in, 48 B ray / hit out, one 36 B triangle, 4 B
 material index, 32 B material, 16 B accumulate, one 80 B node per BVH level,
-L = ceil(log8(N/4)) = 6) over its HIP-event launch durations; the traversal
-alone moves 84 + 80 * L = 564 of them (traversal_only_frac).
+L = ceil(log8(N/4)) = 6) over its launch durations, measured with HIP events
+on the tracer's stream in a second pass over the same K frames (same seeds,
+so the same paths and vertex count; events_pass_ms_per_step is that pass's
+frame time); the traversal alone moves 84 + 80 * L = 564 of them
+(traversal_only_frac).
 cpu_baseline: the CPU oracle (C++/OpenMP restatement of the reference
 shaders, "port") on a bounded band of rows of the same frame.
 
@@ -542,17 +546,29 @@ def main() -> int:
     api.sync()
     log(f"warmup done ({args.warmup} frames)")
 
-    bufs.set_profiling(True)
-    bufs.kernel_stats()  # reset
-    api.sync()
+    # The timed frames, with no per-kernel events: a timed HIP event is a
+    # marker on the stream (about 2 us per record, 20 per frame).
     t0 = time.perf_counter()
     for k in range(args.steps):
         pipe.dispatch(scene, cam, bufs, k)
     api.sync()
     elapsed = time.perf_counter() - t0
+    # The same frames (same seeds, so the same paths and vertex count) once
+    # more with HIP events around every traversal and shading launch on the
+    # tracer's stream: the kernels' durations for the roofline.
+    bufs.set_profiling(True)
+    bufs.kernel_stats()  # reset
+    api.sync()
+    t1 = time.perf_counter()
+    for k in range(args.steps):
+        pipe.dispatch(scene, cam, bufs, k)
+    api.sync()
+    elapsed_events = time.perf_counter() - t1
     ks = bufs.kernel_stats()
+    bufs.set_profiling(False)
     verts = ks["path_vertices"]
-    log(f"timed {args.steps} frames in {elapsed:.3f} s, {verts} path vertices")
+    log(f"timed {args.steps} frames in {elapsed:.3f} s ({elapsed_events:.3f} s with per-kernel "
+        f"events), {verts} path vertices")
 
     value = verts / elapsed
     L = math.ceil(math.log(max(args.triangles, 4) / 4, 8))
@@ -598,6 +614,7 @@ def main() -> int:
             "extend_ms_per_step": ks["extend_ms"] / args.steps,
             "shade_ms_per_step": ks["shade_ms"] / args.steps,
             "extend_launches": ks["extend_launches"],
+            "events_pass_ms_per_step": elapsed_events / args.steps * 1e3,
         },
         "cpu_baseline": None,
     }

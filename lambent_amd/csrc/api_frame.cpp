@@ -36,7 +36,11 @@ class EventTimer final : public KernelTimer {
     void begin(int kind, hipStream_t stream) override {
         if (used_ == pool_.size()) {
             Slot s{};
-            if (hipEventCreate(&s.start) != hipSuccess || hipEventCreate(&s.stop) != hipSuccess) {
+            // Timing only: no system-scope fence (cache writeback and
+            // invalidation) when an event completes, which would also slow
+            // the launches after it.
+            if (hipEventCreateWithFlags(&s.start, hipEventDisableSystemFence) != hipSuccess ||
+                hipEventCreateWithFlags(&s.stop, hipEventDisableSystemFence) != hipSuccess) {
                 failed_ = true;
                 return;
             }
