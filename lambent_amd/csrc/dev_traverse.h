@@ -379,68 +379,68 @@ __device__ __forceinline__ uint32_t nth_set_bit64(uint64_t m, uint32_t j) {
     return b;
 }
 
-// Shared triangle phase, called by all 64 lanes of a wave.  Lane l owns the
-// triangles tbase_l + bit of tmask_l; the wave's (owner, triangle) pairs are
-// spread over all lanes, 64 per round, each tested against its owner's ray
-// (fetched by lane shuffles).  Per ray the closest hit is the minimum of
+// Shared triangle phase, called by all 64 lanes of a wave after each node
+// step.  Lane l holds two runs of untested triangles: (pb_l, pm_l), carried
+// from the pass before, and (nb_l, nm_l) from this pass's step (base + bit
+// of mask).  The wave's (lane, triangle) pairs - every carried run first,
+// then every new run - are spread over all lanes, 64 per round, each tested
+// against its lane's ray (fetched by lane shuffles).  Rounds run only when
+// full, except that carried pairs are always tested: a pass with carried
+// pairs tests everything if fewer than 64 are pending, and otherwise the full
+// rounds; the untested rest (fewer than 64, from this pass's runs) is carried
+// to the next pass.  A lane therefore waits at most one pass for its
+// triangles, and a lane whose traversal has ended keeps its ray until its
+// carried pairs are tested.  Per ray the closest hit is the minimum of
 // (t, primitive id) - the order the one-lane loop applies (nearer t first,
-// then the lower primitive id) - taken with a 64-bit LDS atomicMin on the
-// hit record of lane key_lane_l (the lane itself, or for a helper the lane
-// whose ray it traverses); the lane holding the winning triangle then writes
-// its (u, v) and leaf index there.  any_helper: some lane of the wave is a
-// helper (wave-uniform).
+// then the lower primitive id) - taken with a 64-bit LDS atomicMin on the hit
+// record of lane key_lane_l (the lane itself, or for a helper the lane whose
+// ray it traverses); the lane holding the winning triangle then writes its
+// (u, v) and leaf index there.  Testing later only delays the pruning
+// distance of a ray, never the result.
 template <class Scenes>
-__device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t tbase, uint32_t tmask,
-                                                 const Scenes& sc, const TriShare& ts,
-                                                 uint32_t lane, uint32_t key_lane, bool helper,
-                                                 bool any_helper) {
-    const uint32_t c = __popc(tmask);
-    if (!__ballot(c != 0u)) return;
-    if (!any_helper && !__ballot(c > 1u)) {
-        // at most one triangle per lane: each lane tests its own in place
-        if (c) {
-            const uint32_t leaf = tbase + (uint32_t)__builtin_ctz(tmask);
-            const float4* tp = reinterpret_cast<const float4*>(sc.tris() + leaf);
-            const float4 a = tp[0], b = tp[1], cc = tp[2];
-            float t, u, v;
-            if (intersect_triangle(tv.sh, tv.r.ox, tv.r.oy, tv.r.oz, a, b, cc, tv.r.tmin, tv.h.t,
-                                   &t, &u, &v)) {
-                const uint32_t prim = __float_as_uint(a.w);
-                bool take = t < tv.h.t || prim < tv.h.prim;
-                if constexpr (Scenes::kAlphaTest) take = take && sc.accept(b, u, v);
-                if (take) {
-                    tv.h.t = t;
-                    tv.h.u = u;
-                    tv.h.v = v;
-                    tv.h.prim = prim;
-                    tv.h.leaf = leaf;
-                }
-            }
-        }
-        return;
-    }
+__device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t& pb, uint32_t& pm,
+                                                 uint32_t nb, uint32_t nm, const Scenes& sc,
+                                                 const TriShare& ts, uint32_t lane,
+                                                 uint32_t key_lane, bool helper) {
+    // counts and offsets of both runs in one word (each total is below 2^16:
+    // at most 64 x 24 pairs per pass)
+    const uint32_t c = (uint32_t)__popc(pm) | ((uint32_t)__popc(nm) << 16);
     uint32_t incl = c;
 #pragma unroll
     for (uint32_t d = 1; d < 64; d <<= 1) {
         const uint32_t y = __shfl_up(incl, d);
         if (lane >= d) incl += y;
     }
-    const uint32_t off = incl - c;
-    const uint32_t total = __shfl(incl, 63);
+    const uint32_t totals = __shfl(incl, 63);
+    const uint32_t t_old = totals & 0xFFFFu, total = t_old + (totals >> 16);
+    const uint32_t n_test = (t_old != 0u && total < 64u) ? total : (total & ~63u);
+    const uint32_t excl = incl - c;
+    const uint32_t off_new = t_old + (excl >> 16);
+    if (n_test == 0u) {  // nothing carried and fewer than 64 new pairs: carry them
+        pb = nb;
+        pm = nm;
+        return;
+    }
+    const uint32_t offs = (excl & 0xFFFFu) | (off_new << 16);
     const unsigned long long init = ((unsigned long long)orderable(tv.h.t) << 32) | tv.h.prim;
     ts.key[lane] = init;
-    const uint32_t kxyz = (uint32_t)tv.sh.kx | ((uint32_t)tv.sh.ky << 2) | ((uint32_t)tv.sh.kz << 4);
-    for (uint32_t r0 = 0; r0 < total; r0 += 64) {
+    const uint32_t kxyz = (uint32_t)tv.sh.kx | ((uint32_t)tv.sh.ky << 2) | ((uint32_t)tv.sh.kz << 4) |
+                          (key_lane << 6);
+    for (uint32_t r0 = 0; r0 < n_test; r0 += 64) {
         const uint32_t i = r0 + lane;
+        const bool carried = i < t_old;
+        const uint32_t half = carried ? 0u : 16u;
         uint32_t owner = 0;
 #pragma unroll
         for (uint32_t stp = 32; stp; stp >>= 1) {
             const uint32_t m = owner + stp;
-            if (__shfl(off, (int)m) <= i) owner = m;
+            if (((__shfl(offs, (int)m) >> half) & 0xFFFFu) <= i) owner = m;
         }
-        const uint32_t o_mask = __shfl(tmask, (int)owner);
-        const uint32_t o_base = __shfl(tbase, (int)owner);
-        const uint32_t o_off = __shfl(off, (int)owner);
+        const uint32_t o_pm = __shfl(pm, (int)owner), o_nm = __shfl(nm, (int)owner);
+        const uint32_t o_pb = __shfl(pb, (int)owner), o_nb = __shfl(nb, (int)owner);
+        const uint32_t o_mask = carried ? o_pm : o_nm;
+        const uint32_t o_base = carried ? o_pb : o_nb;
+        const uint32_t o_off = (__shfl(offs, (int)owner) >> half) & 0xFFFFu;
         const float ox = __shfl(tv.r.ox, (int)owner);
         const float oy = __shfl(tv.r.oy, (int)owner);
         const float oz = __shfl(tv.r.oz, (int)owner);
@@ -450,7 +450,7 @@ __device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t tbase, 
         sh.sy = __shfl(tv.sh.sy, (int)owner);
         sh.sz = __shfl(tv.sh.sz, (int)owner);
         const uint32_t ok = __shfl(kxyz, (int)owner);
-        const uint32_t o_key = __shfl(key_lane, (int)owner);
+        const uint32_t o_key = ok >> 6;
         const LeafTri* __restrict__ tris = sc.tris_of(owner);
         sh.kx = (int)(ok & 3u);
         sh.ky = (int)((ok >> 2) & 3u);
@@ -459,7 +459,7 @@ __device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t tbase, 
         unsigned long long mykey = 0;
         float t = 0.f, u = 0.f, v = 0.f;
         uint32_t leaf = 0;
-        if (i < total) {
+        if (i < n_test) {
             leaf = o_base + nth_set_bit(o_mask, i - o_off);
             const float4* tp = reinterpret_cast<const float4*>(tris + leaf);
             const float4 a = tp[0], b = tp[1], cc = tp[2];
@@ -480,6 +480,18 @@ __device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t tbase, 
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
+    }
+    // Every carried pair has been tested; of this pass's run, the pairs from
+    // index n_test on are carried (the run's highest set bits).
+    const uint32_t cn = (uint32_t)__popc(nm);
+    pb = nb;
+    if (n_test >= off_new + cn) {
+        pm = 0u;
+    } else if (n_test <= off_new) {
+        pm = nm;
+    } else {
+        const uint32_t first_left = nth_set_bit(nm, n_test - off_new);
+        pm = nm & ~((1u << first_left) - 1u);
     }
     const unsigned long long k = ts.key[lane];
     if (k != init) {
@@ -507,8 +519,8 @@ __device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t tbase, 
 // traverses those subtrees for the same ray, as a helper.  A helper's
 // triangles update its ray's hit record (the LDS key of the owning lane), it
 // prunes with that record's distance, and it can itself donate.  A ray's
-// result is stored once its own lane has finished and no helper of it is
-// still traversing.  The closest hit (and the tie rule) does not depend on
+// result is stored once its own lane has finished, its carried triangles are
+// tested and no helper of it is still at work.  The closest hit (and the tie rule) does not depend on
 // which lane visits which subtree, so the result is unchanged; the wave's
 // tail - lanes waiting for the longest ray - gets shorter.
 // Source must provide: Ray load(uint64_t slot, uint32_t* scene),
@@ -531,14 +543,15 @@ __device__ __forceinline__ void traverse_queue_t(Scenes& sc, uint64_t count, uin
     uint64_t slot = 0;
     uint32_t item = 0;
     uint32_t key_lane = lane;  // lane holding the hit record of this lane's ray
-    bool helper = false;       // traversing subtrees of key_lane's ray
-    bool pending = false;      // own traversal done, helpers of this ray still running
+    bool own = false;          // holds a ray whose hit is not yet stored
+    bool helper = false;       // works on subtrees of key_lane's ray
+    uint32_t pb = 0, pm = 0;   // triangles carried to the next pass (base, mask)
     bool exhausted = false;    // wave-uniform
     uint32_t stamp = 0;        // wave-uniform
     ts.busy[lane] = ~0u;
     while (true) {
         if (!exhausted) {
-            const uint64_t idle = __ballot(!tv.active);
+            const uint64_t idle = __ballot(!own);
             const uint32_t n_idle = (uint32_t)__popcll(idle);
             if (n_idle >= min(refill, quota)) {
                 const uint32_t take = min(n_idle, quota);
@@ -553,7 +566,7 @@ __device__ __forceinline__ void traverse_queue_t(Scenes& sc, uint64_t count, uin
                     if (++drained == kFetchSegments) exhausted = true;
                 }
                 const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                if (!tv.active && rank < take) {
+                if (!own && rank < take) {
                     const uint64_t mine = lo + base + rank;
                     if (mine < hi) {
                         slot = mine;
@@ -561,12 +574,13 @@ __device__ __forceinline__ void traverse_queue_t(Scenes& sc, uint64_t count, uin
                         sc.bind(item);
                         tv.init(r, sc.num_nodes());
                         st.sp = 0;
+                        own = tv.active;
                         if (!tv.active) src.store(slot, tv.h);
                     }
                 }
             }
         } else {
-            const uint64_t idle = __ballot(!tv.active && !pending);
+            const uint64_t idle = __ballot(!own && !helper);
             const uint64_t donors = __ballot(tv.active && st.sp > 0);
             if (idle && donors) {
                 const uint32_t n = (uint32_t)min(__popcll(idle), __popcll(donors));
@@ -613,19 +627,16 @@ __device__ __forceinline__ void traverse_queue_t(Scenes& sc, uint64_t count, uin
                 }
             }
         }
-        const bool was_active = tv.active;
         uint32_t tbase = 0, tmask = 0;
-        if (was_active) tv.step<BLOCK>(sc.nodes(), st, &tbase, &tmask);
-        const bool any_helper = exhausted && __ballot(helper) != 0ull;
-        shared_triangles(tv, tbase, tmask, sc, ts, lane, key_lane, helper, any_helper);
-        if (was_active && !tv.active) {
-            if (helper) helper = false;
-            else pending = true;
-        }
-        if (__ballot(pending)) {
+        if (tv.active) tv.step<BLOCK>(sc.nodes(), st, &tbase, &tmask);
+        shared_triangles(tv, pb, pm, tbase, tmask, sc, ts, lane, key_lane, helper);
+        const bool done = !tv.active && pm == 0u;  // traversal ended, triangles tested
+        if (helper && done) helper = false;
+        const bool ready = own && done;
+        if (__ballot(ready)) {
             bool waits = false;
             if (__ballot(helper)) {
-                // helpers still traversing mark their rays' lanes
+                // helpers still at work mark their rays' lanes
                 ++stamp;
                 if (helper) ts.busy[key_lane] = stamp;
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -634,12 +645,12 @@ __device__ __forceinline__ void traverse_queue_t(Scenes& sc, uint64_t count, uin
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 __builtin_amdgcn_wave_barrier();
             }
-            if (pending && !waits) {
+            if (ready && !waits) {
                 src.store(slot, tv.h);
-                pending = false;
+                own = false;
             }
         }
-        if (exhausted && !__any(tv.active || pending)) break;
+        if (exhausted && !__any(own || helper)) break;
     }
 }
 
