@@ -346,6 +346,20 @@ __device__ __forceinline__ float from_orderable(uint32_t o) {
     return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave: four DPP row shifts
+// (16-lane rows; lanes shifted in from outside the row read 0), then lane 15
+// of each row added to the next row and lane 31 to the upper half (the GFX9
+// row broadcasts).  Each step is one VALU instruction with no LDS round trip.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);   // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);   // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);   // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);   // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return x;
+}
+
 // Position of the j-th set bit (j from 0) of a 24-bit mask.
 __device__ __forceinline__ uint32_t nth_set_bit(uint32_t m, uint32_t j) {
     uint32_t b = 0, c;
@@ -405,13 +419,8 @@ __device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t& pb, ui
     // counts and offsets of both runs in one word (each total is below 2^16:
     // at most 64 x 24 pairs per pass)
     const uint32_t c = (uint32_t)__popc(pm) | ((uint32_t)__popc(nm) << 16);
-    uint32_t incl = c;
-#pragma unroll
-    for (uint32_t d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(incl, d);
-        if (lane >= d) incl += y;
-    }
-    const uint32_t totals = __shfl(incl, 63);
+    const uint32_t incl = wave_inclusive_sum(c);
+    const uint32_t totals = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     const uint32_t t_old = totals & 0xFFFFu, total = t_old + (totals >> 16);
     const uint32_t n_test = (t_old != 0u && total < 64u) ? total : (total & ~63u);
     const uint32_t excl = incl - c;
