@@ -60,13 +60,13 @@ static_assert(kBinaryLeaf >= 1 && kBinaryLeaf <= kMaxLeaf, "binary leaves must f
 
 // Relative costs of the collapse: one visit of an 8-wide node (its eight
 // child-box tests and the choice of the next node) and one triangle test of a
-// hit leaf slot.  The traversal tests triangles in rounds shared by the 64
-// lanes of a wave, and a round runs whenever any lane has a triangle whether
-// or not it is full, so one more triangle costs far less than a node visit:
-// on the 1M-triangle soup at 1080p the traversal time is least for 0.15 to
-// 0.25 (0.2: 4% below the greedy area-based collapse; 1.0 is 2% above it).
+// hit leaf slot.  The traversal tests triangles in full rounds shared by the
+// 64 lanes of a wave, so one more triangle costs far less than a node visit:
+// on the 1M-triangle soup at 1080p the traversal takes 0.418, 0.415, 0.413,
+// 0.413, 0.415, 0.418 and 0.423 ms per launch at 0.2, 0.25, 0.3, 0.35, 0.4,
+// 0.5 and 0.75.
 #ifndef LMB_BVH_TRI_COST
-#define LMB_BVH_TRI_COST 0.2
+#define LMB_BVH_TRI_COST 0.3
 #endif
 constexpr float kNodeCost = 1.0f;
 constexpr float kTriCost = (float)(LMB_BVH_TRI_COST);
