@@ -129,3 +129,66 @@ def test_bvh_stats_sane():
     assert st.num_triangles == 100_000
     assert 1 < st.max_depth < 20
     assert st.num_nodes > 100_000 / 24
+
+
+@pytest.fixture(scope="module")
+def soup_100k():
+    spec = scenes.procedural(100_000, seed=5, width=16, height=16)
+    return spec, spec.build(), oracle.Bvh(spec.vertices, spec.indices)
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 65, 700, 5000])
+def test_short_queues_where_idle_lanes_help(soup_100k, n):
+    """A queue shorter than the grid's waves gives each wave one ray or a
+    few, so nearly every lane of a wave is idle from the start: idle lanes
+    traverse the deferred subtrees of the rays still running, their
+    triangles go through the carried triangle rounds, and each ray's hit is
+    stored once all of its helpers are done.  The hits must be the same
+    closest hits as the oracle's."""
+    spec, scene, bvh = soup_100k
+    rng = np.random.default_rng(100 + n)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = rng.uniform(-14, 14, size=(n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 3] = 1e-4
+    rays[:, 4:7] = d
+    rays[:, 7] = 1000.0
+    _check_hits(scene.intersect(rays), bvh.intersect(rays))
+
+
+@pytest.mark.parametrize("n", [1, 33, 1000])
+def test_short_queues_on_a_deep_bvh(n):
+    """The same with rays along a chain of nested triangles whose BVH is
+    deeper than the 8-entry LDS stack: the donated stack entries come from
+    the spill area as well."""
+    k = np.arange(1500, dtype=np.float64)
+    x = 1.018 ** -k
+    c = np.stack([x, 0.3 * x, -0.2 * x], axis=1)
+    s = (0.05 * x)[:, None]
+    tri = np.stack([c, c + s * np.array([1.0, 0.0, 0.0]), c + s * np.array([0.0, 1.0, 1.0])], axis=1)
+    v = np.zeros((len(tri) * 3, 4), np.float32)
+    v[:, :3] = tri.reshape(-1, 3).astype(np.float32)
+    scene = api.Scene(v, None, np.zeros(len(tri), np.uint32), api.materials_array([
+        api.Material.new([0.5, 0.5], [0.5, 0.5], [0.5, 0.5], [0, 0], 0, None, None,
+                         api.EmissionScale.from_emission_scale(0.0), None, api.MaterialType.Diffuse)]),
+        api.TextureSet([api.solid_rgba8(200, 200, 200, 255)], [], [], None, None))
+    assert scene.bvh_stats().max_depth > 12
+    rng = np.random.default_rng(7 + n)
+    # from beyond the chain's large end towards the centroids of its small,
+    # deeply nested triangles: every ray crosses the nested boxes level by
+    # level and hits at least its target
+    axis = np.array([1.0, 0.3, -0.2]) / np.linalg.norm([1.0, 0.3, -0.2])
+    target = tri[rng.integers(600, 1500, n)].mean(axis=1)
+    org = 2.5 * axis + rng.normal(scale=1e-3, size=(n, 3))
+    d = target - org
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = org
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 3], rays[:, 7] = 1e-4, 1000.0
+    got = scene.intersect(rays)
+    ref = oracle.intersect_brute(v, None, rays)
+    assert np.array_equal(got["prim"], ref["prim"])
+    hit = ref["prim"] != api.MISS
+    assert hit.any()
+    assert np.array_equal(got["t"][hit].view(np.uint32), ref["t"][hit].view(np.uint32))
