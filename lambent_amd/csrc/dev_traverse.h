@@ -529,9 +529,10 @@ __device__ __forceinline__ void shared_triangles(Traversal& tv, uint32_t& pb, ui
 // triangles update its ray's hit record (the LDS key of the owning lane), it
 // prunes with that record's distance, and it can itself donate.  A ray's
 // result is stored once its own lane has finished, its carried triangles are
-// tested and no helper of it is still at work.  The closest hit (and the tie rule) does not depend on
-// which lane visits which subtree, so the result is unchanged; the wave's
-// tail - lanes waiting for the longest ray - gets shorter.
+// tested and no helper of it is still at work.  The closest hit (and the tie
+// rule) does not depend on which lane visits which subtree, so the result is
+// unchanged; the wave's tail - lanes waiting for the longest ray - gets
+// shorter.
 // Source must provide: Ray load(uint64_t slot, uint32_t* scene),
 // void store(uint64_t slot, Hit).
 template <int BLOCK, class Scenes, class Source>
