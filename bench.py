@@ -32,7 +32,8 @@ bounces) on the same scene: GPU time for each to reach the relMSE that the
 plain tracer has at 16, 64 and 256 spp, judged against a 4096-spp plain
 reference accumulated in fp64; plus the matrix-core rate of the cache's
 inference and training kernels against the bf16 dense peak (with the spread
-rule, and as configured: every path ends at its second vertex), and the CPU
+rule, and as configured: every path ends at its second vertex; with the first
+scatter stratified, and from the reference's hash draws), and the CPU
 oracle's time to the 256-spp relMSE (a band of rows, scaled to the frame).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--triangles N]
@@ -129,7 +130,7 @@ def cache_reference(api, scene, spec, cam, ref_spp: int):
 def cache_leg(api, scene, spec, cam, reference, frames: int, warm: int,
               lr: float = 1e-3, spp: int = 8, steps: int = 32, overlap: bool = True,
               warm_paths: int = 524288, warm_steps: int = 256, ema: float = 0.0,
-              defer_specular: int = 1) -> dict:
+              defer_specular: int = 2, stratify: bool = True) -> dict:
     """Time-to-relMSE of the cached tracer against the plain tracer, both with
     8-bounce paths (RR from bounce 3), judged against the plain reference of
     cache_reference (`reference`).  The cache starts untrained: `warm`
@@ -142,14 +143,17 @@ def cache_leg(api, scene, spec, cam, reference, frames: int, warm: int,
     a time; otherwise the training paths come from the first sample).  Times
     are wall times of the dispatches (host read-back of the images for
     scoring is not counted).  defer_specular: how many specular scatters may
-    move a path's cache vertex one vertex on (0: always the second vertex)."""
+    move a path's cache vertex one vertex on (0: always the second vertex).
+    stratify: the first scatter draws from each pixel's scrambled Sobol
+    sequence (False: the reference's hash draws, as the plain tracer's)."""
     import numpy as np
     W, H = spec.width, spec.height
     popts = dict(bounces=8, rr_start=3, guiding=api.Guiding.OFF,
                  restir_temporal=api.RestirTemporalMode.Disabled)
     ref, plain_pts, ref_text = reference
     frame_opts = api.NeuralCacheOpts(learning_rate=lr, train_steps=steps, overlap_training=overlap,
-                                     ema_decay=ema, defer_specular=defer_specular)
+                                     ema_decay=ema, defer_specular=defer_specular,
+                                     stratify=stratify)
     cache = api.NeuralCache(api.NeuralCacheOpts(**{**frame_opts.__dict__, "train_paths": warm_paths,
                                                    "train_steps": warm_steps}))
     warm_pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=1, **popts))
@@ -177,7 +181,7 @@ def cache_leg(api, scene, spec, cam, reference, frames: int, warm: int,
         if n in checks:
             pts.append({"frames": n, "samples": n * spp, "seconds": t,
                         "relmse": rel_mse(acc / n, ref)})
-    log(f"cache leg (spread bound {defer_specular}): {warm} + {frames} frames in {t:.2f} s, "
+    log(f"cache leg (spread bound {defer_specular}, stratify {stratify}): {warm} + {frames} frames in {t:.2f} s, "
         f"final relMSE {pts[-1]['relmse']:.5f}")
     # Matrix-core figures from a few more frames with per-launch events (kept
     # out of the timed run: the events add about a quarter to a frame).
@@ -209,7 +213,9 @@ def cache_leg(api, scene, spec, cam, reference, frames: int, warm: int,
                     f"and {steps} Adam steps (lr {lr:g}) per frame"
                     f"{' on a second stream beside the samples' if overlap else ''}; {warm} "
                     f"training frames of {warm_paths} paths and {warm_steps} steps (no image, "
-                    f"timed), then frames of {spp} spp",
+                    f"timed), then frames of {spp} spp; first scatter from "
+                    + ("each pixel's Owen-scrambled Sobol sequence" if stratify
+                       else "the reference's hash draws"),
         "reference": ref_text,
         "plain": plain_pts,
         "cache_points": pts,
@@ -647,6 +653,9 @@ def main() -> int:
         # the config as written: every path ends at its second vertex
         out["time_to_relmse_vertex2"] = cache_leg(api, scene, spec, cam, reference,
                                                   defer_specular=0, **leg)
+        # the cache alone: the first scatter from the reference's hash draws
+        out["time_to_relmse_hash_draws"] = cache_leg(api, scene, spec, cam, reference,
+                                                     stratify=False, **leg)
         if not args.no_cpu_baseline:
             try:
                 out["time_to_relmse"]["cpu_plain_256spp"] = cpu_time_to_relmse(spec, reference)

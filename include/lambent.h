@@ -489,7 +489,7 @@ typedef struct lmb_nrc_opts {
                               below 0.1, or any at LOW quality) does not end
                               at its second vertex but at the vertex after
                               its first rough scatter, at most n vertices
-                              later; 0: always the second vertex (<= 64) (1) */
+                              later; 0: always the second vertex (<= 64) (2) */
     uint32_t overlap_training; /* 1: lmb_trace_nrc runs its training (a
                               training pass as lmb_trace_nrc_train's, then the
                               Adam steps) on a stream of the cache, beside
@@ -500,6 +500,15 @@ typedef struct lmb_nrc_opts {
                               the samples of frame k use the weights trained
                               up to frame k - 1, and the call's work is
                               ordered on the caller's stream              (1) */
+    uint32_t stratify;     /* 1: the first scatter of a sample (at the camera
+                              ray's hit) draws its direction and Fresnel
+                              choice from the pixel's Owen-scrambled Sobol
+                              sequence, point seed_offset x samples + sample,
+                              so the samples of a frame, and of frames with
+                              consecutive seed offsets, are stratified over
+                              the lobe; every point is uniform, so each sample
+                              stays unbiased.  0: the reference's hash draws,
+                              as the plain tracer's                      (1) */
 } lmb_nrc_opts;
 
 void lmb_nrc_opts_default(lmb_nrc_opts *opts);

@@ -162,7 +162,8 @@ class _NrcOptsC(ctypes.Structure):
                 ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float),
                 ("train_paths", ctypes.c_uint32), ("train_steps", ctypes.c_uint32),
                 ("seed", ctypes.c_uint32), ("ema_decay", ctypes.c_float),
-                ("defer_specular", ctypes.c_uint32), ("overlap_training", ctypes.c_uint32)]
+                ("defer_specular", ctypes.c_uint32), ("overlap_training", ctypes.c_uint32),
+                ("stratify", ctypes.c_uint32)]
 
 
 class _NrcStatsC(ctypes.Structure):
@@ -1075,7 +1076,8 @@ class NeuralCacheOpts:
     cached frame (steps 1..256), weight-initialisation seed, EMA decay of the
     inference weights, the spread rule's bound (defer_specular) and whether
     the training runs on the cache's own stream beside the frame's samples
-    (overlap_training)."""
+    (overlap_training), and whether the first scatter of a sample draws from
+    the pixel's scrambled Sobol sequence (stratify)."""
     learning_rate: float = 1e-3
     beta1: float = 0.9
     beta2: float = 0.99
@@ -1084,8 +1086,9 @@ class NeuralCacheOpts:
     train_steps: int = 4
     seed: int = 1
     ema_decay: float = 0.0
-    defer_specular: int = 1
+    defer_specular: int = 2
     overlap_training: bool = True
+    stratify: bool = True
 
 
 def _nrc_opts_c(o: NeuralCacheOpts) -> _NrcOptsC:
@@ -1095,6 +1098,7 @@ def _nrc_opts_c(o: NeuralCacheOpts) -> _NrcOptsC:
     c.ema_decay = o.ema_decay
     c.defer_specular = int(o.defer_specular)
     c.overlap_training = 1 if o.overlap_training else 0
+    c.stratify = 1 if o.stratify else 0
     return c
 
 

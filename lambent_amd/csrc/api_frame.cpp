@@ -616,6 +616,8 @@ static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera
     np.train_expect = (float)nrc->opts.train_paths;  // all in the trace's first sample
     np.defer_specular = nrc->opts.defer_specular;
     np.train_only = train_only ? 1u : 0u;
+    np.stratify = !train_only && nrc->opts.stratify ? 1u : 0u;
+    np.seq_base = seed_offset * p.samples;  // consecutive offsets continue one sequence
     const hipStream_t hs = (hipStream_t)stream;
     KernelTimer* tm = bufs->timer.get();
     if (train_only || !nrc->opts.overlap_training || nrc->opts.train_paths == 0) {
@@ -711,6 +713,7 @@ static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera
     NrcTraceParams npt = np;
     npt.train_expect = (float)nrc->opts.train_paths;
     npt.train_only = 1u;
+    npt.stratify = 0u;
     e = launch_trace_nrc(scene->dev, cd, ft, pt, db, npt, ts, tm);
     if (e != hipSuccess) return fail_hip(e, "cached trace training pass");
     st = nrc_run_steps(nrc, db, nxt, db.rec_in, db.rec_target, db.ctl + kNrcCtlRecords, db.rec_cap,

@@ -296,8 +296,9 @@ void lmb_nrc_opts_default(lmb_nrc_opts* o) {
     o->train_steps = 4;
     o->seed = 1;
     o->ema_decay = 0.0f;
-    o->defer_specular = 1u;
+    o->defer_specular = 2u;
     o->overlap_training = 1u;
+    o->stratify = 1u;
 }
 
 static lmb_status validate(const lmb_nrc_opts& o) {
@@ -314,6 +315,7 @@ static lmb_status validate(const lmb_nrc_opts& o) {
         return fail(LMB_ERR_INVALID_ARGUMENT, "defer_specular must be at most 64");
     if (o.overlap_training > 1u)
         return fail(LMB_ERR_INVALID_ARGUMENT, "overlap_training must be 0 or 1");
+    if (o.stratify > 1u) return fail(LMB_ERR_INVALID_ARGUMENT, "stratify must be 0 or 1");
     return LMB_OK;
 }
 

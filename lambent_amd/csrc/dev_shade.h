@@ -252,13 +252,14 @@ __device__ __forceinline__ V3 from_onb_no_tangent(V3 src, V3 up) {
     V3 tangent = normalize3(cross3(up, other));
     return from_onb(src, up, tangent);
 }
-// general.wesl:1230-1244
-__device__ __forceinline__ V3 onb_cosine_weighted_hemisphere(uint32_t seed) {
-    float r1 = rand_f32(seed);
-    float r2 = rand_f32(seed + 1u);
+// general.wesl:1230-1244, from the two uniforms r1 (azimuth) and r2
+__device__ __forceinline__ V3 onb_cosine_weighted_hemisphere_u(float r1, float r2) {
     float phi = kTau * r1;
     float sr2 = sqrtf(r2);
     return v3(cosf(phi) * sr2, sqrtf(1.0f - r2), sinf(phi) * sr2);
+}
+__device__ __forceinline__ V3 onb_cosine_weighted_hemisphere(uint32_t seed) {
+    return onb_cosine_weighted_hemisphere_u(rand_f32(seed), rand_f32(seed + 1u));
 }
 __device__ __forceinline__ V3 cosine_weighted_hemisphere(uint32_t seed, V3 normal, V3 tangent) {
     return from_onb(onb_cosine_weighted_hemisphere(seed), normal, tangent);
@@ -270,16 +271,17 @@ __device__ __forceinline__ V3 sample_ggx_internal(float u1, float u2, V3 wi) {
     float sin_theta = sqrtf(fminf(fmaxf(fmaf(y, -y, 1.0f), 0.0f), 1.0f));
     return v3(sin_theta * cosf(phi), y, sin_theta * sinf(phi)) + wi;
 }
-// general.wesl:1246-1270
-__device__ __forceinline__ V3 sample_vndf_ggx(uint32_t seed, V3 view, float rx, float ry) {
-    float u1 = rand_f32(seed);
-    float u2 = rand_f32(seed + 1u);
+// general.wesl:1246-1270, from the two uniforms u1 (azimuth) and u2
+__device__ __forceinline__ V3 sample_vndf_ggx_u(float u1, float u2, V3 view, float rx, float ry) {
     V3 in_dir = v3(view.x * rx, view.y, view.z * ry);
     if (v3_near_zero(in_dir)) in_dir = v3(0.0f, -1.0f, 0.0f);
     V3 wm_std = sample_ggx_internal(u1, u2, normalize3(in_dir));
     V3 out = v3(wm_std.x * rx, wm_std.y, wm_std.z * ry);
     if (v3_near_zero(out)) out = v3(0.0f, 1.0f, 0.0f);
     return normalize3(out);
+}
+__device__ __forceinline__ V3 sample_vndf_ggx(uint32_t seed, V3 view, float rx, float ry) {
+    return sample_vndf_ggx_u(rand_f32(seed), rand_f32(seed + 1u), view, rx, ry);
 }
 // general.wesl:1206-1210 Schlick
 __device__ __forceinline__ float reflectance(float cosine, float ref_idx) {
@@ -305,16 +307,26 @@ __device__ __forceinline__ V3 oren_nayar_weight(V3 incoming, V3 reflected, float
 }
 
 // --------------------------------------------------------- quality hooks
-// low.wesl / medium.wesl / high.wesl get_normal
+// low.wesl / medium.wesl / high.wesl get_normal, from the uniforms u1, u2
+// (the reference draws them at seed and seed + 1)
+template <uint32_t Q>
+__device__ __forceinline__ V3 get_normal_u(V3 ray_dir, V3 normal, V3 tangent, float u1, float u2,
+                                           float rx, float ry) {
+    if constexpr (Q == kQualityLow) {
+        return normal;
+    } else {
+        V3 ray_onb = to_onb(ray_dir, normal, tangent);
+        V3 n_onb = sample_vndf_ggx_u(u1, u2, ray_onb, rx, ry);
+        return from_onb(n_onb, normal, tangent);
+    }
+}
 template <uint32_t Q>
 __device__ __forceinline__ V3 get_normal(V3 ray_dir, V3 normal, V3 tangent, uint32_t seed,
                                          float rx, float ry) {
     if constexpr (Q == kQualityLow) {
         return normal;
     } else {
-        V3 ray_onb = to_onb(ray_dir, normal, tangent);
-        V3 n_onb = sample_vndf_ggx(seed, ray_onb, rx, ry);
-        return from_onb(n_onb, normal, tangent);
+        return get_normal_u<Q>(ray_dir, normal, tangent, rand_f32(seed), rand_f32(seed + 1u), rx, ry);
     }
 }
 // make_diffuse (note: the world-space new direction is passed as the
@@ -437,17 +449,29 @@ __device__ __forceinline__ float pick_ref_idx(float* picked, float low, float hi
     }
 }
 
+// Uniforms that stand in for the reference's draws in one scatter (a
+// stratified sampler's point): u0, u1 the direction pair (the microfacet
+// normal's, drawn at seed and seed + 1, and the cosine lobe's, drawn later;
+// a scatter uses one of the two for its direction), u2 the Fresnel choice.
+struct ScatterUniforms {
+    float u0, u1, u2;
+};
+
 // general.wesl:714-901 with every guiding candidate invalid ("plain"): the
 // Fresnel choice, the refractive-index pick, and diffuse / metal / dielectric
 // scattering.  Advances `seed` exactly as the reference does, multiplies
 // `color` by the sampled weight and returns the continuation direction.  The
-// caller accumulates the vertex emission first (general.wesl:714).
-template <uint32_t Q>
+// caller accumulates the vertex emission first (general.wesl:714).  With
+// kGiven the direction pair and the Fresnel choice come from `su` instead of
+// the seed's draws (the seed still advances as in the reference).
+template <uint32_t Q, bool kGiven = false>
 __device__ __forceinline__ V3 surface_scatter(const Isect& it, V3 dir, uint32_t& seed, V3& color,
-                                              float& picked) {
-    const V3 assign_normal = get_normal<Q>(dir, it.normal, it.tangent, seed, it.rough_x, it.rough_y);
+                                              float& picked, ScatterUniforms su = {}) {
+    const float g1 = kGiven ? su.u0 : rand_f32(seed);
+    const float g2 = kGiven ? su.u1 : rand_f32(seed + 1u);
+    const V3 assign_normal = get_normal_u<Q>(dir, it.normal, it.tangent, g1, g2, it.rough_x, it.rough_y);
     const V3 transparent_normal =
-        get_normal<Q>(dir, it.normal, it.tangent, seed, it.rough_tx, it.rough_ty);
+        get_normal_u<Q>(dir, it.normal, it.tangent, g1, g2, it.rough_tx, it.rough_ty);
     V3 normal = transparent_normal;
     bool should_reflect = false;
     seed += 1u;
@@ -461,7 +485,7 @@ __device__ __forceinline__ V3 surface_scatter(const Isect& it, V3 dir, uint32_t&
         const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
         const bool cannot_refract = ri * sin_theta > 1.0f;
         const float pdf = reflectance(cos_theta, ri);
-        const float rnd = rand_f32(seed);
+        const float rnd = kGiven ? su.u2 : rand_f32(seed);
         if (cannot_refract || pdf > rnd) should_reflect = true;
         seed += 1u;
     }
@@ -474,7 +498,9 @@ __device__ __forceinline__ V3 surface_scatter(const Isect& it, V3 dir, uint32_t&
     V3 ndir = dir;
     if (!reflecting) {
         if (it.ty == kMatDiffuse) {
-            const V3 dir_cos = cosine_weighted_hemisphere(seed, it.normal, it.tangent);
+            const V3 dir_cos =
+                kGiven ? from_onb(onb_cosine_weighted_hemisphere_u(su.u0, su.u1), it.normal, it.tangent)
+                       : cosine_weighted_hemisphere(seed, it.normal, it.tangent);
             const float cos_pdf = fmaxf(dot3(dir_cos, it.normal) / kPi, 0.0f);
             seed += 3u;  // theta, xi and the lobe choice
             const float avg_pdf = 1.0f * cos_pdf;

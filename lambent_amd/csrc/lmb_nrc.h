@@ -189,6 +189,9 @@ struct NrcTraceParams {
     uint32_t wide;            // > 1: samples traced up to this many at a time (path id
                               // pixel * wide + j for sample `sample` + j)
     uint32_t wide_count;      // samples of the current wide group (<= wide)
+    uint32_t stratify;        // first scatter from the pixel's scrambled Sobol point
+                              // seq_base + sample (+ j in a wide group)
+    uint32_t seq_base;
 };
 hipError_t launch_trace_nrc(const SceneDev& s, const CameraDev& cam, const FrameDev& f,
                             const TraceParams& p, const NrcDev& d, NrcTraceParams np,

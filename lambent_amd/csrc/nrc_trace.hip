@@ -23,6 +23,15 @@
 // scatter (the spread rule of neural radiance caching), so the cache's error
 // there is blurred by a lobe instead of shown sharply, and specular chains
 // stop n vertices late at most (their short queues are latency-bound).
+// With np.stratify the first scatter (at the camera ray's hit) takes its
+// direction pair and Fresnel choice from the pixel's Owen-scrambled Sobol
+// sequence (dev_qmc.h), point seq_base + sample index, with seq_base the
+// seed offset times the trace's sample count: the samples of a frame (and of
+// frames with consecutive seed offsets) spread over the lobe instead of
+// falling independently, and
+// every point is still uniform, so each sample stays unbiased.  Once the
+// cache ends the path at its next vertex, that scatter carries most of a
+// sample's variance.
 // Every path still meets at most one cache vertex; training continuations
 // (rslot != 0) never meet another.  Later samples of a trace reuse the
 // camera-ray hits, carry no training paths and stop after the last possible
@@ -35,6 +44,7 @@
 
 #include "dev_common.h"
 #include "dev_compact.h"
+#include "dev_qmc.h"
 #include "dev_shade.h"
 #include "lmb_frame_dev.h"
 #include "lmb_nrc.h"
@@ -200,7 +210,17 @@ __global__ __launch_bounds__(kNrcTraceBlock) void k_shade_nrc(SceneDev s, FrameD
                         f.pix_sum[pid] = make_float4(ps.x + scol.x, ps.y + scol.y, ps.z + scol.z, 0.f);
                     }
                 } else {
-                    const V3 ndir = surface_scatter<Q>(it, dir, seed, color, picked);
+                    V3 ndir;
+                    if (bounce == 0 && np.stratify) {
+                        // the pixel's stratified sequence; sample index
+                        // seq_base + sample (+ j of a wide group)
+                        const uint32_t j = np.wide > 1u ? pid % np.wide : 0u;
+                        float u[3];
+                        qmc_point3(np.seq_base + np.sample + j, np.wide > 1u ? pid / np.wide : pid, u);
+                        ndir = surface_scatter<Q, true>(it, dir, seed, color, picked, {u[0], u[1], u[2]});
+                    } else {
+                        ndir = surface_scatter<Q>(it, dir, seed, color, picked);
+                    }
                     bool alive = bounce + 1 < p.bounces;
                     if (alive && bounce >= p.rr_start) {
                         const float q = fminf(fmaxf(fmaxf(color.x, color.y), color.z), 1.0f);

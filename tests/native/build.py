@@ -1,5 +1,7 @@
 """Builds the test-only probe library libguide_probe.so (gfx950) from
-guide_probe.hip, with the product library's compile flags and headers.
+guide_probe.hip, with the product library's compile flags and headers, and
+libqmc_host.so, the cached tracer's stratified sampler (dev_qmc.h) for the
+CPU.
 
 Test infrastructure: only tests/ load it.
 """
@@ -14,6 +16,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 ROOT = HERE.parents[1]
 LIB = HERE / "libguide_probe.so"
+QMC_LIB = HERE / "libqmc_host.so"
 
 
 def _product_build():
@@ -34,6 +37,15 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}", file=sys.stderr)
+    tmp = QMC_LIB.with_suffix(".so.tmp")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{pb.CSRC}", "-o", str(tmp),
+           str(HERE / "qmc_host.cpp")]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"sampler build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, QMC_LIB)
+    if verbose:
+        print(f"built {QMC_LIB}", file=sys.stderr)
     return LIB
 
 
