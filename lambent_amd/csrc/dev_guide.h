@@ -142,15 +142,35 @@ __device__ __forceinline__ void astore(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Two adjacent words of a cell in one relaxed 8-byte atomic access (the
+// cell's words are 8-byte aligned in pairs from its start: 56-B cells in an
+// array aligned to 256 B).  Reading or writing two of the reference's
+// separately atomic words at one instant is one of the interleavings its
+// racy grid allows, and it halves the device-scope (L2-bypassing) accesses.
+__device__ __forceinline__ uint64_t aload2(const uint32_t* p) {
+    return __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void astore2(uint32_t* p, uint32_t lo, uint32_t hi) {
+    __hip_atomic_store(reinterpret_cast<uint64_t*>(p), ((uint64_t)hi << 32) | lo, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t lo32(uint64_t v) { return (uint32_t)v; }
+__device__ __forceinline__ uint32_t hi32(uint64_t v) { return (uint32_t)(v >> 32); }
+
 __device__ __forceinline__ Chain world_chain_read(const WorldMarkovChain* w) {
+    // words 6..13: light source (3), mean cosine, weight sum, samples, score,
+    // timeout
+    const uint64_t a = aload2(&w->chain_light_source[0]);
+    const uint64_t b = aload2(&w->chain_light_source[2]);
+    const uint64_t c2 = aload2(&w->chain_weight_sum);
+    const uint64_t d = aload2(&w->chain_score);
     Chain c;
-    c.ls = v3(__uint_as_float(aload(&w->chain_light_source[0])),
-              __uint_as_float(aload(&w->chain_light_source[1])),
-              __uint_as_float(aload(&w->chain_light_source[2])));
-    c.mean_cosine = __uint_as_float(aload(&w->chain_mean_cosine));
-    c.weight_sum = __uint_as_float(aload(&w->chain_weight_sum));
-    c.num_samples = aload(&w->chain_num_samples);
-    c.score = __uint_as_float(aload(&w->chain_score));
+    c.ls = v3(__uint_as_float(lo32(a)), __uint_as_float(hi32(a)), __uint_as_float(lo32(b)));
+    c.mean_cosine = __uint_as_float(hi32(b));
+    c.weight_sum = __uint_as_float(lo32(c2));
+    c.num_samples = hi32(c2);
+    c.score = __uint_as_float(lo32(d));
     return c;
 }
 
@@ -180,6 +200,43 @@ __device__ __forceinline__ Chain grid_load(WorldMarkovChain* cur, const WorldMar
     return s;
 }
 
+// N grid_loads at once (fill_mc_world_space_states' candidates): the key
+// words of both tables for every candidate are loaded together, then the
+// chains of the cells whose keys matched, so a lane waits for two memory
+// round trips instead of two per candidate.  Each candidate returns what
+// grid_load returns for it; last frame's key is also read where the current
+// cell matched (a relaxed load with no effect on the result).
+template <int N>
+__device__ __forceinline__ void grid_load_n(WorldMarkovChain* cur, const WorldMarkovChain* old,
+                                            const GridKey (&k)[N], const bool (&force_old)[N],
+                                            Chain (&out)[N]) {
+    uint32_t kc[N], ko[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        kc[i] = force_old[i] ? 0u : aload(&cur[k[i].idx].secondary_hash);
+        ko[i] = aload(&old[k[i].idx].secondary_hash);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const bool use_cur = !force_old[i] && kc[i] == k[i].secondary;  // the key is never 0
+        if (use_cur) {
+            out[i] = world_chain_read(cur + k[i].idx);
+            cur[k[i].idx].timeout = kCacheTimeout;
+        } else if (ko[i] == k[i].secondary) {
+            out[i] = world_chain_read(old + k[i].idx);
+        } else {
+            out[i] = chain_zero();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        out[i].ls = strip_nan3(out[i].ls);
+        out[i].mean_cosine = strip_nan(out[i].mean_cosine);
+        out[i].weight_sum = strip_nan(out[i].weight_sum);
+        out[i].score = strip_nan(out[i].score);
+    }
+}
+
 // general.wesl:228-246 grid_store_markov_chain
 __device__ __forceinline__ void grid_store(WorldMarkovChain* cur, const GridKey& k,
                                            const Chain& s) {
@@ -187,35 +244,35 @@ __device__ __forceinline__ void grid_store(WorldMarkovChain* cur, const GridKey&
     if (aload(&w->secondary_hash) != k.secondary) {
         if (atomicCAS(&w->secondary_hash, 0u, k.secondary) != 0u) return;
     }
-    astore(&w->chain_light_source[0], __float_as_uint(s.ls.x));
-    astore(&w->chain_light_source[1], __float_as_uint(s.ls.y));
-    astore(&w->chain_light_source[2], __float_as_uint(s.ls.z));
-    astore(&w->chain_mean_cosine, __float_as_uint(s.mean_cosine));
-    astore(&w->chain_weight_sum, __float_as_uint(s.weight_sum));
-    astore(&w->chain_num_samples, s.num_samples);
-    astore(&w->chain_score, __float_as_uint(s.score));
-    w->timeout = kCacheTimeout;
+    astore2(&w->chain_light_source[0], __float_as_uint(s.ls.x), __float_as_uint(s.ls.y));
+    astore2(&w->chain_light_source[2], __float_as_uint(s.ls.z), __float_as_uint(s.mean_cosine));
+    astore2(&w->chain_weight_sum, __float_as_uint(s.weight_sum), s.num_samples);
+    astore2(&w->chain_score, __float_as_uint(s.score), kCacheTimeout);  // and the timeout
 }
 
 // general.wesl:248-271 grid_load_radiance_estimate
 __device__ __forceinline__ V3 grid_radiance(WorldMarkovChain* cur, const WorldMarkovChain* old,
                                             const GridKey& k) {
-    uint32_t n = aload(&cur[k.idx].num_samples);
-    V3 rad = v3(__uint_as_float(aload(&cur[k.idx].luminance[0])),
-                __uint_as_float(aload(&cur[k.idx].luminance[1])),
-                __uint_as_float(aload(&cur[k.idx].luminance[2])));
-    if (aload(&cur[k.idx].secondary_hash) != k.secondary) {
-        if (aload(&old[k.idx].secondary_hash) == k.secondary) {
-            n = aload(&old[k.idx].num_samples);
-            rad = v3(__uint_as_float(aload(&old[k.idx].luminance[0])),
-                     __uint_as_float(aload(&old[k.idx].luminance[1])),
-                     __uint_as_float(aload(&old[k.idx].luminance[2])));
+    // words 0..4: secondary hash, samples, luminance (3); the key read with
+    // the samples (the reference reads it after the radiance)
+    const WorldMarkovChain* wc = cur + k.idx;
+    uint64_t h = aload2(&wc->secondary_hash);
+    uint64_t l01 = aload2(&wc->luminance[0]);
+    uint64_t l2 = aload2(&wc->luminance[2]);
+    if (lo32(h) != k.secondary) {
+        const WorldMarkovChain* wo = old + k.idx;
+        h = aload2(&wo->secondary_hash);
+        if (lo32(h) == k.secondary) {
+            l01 = aload2(&wo->luminance[0]);
+            l2 = aload2(&wo->luminance[2]);
         } else {
             return v3s(0.0f);
         }
     } else {
         cur[k.idx].timeout = kCacheTimeout;
     }
+    const uint32_t n = hi32(h);
+    const V3 rad = v3(__uint_as_float(lo32(l01)), __uint_as_float(hi32(l01)), __uint_as_float(lo32(l2)));
     if (n == 0u) return v3s(0.0f);
     V3 r = rad / (float)n;
     return v3(fminf(fmaxf(r.x, 0.0f), 1000.0f), fminf(fmaxf(r.y, 0.0f), 1000.0f),
@@ -232,20 +289,23 @@ __device__ __forceinline__ void grid_add_radiance(WorldMarkovChain* cur, const G
         if (atomicCAS(&w->secondary_hash, 0u, k.secondary) != 0u) return;
     }
     if (atomicExch(&w->lock, 1u) == 1u) return;
-    uint32_t n = aload(&w->num_samples);
+    // With the lock held: the sample count and the three sums read together
+    // (words 1..5), the sums written back as a pair and a pair with the
+    // lock's release (words 2..5).
+    const uint32_t n = aload(&w->num_samples);
+    const uint64_t l01 = aload2(&w->luminance[0]);
+    const uint32_t l2 = aload(&w->luminance[2]);
     if (n >= kMaxNumSamples) {
         astore(&w->lock, 0u);
         w->timeout = kCacheTimeout;
         return;
     }
     atomicAdd(&w->num_samples, 1u);
-    float r = strip_nan(c.x + __uint_as_float(aload(&w->luminance[0])));
-    astore(&w->luminance[0], __float_as_uint(fmaxf(r, 0.0f)));
-    float g = strip_nan(c.y + __uint_as_float(aload(&w->luminance[1])));
-    astore(&w->luminance[1], __float_as_uint(fmaxf(g, 0.0f)));
-    float b = strip_nan(c.z + __uint_as_float(aload(&w->luminance[2])));
-    astore(&w->luminance[2], __float_as_uint(fmaxf(b, 0.0f)));
-    astore(&w->lock, 0u);
+    const float r = strip_nan(c.x + __uint_as_float(lo32(l01)));
+    const float g = strip_nan(c.y + __uint_as_float(hi32(l01)));
+    const float b = strip_nan(c.z + __uint_as_float(l2));
+    astore2(&w->luminance[0], __float_as_uint(fmaxf(r, 0.0f)), __float_as_uint(fmaxf(g, 0.0f)));
+    astore2(&w->luminance[2], __float_as_uint(fmaxf(b, 0.0f)), 0u);  // and the lock's release
     w->timeout = kCacheTimeout;
 }
 
@@ -383,14 +443,19 @@ __device__ __forceinline__ uint64_t key_word(const GridKey& k) {
 // lanes of a wave that write one cell have one lane's stores land and one
 // lane take its try-lock; lanes whose keys share an index but not a
 // secondary hash are different keys, and each keeps its own writer.
+// One pass per distinct key of the wave: the lowest lane not yet settled
+// broadcasts its key, which settles every lane holding it, and is that
+// key's writer.
 __device__ __forceinline__ bool first_writer_of_key(const GridKey& k) {
-    bool first = true;
-    const uint64_t live = __ballot(1);
-    for (int l = 0; l < 64; ++l) {
-        const uint32_t oi = __shfl(k.idx, l);
-        const uint32_t os = __shfl(k.secondary, l);
-        if (l < (int)__lane_id() && ((live >> l) & 1ull) && oi == k.idx && os == k.secondary)
-            first = false;
+    bool first = false;
+    uint64_t todo = __ballot(1);
+    const int me = (int)__lane_id();
+    while (todo) {
+        const int l = __builtin_ctzll(todo);  // wave-uniform
+        const uint32_t li = __shfl(k.idx, l);
+        const uint32_t ls = __shfl(k.secondary, l);
+        todo &= ~__ballot(li == k.idx && ls == k.secondary);
+        first |= me == l;
     }
     return first;
 }

@@ -288,8 +288,11 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
                 const uint32_t ty = meta & 0xFFu;
                 const bool should_reflect = (meta & 0x100u) != 0u;
                 uint32_t seed = f.ps_seed[pid];
-                // fill_mc_world_space_states (general.wesl:389-400)
+                // fill_mc_world_space_states (general.wesl:389-400): the
+                // candidates' keys first, then their cells together
                 Chain wc[kWorldResamples];
+                GridKey wk[kWorldResamples];
+                bool wforce[kWorldResamples];
 #pragma unroll
                 for (uint32_t i = 0; i < kWorldResamples; ++i) {
                     const bool rand_candidate = rand_f32(seed) < 0.7f;
@@ -301,10 +304,11 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
                     const V3 jitter =
                         v3(jx - floorf(jx), 0.0f, jz - floorf(jz)) * jitter_size * jitter_mul;
                     const V3 pos = origin + from_onb_no_tangent(jitter, inormal);
-                    const GridKey k = grid_key(pos, inormal, dir, camera_loc, f.world_size);
-                    wc[i] = grid_load(f.wmc, f.wmc_old, k, rand_candidate);
+                    wk[i] = grid_key(pos, inormal, dir, camera_loc, f.world_size);
+                    wforce[i] = rand_candidate;
                     seed += 1u;
                 }
+                grid_load_n(f.wmc, f.wmc_old, wk, wforce, wc);
                 const Chain tent = chain_load(g.tent[pid]);
                 const uint32_t x = pid % f.width, y = pid / f.width;
                 const ScreenCands sc = screen_candidates(g.cand_seed[pid], x, y, f.width, kSubgroup);
