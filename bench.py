@@ -92,6 +92,68 @@ def rel_mse(img, ref) -> float:
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
 
 
+class _Hip:
+    """The HIP runtime that the library itself is linked against (the same
+    loaded libamdhip64), through ctypes: events on the null stream, which the
+    library's calls use when given no stream, device allocations and copies.
+    (torch brings a HIP runtime of its own, which cannot start in a process
+    where the library's already has.)"""
+
+    def __init__(self):
+        import ctypes
+        self.ct = ctypes
+        self.lib = ctypes.CDLL("libamdhip64.so")
+        self.lib.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.lib.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.lib.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                                 ctypes.c_void_p]
+        self.lib.hipEventDestroy.argtypes = [ctypes.c_void_p]
+        self.lib.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.lib.hipFree.argtypes = [ctypes.c_void_p]
+        self.lib.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                            ctypes.c_int, ctypes.c_void_p]
+        self.lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.c_int]
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with HIP error {rc}")
+
+    def event(self):
+        e = self.ct.c_void_p()
+        self._ok(self.lib.hipEventCreate(self.ct.byref(e)), "hipEventCreate")
+        return e
+
+    def record(self, e):
+        self._ok(self.lib.hipEventRecord(e, None), "hipEventRecord")
+
+    def elapsed_s(self, a, b) -> float:
+        ms = self.ct.c_float()
+        self._ok(self.lib.hipEventElapsedTime(self.ct.byref(ms), a, b), "hipEventElapsedTime")
+        return ms.value * 1e-3
+
+    def destroy(self, e):
+        self.lib.hipEventDestroy(e)
+
+    def sync(self):
+        self._ok(self.lib.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+    def malloc(self, n: int):
+        p = self.ct.c_void_p()
+        self._ok(self.lib.hipMalloc(self.ct.byref(p), n), "hipMalloc")
+        return p
+
+    def free(self, p):
+        self.lib.hipFree(p)
+
+    def copy_d2d_async(self, dst, src: int, n: int):
+        self._ok(self.lib.hipMemcpyAsync(dst, self.ct.c_void_p(src), n, 3, None), "hipMemcpyAsync")
+
+    def read(self, src, out):
+        self._ok(self.lib.hipMemcpy(out.ctypes.data_as(self.ct.c_void_p), src, out.nbytes, 2),
+                 "hipMemcpy")
+
+
 def cache_reference(api, scene, spec, cam, ref_spp: int):
     """The plain tracer's 4096-spp fp64 reference and its 16 / 64 / 256-spp
     points (8 bounces, RR from 3) for the time-to-relMSE legs."""
@@ -100,16 +162,20 @@ def cache_reference(api, scene, spec, cam, ref_spp: int):
     popts = dict(bounces=8, rr_start=3, guiding=api.Guiding.OFF,
                  restir_temporal=api.RestirTemporalMode.Disabled)
 
+    hip = _Hip()
+    ev0, ev1 = hip.event(), hip.event()
+
     def plain(spp, seed, timed=False):
         pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=spp, **popts))
         b = api.DataBuffers(W, H, 1, api.BufferType(0))
         if timed:  # buffers allocated and the kernels loaded outside the timed call
             pipe.dispatch(scene, cam, b, seed + 1_000_000)
-        api.sync()
-        t0 = time.perf_counter()
+        hip.sync()
+        hip.record(ev0)
         pipe.dispatch(scene, cam, b, seed)
-        api.sync()
-        dt = time.perf_counter() - t0
+        hip.record(ev1)
+        hip.sync()
+        dt = hip.elapsed_s(ev0, ev1)
         return b.output(api.Output.COLOR)[..., :3].astype(np.float64), dt
 
     t0 = time.perf_counter()
@@ -128,7 +194,7 @@ def cache_reference(api, scene, spec, cam, ref_spp: int):
 
 
 def cache_leg(api, scene, spec, cam, reference, frames: int, warm: int,
-              lr: float = 1e-3, spp: int = 8, steps: int = 32, overlap: bool = True,
+              lr: float = 1e-3, spp: int = 8, steps: int = 16, overlap: bool = True,
               warm_paths: int = 524288, warm_steps: int = 256, ema: float = 0.0,
               defer_specular: int = 2, stratify: bool = True) -> dict:
     """Time-to-relMSE of the cached tracer against the plain tracer, both with
@@ -140,9 +206,12 @@ def cache_leg(api, scene, spec, cam, reference, frames: int, warm: int,
     frame), `steps` Adam steps per frame over the frame's 65536 training
     records (overlap: a training pass of its own and the steps run on the
     cache's own stream beside the frame's samples, which are traced eight at
-    a time; otherwise the training paths come from the first sample).  Times
-    are wall times of the dispatches (host read-back of the images for
-    scoring is not counted).  defer_specular: how many specular scatters may
+    a time; otherwise the training paths come from the first sample).  The
+    frames run back to back on one stream as in a progressive render, each
+    image pushed into a frame ring on the GPU; times are stream times
+    (events) to the end of each checkpoint frame, and the checkpoint averages
+    are scored after the run.  The plain tracer's points are timed the same way
+    (one dispatch each).  defer_specular: how many specular scatters may
     move a path's cache vertex one vertex on (0: always the second vertex).
     stratify: the first scatter draws from each pixel's scrambled Sobol
     sequence (False: the reference's hash draws, as the plain tracer's)."""
@@ -159,28 +228,57 @@ def cache_leg(api, scene, spec, cam, reference, frames: int, warm: int,
     warm_pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=1, **popts))
     pipe = api.RayTracer(api.Medium).create_pipeline(api.PathTracerOpts(samples=spp, **popts))
     b = api.DataBuffers(W, H, 1, api.BufferType(0))
-    t = 0.0
+    # A progressive render: the warm-up and the frames back to back on one
+    # stream, each frame's image pushed into a frame ring (the example
+    # application's accumulation, one slot per frame); at a checkpoint the
+    # ring's average is copied into a snapshot on the device.  Times are
+    # stream times (events) to the end of each checkpoint frame, less the
+    # averages and copies of the checkpoints before it; the snapshots are
+    # read and scored after the run.
+    hip = _Hip()
+    ring = api.FrameRing(W, H, frames)
+    rgb_ptr, rgb_bytes = ring.output_device_ptr()
+    checks = sorted(set(range(1, 65, 2)) | set(range(64, frames + 1, 8)) | {frames})
+    snap_mem = {n: hip.malloc(rgb_bytes) for n in checks}
+    ev_start, ev_warm, ev_frames = hip.event(), hip.event(), hip.event()
+    hip.sync()
+    hip.record(ev_start)
     for f in range(warm):
-        api.sync()
-        t0 = time.perf_counter()
         warm_pipe.train_cached(scene, cam, b, cache, f)
-        api.sync()
-        t += time.perf_counter() - t0
-    t_warm = t
+    hip.record(ev_warm)
+    hip.sync()  # the options change below is a host call
     cache.set_opts(frame_opts)
-    acc = np.zeros((H, W, 3))
-    checks = set(range(1, 65, 2)) | set(range(64, frames + 1, 8)) | {frames}
-    pts = []
+    hip.record(ev_frames)
+    snaps = []
     for n in range(1, frames + 1):
-        api.sync()
-        t0 = time.perf_counter()
         pipe.dispatch_cached(scene, cam, b, cache, warm + n)
-        api.sync()
-        t += time.perf_counter() - t0
-        acc += b.output(api.Output.COLOR)[..., :3]
-        if n in checks:
-            pts.append({"frames": n, "samples": n * spp, "seconds": t,
-                        "relmse": rel_mse(acc / n, ref)})
+        ring.push(b)
+        if n in snap_mem:
+            ev_a, ev_b = hip.event(), hip.event()
+            hip.record(ev_a)
+            ring.average_on_device()
+            hip.copy_d2d_async(snap_mem[n], rgb_ptr, rgb_bytes)
+            hip.record(ev_b)
+            snaps.append((n, ev_a, ev_b))
+    hip.sync()
+    t_warm = hip.elapsed_s(ev_start, ev_warm)
+    pts = []  # the host gap at the options change is not counted
+    excluded = 0.0
+    img = np.empty((H, W, 3), np.float32)
+    for n, ev_a, ev_b in snaps:
+        t_frames = hip.elapsed_s(ev_frames, ev_a) - excluded
+        excluded += hip.elapsed_s(ev_a, ev_b)
+        hip.read(snap_mem[n], img)
+        pts.append({"frames": n, "samples": n * spp, "seconds": t_warm + t_frames,
+                    "relmse": rel_mse(img.astype(np.float64), ref)})
+        hip.destroy(ev_a)
+        hip.destroy(ev_b)
+    for p in snap_mem.values():
+        hip.free(p)
+    for e in (ev_start, ev_warm, ev_frames):
+        hip.destroy(e)
+    ring.close()
+    t = pts[-1]["seconds"]
     log(f"cache leg (spread bound {defer_specular}, stratify {stratify}): {warm} + {frames} frames in {t:.2f} s, "
         f"final relMSE {pts[-1]['relmse']:.5f}")
     # Matrix-core figures from a few more frames with per-launch events (kept
@@ -512,7 +610,7 @@ def main() -> int:
     ap.add_argument("--cache-warmup", type=int, default=4)
     ap.add_argument("--cache-lr", type=float, default=1e-3)
     ap.add_argument("--cache-spp", type=int, default=8)
-    ap.add_argument("--cache-steps", type=int, default=32)
+    ap.add_argument("--cache-steps", type=int, default=16)
     ap.add_argument("--cache-ema", type=float, default=0.0,
                     help="inference from an exponential moving average of the weights")
     ap.add_argument("--cache-warm-paths", type=int, default=524288)

@@ -972,6 +972,11 @@ class FrameRing:
         _check(_lib.lmb_frame_ring_output(self._h, ctypes.byref(p), ctypes.byref(n)))
         return p.value, n.value
 
+    def average_on_device(self, stream=None) -> None:
+        """Enqueues the average into the device buffer of output_device_ptr()
+        without reading it back."""
+        _check(_lib.lmb_frame_ring_average(self._h, stream))
+
     def close(self):
         if self._h:
             _lib.lmb_frame_ring_destroy(self._h)

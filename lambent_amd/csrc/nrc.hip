@@ -67,6 +67,28 @@ __device__ __forceinline__ void fill_lds(bf16x8* __restrict__ lds, const bf16x8*
     }
 }
 
+// fill_lds over two sources into one LDS range: elements [0, n0) from s0,
+// [n0, n0 + n1) from s1, kFillBatch2 loads in flight per thread.
+constexpr int kFillBatch2 = 14;
+template <int BLOCK>
+__device__ __forceinline__ void fill_lds2(bf16x8* __restrict__ lds, const bf16x8* __restrict__ s0,
+                                          int n0, const bf16x8* __restrict__ s1, int n1) {
+    const int n = n0 + n1;
+    for (int base = 0; base < n; base += BLOCK * kFillBatch2) {
+        bf16x8 v[kFillBatch2];
+#pragma unroll
+        for (int k = 0; k < kFillBatch2; ++k) {
+            const int i = base + k * BLOCK + (int)threadIdx.x;
+            if (i < n) v[k] = i < n0 ? s0[i] : s1[i - n0];
+        }
+#pragma unroll
+        for (int k = 0; k < kFillBatch2; ++k) {
+            const int i = base + k * BLOCK + (int)threadIdx.x;
+            if (i < n) lds[i] = v[k];
+        }
+    }
+}
+
 struct FragPtr {
     const bf16x8* p;
     __device__ __forceinline__ bf16x8 operator()(int frag, int lane) const {
@@ -302,12 +324,8 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_fwd_bwd(NrcDev d, const NrcQu
                                                            uint32_t count_limit, uint32_t steps,
                                                            uint32_t step) {
     __shared__ bf16x8 lds[(kNrcFwdFrags + kNrcBwdFrags) * 64];
-    {
-        const bf16x8* sf = reinterpret_cast<const bf16x8*>(d.fwd);
-        const bf16x8* sb = reinterpret_cast<const bf16x8*>(d.bwd);
-        fill_lds<kNrcBlock>(lds, sf, kNrcFwdFrags * 64);
-        fill_lds<kNrcBlock>(lds + kNrcFwdFrags * 64, sb, kNrcBwdFrags * 64);
-    }
+    fill_lds2<kNrcBlock>(lds, reinterpret_cast<const bf16x8*>(d.fwd), kNrcFwdFrags * 64,
+                         reinterpret_cast<const bf16x8*>(d.bwd), kNrcBwdFrags * 64);
     __syncthreads();
     const FragPtr fr{lds}, br{lds + kNrcFwdFrags * 64};
     const uint32_t S = d.batch;
