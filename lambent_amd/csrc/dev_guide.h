@@ -158,6 +158,27 @@ __device__ __forceinline__ void astore2(uint32_t* p, uint32_t lo, uint32_t hi) {
 __device__ __forceinline__ uint32_t lo32(uint64_t v) { return (uint32_t)v; }
 __device__ __forceinline__ uint32_t hi32(uint64_t v) { return (uint32_t)(v >> 32); }
 
+// Plain (L2-cached) reads of cells that no thread writes while the kernel
+// runs: last frame's table, which only advance_frame writes, and in
+// k_g_guide the current table too (its writers are k_g_shade and
+// k_g_finish, other launches on the same stream; k_g_guide itself writes
+// only timeouts).  They return what the relaxed atomic loads would, and
+// neighbouring vertices that read one cell share its L2 line instead of
+// each reading it from memory (device-scope atomics bypass the L2).
+__device__ __forceinline__ Chain world_chain_read_plain(const WorldMarkovChain* w) {
+    const uint2 a = *reinterpret_cast<const uint2*>(&w->chain_light_source[0]);
+    const uint2 b = *reinterpret_cast<const uint2*>(&w->chain_light_source[2]);
+    const uint2 c2 = *reinterpret_cast<const uint2*>(&w->chain_weight_sum);
+    const uint32_t d = w->chain_score;
+    Chain c;
+    c.ls = v3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(b.x));
+    c.mean_cosine = __uint_as_float(b.y);
+    c.weight_sum = __uint_as_float(c2.x);
+    c.num_samples = c2.y;
+    c.score = __uint_as_float(d);
+    return c;
+}
+
 __device__ __forceinline__ Chain world_chain_read(const WorldMarkovChain* w) {
     // words 6..13: light source (3), mean cosine, weight sum, samples, score,
     // timeout
@@ -185,8 +206,8 @@ __device__ __forceinline__ Chain grid_load(WorldMarkovChain* cur, const WorldMar
                                            const GridKey& k, bool force_old) {
     Chain s;
     if (force_old || aload(&cur[k.idx].secondary_hash) != k.secondary) {
-        if (aload(&old[k.idx].secondary_hash) == k.secondary)
-            s = world_chain_read(old + k.idx);
+        if (old[k.idx].secondary_hash == k.secondary)  // last frame's table: plain reads
+            s = world_chain_read_plain(old + k.idx);
         else
             s = chain_zero();
     } else {
@@ -206,24 +227,28 @@ __device__ __forceinline__ Chain grid_load(WorldMarkovChain* cur, const WorldMar
 // round trips instead of two per candidate.  Each candidate returns what
 // grid_load returns for it; last frame's key is also read where the current
 // cell matched (a relaxed load with no effect on the result).
-template <int N>
+// kCurStill: no thread writes the current table's keys or chains while
+// the kernel runs (k_g_guide), so its reads are plain too.
+template <bool kCurStill, int N>
 __device__ __forceinline__ void grid_load_n(WorldMarkovChain* cur, const WorldMarkovChain* old,
                                             const GridKey (&k)[N], const bool (&force_old)[N],
                                             Chain (&out)[N]) {
     uint32_t kc[N], ko[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        kc[i] = force_old[i] ? 0u : aload(&cur[k[i].idx].secondary_hash);
-        ko[i] = aload(&old[k[i].idx].secondary_hash);
+        kc[i] = force_old[i] ? 0u
+                : kCurStill  ? cur[k[i].idx].secondary_hash
+                             : aload(&cur[k[i].idx].secondary_hash);
+        ko[i] = old[k[i].idx].secondary_hash;
     }
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const bool use_cur = !force_old[i] && kc[i] == k[i].secondary;  // the key is never 0
         if (use_cur) {
-            out[i] = world_chain_read(cur + k[i].idx);
+            out[i] = kCurStill ? world_chain_read_plain(cur + k[i].idx) : world_chain_read(cur + k[i].idx);
             cur[k[i].idx].timeout = kCacheTimeout;
         } else if (ko[i] == k[i].secondary) {
-            out[i] = world_chain_read(old + k[i].idx);
+            out[i] = world_chain_read_plain(old + k[i].idx);
         } else {
             out[i] = chain_zero();
         }
@@ -260,11 +285,11 @@ __device__ __forceinline__ V3 grid_radiance(WorldMarkovChain* cur, const WorldMa
     uint64_t l01 = aload2(&wc->luminance[0]);
     uint64_t l2 = aload2(&wc->luminance[2]);
     if (lo32(h) != k.secondary) {
-        const WorldMarkovChain* wo = old + k.idx;
-        h = aload2(&wo->secondary_hash);
+        const WorldMarkovChain* wo = old + k.idx;  // last frame's table: plain reads
+        h = *reinterpret_cast<const uint64_t*>(&wo->secondary_hash);
         if (lo32(h) == k.secondary) {
-            l01 = aload2(&wo->luminance[0]);
-            l2 = aload2(&wo->luminance[2]);
+            l01 = *reinterpret_cast<const uint64_t*>(&wo->luminance[0]);
+            l2 = *reinterpret_cast<const uint64_t*>(&wo->luminance[2]);
         } else {
             return v3s(0.0f);
         }

@@ -308,7 +308,8 @@ __global__ __launch_bounds__(kGBlock) void k_g_guide(SceneDev s, FrameDev f, Gui
                     wforce[i] = rand_candidate;
                     seed += 1u;
                 }
-                grid_load_n(f.wmc, f.wmc_old, wk, wforce, wc);
+                // k_g_guide writes no key or chain of the grid: plain reads
+                grid_load_n<true>(f.wmc, f.wmc_old, wk, wforce, wc);
                 const Chain tent = chain_load(g.tent[pid]);
                 const uint32_t x = pid % f.width, y = pid / f.width;
                 const ScreenCands sc = screen_candidates(g.cand_seed[pid], x, y, f.width, kSubgroup);
