@@ -334,6 +334,71 @@ __device__ __forceinline__ void grid_add_radiance(WorldMarkovChain* cur, const G
     w->timeout = kCacheTimeout;
 }
 
+// grid_add_radiance, then (when `update`) grid_load, update and grid_store
+// of the same key, as the training of a path vertex performs them
+// (general.wesl:926-935), with the cell's key read once.  Within a launch a
+// key only changes from 0 to a claimed value, so the add's key check and
+// claim settle what the load's and the store's key checks would see: the
+// cell is this key's if it held it or the claim found 0 or this key (a
+// claim that found this key makes the add give up, as in grid_add_radiance,
+// while the load and store still see the key).  The chain is read before the
+// lock is taken, so its round trip overlaps the lock's; the chain words are
+// not written by the add, and reading them before the add's stores is one
+// of the orders the relaxed atomics allow.  upd(chain) returns the chain
+// to store; it sees the loaded chain whenever `update` is set.
+template <class Upd>
+__device__ __forceinline__ void grid_add_and_train(WorldMarkovChain* cur, const WorldMarkovChain* old,
+                                                   const GridKey& k, V3 radiance, bool update,
+                                                   Upd&& upd) {
+    WorldMarkovChain* w = cur + k.idx;
+    bool ours = true, add = true;
+    if (aload(&w->secondary_hash) != k.secondary) {
+        const uint32_t prev = atomicCAS(&w->secondary_hash, 0u, k.secondary);
+        ours = prev == 0u || prev == k.secondary;
+        add = prev == 0u;
+    }
+    Chain s = chain_zero();
+    if (update) {
+        if (ours) {
+            s = world_chain_read(w);
+            w->timeout = kCacheTimeout;
+        } else if (old[k.idx].secondary_hash == k.secondary) {  // last frame's table: plain reads
+            s = world_chain_read_plain(old + k.idx);
+        }
+        s.ls = strip_nan3(s.ls);
+        s.mean_cosine = strip_nan(s.mean_cosine);
+        s.weight_sum = strip_nan(s.weight_sum);
+        s.score = strip_nan(s.score);
+    }
+    if (add && atomicExch(&w->lock, 1u) != 1u) {
+        const V3 c = v3(fminf(fmaxf(radiance.x, 0.0f), 100.0f), fminf(fmaxf(radiance.y, 0.0f), 100.0f),
+                        fminf(fmaxf(radiance.z, 0.0f), 100.0f));
+        const uint32_t n = aload(&w->num_samples);
+        const uint64_t l01 = aload2(&w->luminance[0]);
+        const uint32_t l2 = aload(&w->luminance[2]);
+        if (n >= kMaxNumSamples) {
+            astore(&w->lock, 0u);
+            w->timeout = kCacheTimeout;
+        } else {
+            atomicAdd(&w->num_samples, 1u);
+            const float r = strip_nan(c.x + __uint_as_float(lo32(l01)));
+            const float g = strip_nan(c.y + __uint_as_float(hi32(l01)));
+            const float b = strip_nan(c.z + __uint_as_float(l2));
+            astore2(&w->luminance[0], __float_as_uint(fmaxf(r, 0.0f)), __float_as_uint(fmaxf(g, 0.0f)));
+            astore2(&w->luminance[2], __float_as_uint(fmaxf(b, 0.0f)), 0u);  // and the lock's release
+            w->timeout = kCacheTimeout;
+        }
+    }
+    if (update) {
+        const Chain st = upd(s);  // as grid_store, stored only when the cell is this key's
+        if (!ours) return;
+        astore2(&w->chain_light_source[0], __float_as_uint(st.ls.x), __float_as_uint(st.ls.y));
+        astore2(&w->chain_light_source[2], __float_as_uint(st.ls.z), __float_as_uint(st.mean_cosine));
+        astore2(&w->chain_weight_sum, __float_as_uint(st.weight_sum), st.num_samples);
+        astore2(&w->chain_score, __float_as_uint(st.score), kCacheTimeout);  // and the timeout
+    }
+}
+
 // general.wesl:495-498
 __device__ __forceinline__ V3 get_light_dir(const Chain& s, V3 current) {
     return safe_div_v3(s.ls, s.weight_sum) - current;

@@ -123,14 +123,13 @@ __device__ __forceinline__ void train_previous(const FrameDev& f, const GuidedDe
                                                uint32_t* seed) {
     const float4 ln = g.lastn[pid], ld = g.lastd[pid];
     const GridKey kp = grid_key(prev_pos, xyz(ln), xyz(ld), camera_loc, f.world_size);
-    grid_add_radiance(f.wmc, kp, total);
     const float weight = length3(total);
+    const float sum_mc = g.sum_mc[pid];
+    // grid_add_radiance, then for weight > 0 grid_load / update / grid_store
+    grid_add_and_train(f.wmc, f.wmc_old, kp, total, weight > 0.0f, [&](const Chain& wc) {
+        return update_markov_state(prev_pos, new_pos, weight, *seed, false, wc, 1.0f, sum_mc);
+    });
     if (weight > 0.0f) {
-        const float sum_mc = g.sum_mc[pid];
-        const Chain wc = grid_load(f.wmc, f.wmc_old, kp, false);
-        const Chain st = update_markov_state(prev_pos, new_pos, weight, *seed, false, wc, 1.0f,
-                                             sum_mc);
-        grid_store(f.wmc, kp, st);
         const Chain t = chain_load(g.tent[pid]);
         g.tent[pid] = chain_store(update_markov_state(prev_pos, new_pos, length3(radiance_out),
                                                       *seed, last_not_vmf, t,
@@ -505,10 +504,13 @@ __global__ __launch_bounds__(kGBlock) void k_g_finish(FrameDev f, GuidedDev g, T
             const V3 point = xyz(sp), normal = xyz(sn);
             const V3 vis_point = xyz(g.s_vis_point[pix]);
             const V3 sout = xyz(g.sout[pix]);
-            const Chain wc = grid_load(f.wmc, f.wmc_old, k, false);
-            const Chain st = update_markov_state(point, vis_point, length3(sout), seed, false, wc, 1.0f,
-                                                 g.sum_mc[pix]);
             if (writer) {
+                // Only the writer's chain update lands, so only it loads
+                // the chain; another lane's load would differ in nothing
+                // but a timeout write that the writer's store repeats.
+                const Chain wc = grid_load(f.wmc, f.wmc_old, k, false);
+                const Chain st = update_markov_state(point, vis_point, length3(sout), seed, false, wc,
+                                                     1.0f, g.sum_mc[pix]);
                 grid_store(f.wmc, k, st);
                 grid_add_radiance(f.wmc, k, sout);
             }

@@ -189,8 +189,23 @@ __global__ void k_grid_ops(WorldMarkovChain* cur, const WorldMarkovChain* old, u
             grid_store(cur, k, ldc(chains_in, i));
         } else if (ops[i] == 2u) {
             r = grid_radiance(cur, old, k);
-        } else {
+        } else if (ops[i] == 3u) {
             grid_add_radiance(cur, k, v3(a[9], a[10], a[11]));
+        } else {
+            // the fused add / load / update / store of a vertex's training,
+            // with a fixed update of the loaded chain (the test restates it)
+            grid_add_and_train(cur, old, k, v3(a[9], a[10], a[11]), force[i] != 0u,
+                               [&](const Chain& wc) {
+                                   c = wc;
+                                   const Chain ci = ldc(chains_in, i);
+                                   Chain st = wc;
+                                   st.ls = ci.ls;
+                                   st.mean_cosine = wc.mean_cosine * 0.5f;
+                                   st.weight_sum = wc.weight_sum + 1.0f;
+                                   st.num_samples = wc.num_samples + 3u;
+                                   st.score = ci.score;
+                                   return st;
+                               });
         }
         stc(chains_out, i, c);
         rad_out[3 * i] = r.x;

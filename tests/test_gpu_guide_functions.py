@@ -234,8 +234,8 @@ def test_world_grid_operations_bit_exact(probe):
                        chain=random_chains(rng, 1)[0])
         oracle.grid_op(oracle.GRID_ADD, old, cur, cam, pts[i], nrm[i], inc[i], radiance=(1, 2, 3))
     cur[:] = 0
-    nops = 600
-    ops = rng.integers(0, 4, nops).astype(np.uint32)
+    nops = 800
+    ops = rng.integers(0, 5, nops).astype(np.uint32)  # 4: fused add and train
     which = rng.integers(0, 40, nops)
     args = np.zeros((nops, 12), np.float32)
     args[:, 0:3] = pts[which]
@@ -250,6 +250,21 @@ def test_world_grid_operations_bit_exact(probe):
     ok(probe.probe_grid_ops(gpu_cur.ctypes.data_as(ctypes.c_void_p), old.ctypes.data_as(ctypes.c_void_p),
                             table, F(cam), U(ops), F(args), U(force), U(chains), nops, U(cout), F(rout)))
     for i in range(nops):
+        if ops[i] == 4:
+            # grid_add_and_train: the add, then (flag set) load, update, store
+            geo = (cam, args[i, 0:3], args[i, 3:6], args[i, 6:9])
+            oracle.grid_op(oracle.GRID_ADD, cur, old, *geo, radiance=args[i, 9:12])
+            if force[i]:
+                wc = oracle.grid_op(oracle.GRID_LOAD, cur, old, *geo)
+                np.testing.assert_array_equal(cout[i], wc, err_msg=f"op {i}")
+                st = wc.copy()
+                st[0:3] = chains[i][0:3]
+                st[3] = (wc[3:4].view(np.float32) * np.float32(0.5)).view(np.uint32)[0]
+                st[4] = (wc[4:5].view(np.float32) + np.float32(1.0)).view(np.uint32)[0]
+                st[5] = (int(wc[5]) + 3) & 0xFFFFFFFF
+                st[6] = chains[i][6]
+                oracle.grid_op(oracle.GRID_STORE, cur, old, *geo, chain=st)
+            continue
         r = oracle.grid_op(int(ops[i]), cur, old, cam, args[i, 0:3], args[i, 3:6], args[i, 6:9],
                            force_old=bool(force[i]), chain=chains[i], radiance=args[i, 9:12])
         if ops[i] == oracle.GRID_LOAD:
