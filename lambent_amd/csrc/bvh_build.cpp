@@ -2,7 +2,10 @@
 // driver-built BLAS of the reference (examples/cornell_box/src/main.rs:464-563;
 // the reference's ray queries traverse it at general.wesl:1052-1078).
 //
-// 1. Binned-SAH binary BVH (16 bins per axis) down to single triangles,
+// 0. Pre-splitting: the triangle references whose boxes waste the most area
+//    are split by clipping, up to half as many extra references as
+//    triangles.
+// 1. Binned-SAH binary BVH (16 bins per axis) down to single references,
 //    subtrees above 64k primitives built on separate threads.
 // 2. Collapse to 8-wide by dynamic programming over the binary tree
 //    (the cost-optimal collapse of Ylitie, Karras and Laine 2017, "Efficient
@@ -71,6 +74,28 @@ static_assert(kBinaryLeaf >= 1 && kBinaryLeaf <= kMaxLeaf, "binary leaves must f
 constexpr float kNodeCost = 1.0f;
 constexpr float kTriCost = (float)(LMB_BVH_TRI_COST);
 constexpr uint32_t kParallelThreshold = 1u << 16;
+
+// Pre-splitting budget: up to this many extra references per 100 triangles
+// (0: none).  On the 1M-triangle soup at 1080p, k_extend takes 2.05, 1.99,
+// 1.976 and 1.972 ms per frame at 0, 25, 50 and 100 (108k, 139k, 169k and
+// 226k nodes; 101, 115, 130 and 158 MB of BVH).
+#ifndef LMB_BVH_PRESPLIT
+#define LMB_BVH_PRESPLIT 50
+#endif
+constexpr uint32_t kPresplitPercent = LMB_BVH_PRESPLIT;
+#ifndef LMB_BVH_PRESPLIT_RATIO
+#define LMB_BVH_PRESPLIT_RATIO 3.0
+#endif
+constexpr double kPresplitRatio = LMB_BVH_PRESPLIT_RATIO;
+// Scenes below this many triangles are not pre-split: there the duplicates
+// cost more triangle tests than the tighter boxes save (the 36-triangle
+// Cornell box at 256 x 256, 16 spp: 1.14 -> 1.20 ms per frame when split;
+// the 64-scene batch of 1k-500k triangles: 59.3 -> 60.6 ms when split, and
+// 60.4 ms when only its scenes of 2^17 triangles or more are).
+#ifndef LMB_BVH_PRESPLIT_MIN
+#define LMB_BVH_PRESPLIT_MIN 524288
+#endif
+constexpr size_t kPresplitMinTriangles = LMB_BVH_PRESPLIT_MIN;
 
 inline float half_area(const float lo[3], const float hi[3]) {
     float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
@@ -309,6 +334,158 @@ inline bool finite3(const float* v) {
 // 2^e >= extent / 253 so that q_hi <= 255 with a plane to spare, and
 // 2^e >= max|coord| / 2^22 so that every dequantized plane (k + q) * 2^e is
 // an exact float.
+// Largest float <= x and smallest float >= x.
+inline float float_down(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = std::nextafter(f, -std::numeric_limits<float>::infinity());
+    return f;
+}
+inline float float_up(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+    return f;
+}
+
+// Bounds of the part of triangle (a, b, c) inside the box [lo, hi] and on
+// one side of the plane x_axis = plane (side 0: <=, 1: >=), by clipping the
+// triangle polygon in double precision (Sutherland-Hodgman); rounded outward
+// to floats.  False when the part is empty.
+bool clipped_bounds(const double tri[3][3], const float lo[3], const float hi[3], int axis,
+                    double plane, int side, float out_lo[3], float out_hi[3]) {
+    double poly[16][3], tmp[16][3];
+    int n = 3;
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) poly[i][k] = tri[i][k];
+    // planes: the box's six, then the split plane
+    for (int pl = 0; pl < 7 && n > 0; ++pl) {
+        int ax;
+        double v;
+        bool keep_below;
+        if (pl < 6) {
+            ax = pl >> 1;
+            keep_below = (pl & 1) != 0;  // even: keep >= lo, odd: keep <= hi
+            v = keep_below ? (double)hi[ax] : (double)lo[ax];
+        } else {
+            ax = axis;
+            keep_below = side == 0;
+            v = plane;
+        }
+        int m = 0;
+        for (int i = 0; i < n; ++i) {
+            const double* p = poly[i];
+            const double* q = poly[(i + 1) % n];
+            const double dp = keep_below ? v - p[ax] : p[ax] - v;
+            const double dq = keep_below ? v - q[ax] : q[ax] - v;
+            if (dp >= 0.0 && m < 16) {
+                for (int k = 0; k < 3; ++k) tmp[m][k] = p[k];
+                ++m;
+            }
+            if ((dp >= 0.0) != (dq >= 0.0) && m < 16) {
+                const double s = dp / (dp - dq);
+                for (int k = 0; k < 3; ++k) tmp[m][k] = p[k] + s * (q[k] - p[k]);
+                tmp[m][ax] = v;
+                ++m;
+            }
+        }
+        n = m;
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) poly[i][k] = tmp[i][k];
+    }
+    if (n == 0) return false;
+    for (int k = 0; k < 3; ++k) {
+        double l = poly[0][k], h = poly[0][k];
+        for (int i = 1; i < n; ++i) {
+            l = std::min(l, poly[i][k]);
+            h = std::max(h, poly[i][k]);
+        }
+        // within the reference's box, whatever the clipping's rounding
+        out_lo[k] = std::max(lo[k], float_down(l));
+        out_hi[k] = std::min(hi[k], float_up(h));
+        if (!(out_lo[k] <= out_hi[k])) return false;
+    }
+    return true;
+}
+
+// Pre-splitting of references (early split clipping: Ernst and Greiner 2007,
+// Karras and Aila 2013): the references whose boxes waste the most area
+// beyond their triangle (box half-area minus the triangle's area) are split
+// in half along their longest axis, each half bounding the triangle's part
+// inside it, until `extra` references have been added.  Every point of a
+// triangle stays inside one of its references' boxes, so a leaf of any of
+// them finds every hit of the triangle; the leaves store the triangle once
+// per reference.
+void presplit(std::vector<PrimRef>& refs, const float* vertices, const uint32_t* indices,
+              size_t extra) {
+    if (extra == 0) return;
+    auto tri_of = [&](uint32_t t, double tri[3][3]) {
+        for (int v = 0; v < 3; ++v) {
+            const uint64_t i = indices ? indices[3 * (uint64_t)t + v] : 3 * (uint64_t)t + v;
+            for (int k = 0; k < 3; ++k) tri[v][k] = vertices[4 * i + k];
+        }
+    };
+    auto tri_area = [&](const double tri[3][3]) {
+        double e1[3], e2[3];
+        for (int k = 0; k < 3; ++k) {
+            e1[k] = tri[1][k] - tri[0][k];
+            e2[k] = tri[2][k] - tri[0][k];
+        }
+        const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                     cz = e1[0] * e2[1] - e1[1] * e2[0];
+        return 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+    };
+    // (waste, reference index); a split part's triangle area is taken as the
+    // whole triangle's times its share of the parent box's half-area
+    struct Item {
+        double waste;
+        uint32_t ref;
+        double area;
+        bool operator<(const Item& o) const { return waste < o.waste; }
+    };
+    std::vector<Item> heap;
+    heap.reserve(refs.size() + extra);
+    for (uint32_t i = 0; i < refs.size(); ++i) {
+        double tri[3][3];
+        tri_of(refs[i].id, tri);
+        const double a = tri_area(tri);
+        heap.push_back({(double)half_area(refs[i].lo, refs[i].hi) - a, i, a});
+    }
+    std::make_heap(heap.begin(), heap.end());
+    size_t added = 0;
+    while (added < extra && !heap.empty()) {
+        std::pop_heap(heap.begin(), heap.end());
+        const Item it = heap.back();
+        heap.pop_back();
+        if (!(it.waste > 0.0)) break;
+        // Only boxes at least kPresplitRatio times their part's area: an
+        // axis-aligned right triangle (a wall's half) has a box of twice its
+        // area, and splitting it only adds triangle tests.
+        if (!(it.waste + it.area > kPresplitRatio * it.area)) continue;
+        PrimRef r = refs[it.ref];
+        int axis = 0;
+        for (int k = 1; k < 3; ++k)
+            if (r.hi[k] - r.lo[k] > r.hi[axis] - r.lo[axis]) axis = k;
+        const double plane = 0.5 * ((double)r.lo[axis] + (double)r.hi[axis]);
+        double tri[3][3];
+        tri_of(r.id, tri);
+        PrimRef a = r, b = r;
+        const bool ha = clipped_bounds(tri, r.lo, r.hi, axis, plane, 0, a.lo, a.hi);
+        const bool hb = clipped_bounds(tri, r.lo, r.hi, axis, plane, 1, b.lo, b.hi);
+        if (!ha || !hb) continue;  // one side is empty: nothing to gain, drop the item
+        const double area_r = std::max(1e-30, (double)half_area(r.lo, r.hi));
+        for (PrimRef* p : {&a, &b})
+            for (int k = 0; k < 3; ++k) p->c[k] = 0.5f * p->lo[k] + 0.5f * p->hi[k];
+        refs[it.ref] = a;
+        refs.push_back(b);
+        ++added;
+        const double ta = it.area * half_area(a.lo, a.hi) / area_r;
+        const double tb = it.area * half_area(b.lo, b.hi) / area_r;
+        heap.push_back({(double)half_area(a.lo, a.hi) - ta, it.ref, ta});
+        std::push_heap(heap.begin(), heap.end());
+        heap.push_back({(double)half_area(b.lo, b.hi) - tb, (uint32_t)(refs.size() - 1), tb});
+        std::push_heap(heap.begin(), heap.end());
+    }
+}
+
 int pick_exponent(float lo, float hi) {
     double extent = (double)hi - (double)lo;
     double need = std::max(extent / 253.0,
@@ -397,6 +574,8 @@ int build_bvh8(const float* vertices, uint64_t num_vertices, const uint32_t* ind
                             .count();
         return 0;
     }
+    if (refs.size() >= kPresplitMinTriangles)
+        presplit(refs, vertices, indices, refs.size() * kPresplitPercent / 100);
 
     BinaryBuilder bb(refs);
     bb.build();
