@@ -282,7 +282,10 @@ def cache_leg(api, scene, spec, cam, reference, frames: int, warm: int,
     log(f"cache leg (spread bound {defer_specular}, stratify {stratify}): {warm} + {frames} frames in {t:.2f} s, "
         f"final relMSE {pts[-1]['relmse']:.5f}")
     # Matrix-core figures from a few more frames with per-launch events (kept
-    # out of the timed run: the events add about a quarter to a frame).
+    # out of the timed run: the events add about a quarter to a frame): the
+    # training rate from frames that train, the inference rate from frames
+    # with the training paused, so that no training kernel shares the GPU
+    # with the inference launch.
     b.set_profiling(True)
     b.kernel_stats()
     prof_frames = 4
@@ -290,16 +293,22 @@ def cache_leg(api, scene, spec, cam, reference, frames: int, warm: int,
         pipe.dispatch_cached(scene, cam, b, cache, warm + frames + 1 + f)
     ks = b.kernel_stats()
     st = cache.stats()
+    cache.set_opts(api.NeuralCacheOpts(**{**frame_opts.__dict__, "train_paths": 0}))
+    for f in range(prof_frames):
+        pipe.dispatch_cached(scene, cam, b, cache, warm + frames + 1 + prof_frames + f)
+    ks_inf = b.kernel_stats()
+    st_inf = cache.stats()
+    b.set_profiling(False)
     targets = []
     for p in plain_pts:
         hit = next((q for q in pts if q["relmse"] <= p["relmse"]), None)
         targets.append({"plain_spp": p["spp"], "relmse": p["relmse"], "plain_seconds": p["seconds"],
                         "cache_seconds": hit["seconds"] if hit else None,
                         "speedup": p["seconds"] / hit["seconds"] if hit else None})
-    infer_launches = ks["cache_infer_launches"]
-    infer_ms = ks["cache_infer_ms"]
+    infer_launches = ks_inf["cache_infer_launches"]
+    infer_ms = ks_inf["cache_infer_ms"]
     train_ms = ks["cache_train_ms"]
-    infer_flops = st["queries"] * NRC_FLOP_FWD * infer_launches
+    infer_flops = st_inf["queries"] * NRC_FLOP_FWD * infer_launches
     train_flops = st["records"] * NRC_FLOP_TRAIN * prof_frames
     infer_tf = infer_flops / (infer_ms * 1e-3) / 1e12 if infer_ms else 0.0
     train_tf = train_flops / (train_ms * 1e-3) / 1e12 if train_ms else 0.0
@@ -324,7 +333,8 @@ def cache_leg(api, scene, spec, cam, reference, frames: int, warm: int,
         "mfma": {
             "infer_tflops": infer_tf, "infer_frac": infer_tf / MFMA_BF16_PEAK_TFLOPS,
             "infer_ms_per_launch": infer_ms / max(infer_launches, 1),
-            "queries_per_launch": st["queries"],
+            "queries_per_launch": st_inf["queries"],
+            "infer_measured_on": "frames with the training paused (no training kernel beside it)",
             "train_tflops": train_tf, "train_frac": train_tf / MFMA_BF16_PEAK_TFLOPS,
             "train_ms_per_frame": train_ms / prof_frames,
             "records_per_frame": st["records"],

@@ -499,7 +499,10 @@ typedef struct lmb_nrc_opts {
                               the samples on the caller's stream.  Either way
                               the samples of frame k use the weights trained
                               up to frame k - 1, and the call's work is
-                              ordered on the caller's stream              (1) */
+                              ordered on the caller's stream.  With 1 and
+                              train_paths = 0 (a frozen cache) the samples
+                              are traced as with training, up to eight at a
+                              time, and nothing else runs               (1) */
     uint32_t stratify;     /* 1: the first scatter of a sample (at the camera
                               ray's hit) draws its direction and Fresnel
                               choice from the pixel's Owen-scrambled Sobol

@@ -620,7 +620,8 @@ static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera
     np.seq_base = seed_offset * p.samples;  // consecutive offsets continue one sequence
     const hipStream_t hs = (hipStream_t)stream;
     KernelTimer* tm = bufs->timer.get();
-    if (train_only || !nrc->opts.overlap_training || nrc->opts.train_paths == 0) {
+    const bool train = nrc->opts.train_paths > 0;
+    if (train_only || !nrc->opts.overlap_training) {
         st = nrc_sync_frags(nrc, hs);
         if (st) return st;
         nrc->rec_in_train_ctl = false;
@@ -641,7 +642,8 @@ static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera
     // not read.  The samples are enqueued first, so the GPU starts on them
     // while the host enqueues the training; at the end the caller's stream
     // waits for the training, so the call's work is complete when the
-    // caller's stream is.
+    // caller's stream is.  With train_paths = 0 (a frozen cache) the samples
+    // are traced the same way and nothing runs on the training stream.
     st = nrc_ensure_overlap(nrc, npix);
     if (st) return st;
     // Samples traced up to kWideSamples at a time (path id pixel * w + j must
@@ -674,10 +676,12 @@ static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera
     db.fwd_pub = nullptr;
     // The training pass's pixel fraction from the cache vertices per sample
     // counted so far on the caller's stream.
-    e = hipMemcpyAsync(db.ctl + kNrcCtlHitsPrev, da.ctl + kNrcCtlHitsPrev, 4,
-                       hipMemcpyDeviceToDevice, hs);
-    if (e == hipSuccess) e = hipEventRecord(nrc->ev_start, hs);
-    if (e != hipSuccess) return fail_hip(e, "training stream order");
+    if (train) {
+        e = hipMemcpyAsync(db.ctl + kNrcCtlHitsPrev, da.ctl + kNrcCtlHitsPrev, 4,
+                           hipMemcpyDeviceToDevice, hs);
+        if (e == hipSuccess) e = hipEventRecord(nrc->ev_start, hs);
+        if (e != hipSuccess) return fail_hip(e, "training stream order");
+    }
     np.train_expect = 0.0f;  // no training paths among the samples
     if (wide) {
         FrameDev fw = bufs->dev;
@@ -693,6 +697,11 @@ static lmb_status trace_nrc(lmb_tracer tracer, lmb_scene scene, const lmb_camera
         e = launch_trace_nrc(scene->dev, cd, bufs->dev, p, da, np, hs, tm);
     }
     if (e != hipSuccess) return fail_hip(e, "cached trace launch");
+    if (!train) {  // frozen: the samples read the current set, nothing is published
+        nrc->inf_valid = true;
+        nrc->rec_in_train_ctl = false;
+        return LMB_OK;
+    }
     e = hipStreamWaitEvent(ts, nrc->ev_start, 0);
     if (e != hipSuccess) return fail_hip(e, "training stream order");
     FrameDev ft = bufs->dev;
