@@ -5,6 +5,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -30,6 +31,10 @@ struct lmb_batch_s {
     DeviceBuffer out_color, out_normal, out_albedo, own_seed, pix_sum, ps_seed, ps_through,
         ps_accum, queue0, queue1, hits, counters;
     DeviceBuffer hits_primary;  // per pixel: camera-ray hits reused by later samples
+    // Samples traced together (path id pixel * group + j) and their per-path
+    // state; group 1: one sample at a time.
+    uint32_t group = 1;
+    DeviceBuffer grp_seed, grp_through, grp_accum, grp_sum, grp_queue0, grp_queue1, grp_hits;
     uint64_t queue_cap = 0;
     SpillBuffer spill;
     FrameDev dev{};
@@ -122,6 +127,50 @@ lmb_status apply_config(lmb_batch b, const lmb_batch_config& c) {
     return LMB_OK;
 }
 
+// Samples a batch of npix pixels traces together: as many as keep a group
+// at or under 2^27 paths (132 B of path state each, 17.7 GB; the environment
+// variable LMB_BATCH_GROUP_PATHS overrides it, 0 = one sample at a time).
+uint32_t batch_group(uint64_t npix, uint32_t samples) {
+    uint64_t target = 1ull << 27;
+    if (const char* env = std::getenv("LMB_BATCH_GROUP_PATHS")) target = std::strtoull(env, nullptr, 10);
+    if (npix == 0 || samples < 2) return 1;
+    const uint64_t g = std::min<uint64_t>(samples, target / npix);
+    return g >= 2 ? (uint32_t)g : 1u;
+}
+
+// Per-path buffers of the sample groups; without the memory for them the
+// batch traces one sample at a time.
+void alloc_groups(lmb_batch b) {
+    b->group = batch_group(b->npix, b->opts.samples);
+    if (b->group < 2) return;
+    const uint64_t paths = b->npix * b->group;
+    if (b->grp_seed.alloc(paths * 4) || b->grp_through.alloc(paths * 16) ||
+        b->grp_accum.alloc(paths * 16) || b->grp_sum.alloc(paths * 16) ||
+        b->grp_queue0.alloc(paths * 32) || b->grp_queue1.alloc(paths * 32) ||
+        b->grp_hits.alloc(paths * 16)) {
+        for (DeviceBuffer* d : {&b->grp_seed, &b->grp_through, &b->grp_accum, &b->grp_sum,
+                                &b->grp_queue0, &b->grp_queue1, &b->grp_hits})
+            d->release();
+        b->group = 1;
+    }
+}
+
+// The frame view of the sample groups (nullptr: one sample at a time).
+const FrameDev* group_view(lmb_batch b, FrameDev* g) {
+    if (b->group < 2) return nullptr;
+    *g = b->dev;
+    g->group = b->group;
+    g->grid_paths = b->npix * b->group;
+    g->ps_seed = b->grp_seed.as<uint32_t>();
+    g->ps_through = b->grp_through.as<float4>();
+    g->ps_accum = b->grp_accum.as<float4>();
+    g->pix_sum = b->grp_sum.as<float4>();
+    g->queue[0] = b->grp_queue0.as<float4>();
+    g->queue[1] = b->grp_queue1.as<float4>();
+    g->hits = b->grp_hits.as<uint4>();
+    return g;
+}
+
 TraceParams params_of(const lmb_tracer_opts& o) {
     TraceParams p;
     p.quality = o.quality;
@@ -143,7 +192,9 @@ lmb_status ensure_graph(lmb_batch b) {
     if (e != hipSuccess) return fail_hip(e, "hipStreamCreate");
     e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return fail_hip(e, "hipStreamBeginCapture");
-    hipError_t le = launch_batch(b->bdev, b->dev, params_of(b->opts), b->launch, b->cap_stream);
+    FrameDev gv;
+    hipError_t le = launch_batch(b->bdev, b->dev, group_view(b, &gv), params_of(b->opts), b->launch,
+                                 b->cap_stream);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(b->cap_stream, &g);
     if (le != hipSuccess) {
@@ -178,7 +229,8 @@ lmb_status run_frame(lmb_batch b, uint32_t seed_offset, hipStream_t stream) {
         if (e != hipSuccess) return fail_hip(e, "hipGraphLaunch");
         return LMB_OK;
     }
-    e = launch_batch(b->bdev, b->dev, params_of(b->opts), b->launch, stream);
+    FrameDev gv;
+    e = launch_batch(b->bdev, b->dev, group_view(b, &gv), params_of(b->opts), b->launch, stream);
     if (e != hipSuccess) return fail_hip(e, "batch launch");
     return LMB_OK;
 }
@@ -285,6 +337,7 @@ lmb_status lmb_batch_create(const lmb_batch_item* items, uint32_t n, const lmb_t
     f.ps_accum = b->ps_accum.as<float4>();
     f.counters = b->counters.as<uint32_t>();
     f.hits_primary = b->hits_primary.as<uint4>();  // indexed by pixel here
+    alloc_groups(b);
     BatchDev& bd = b->bdev;
     bd.scenes = b->scene_table.as<SceneDev>();
     bd.num_items = n;

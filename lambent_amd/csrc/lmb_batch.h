@@ -37,9 +37,14 @@ struct BatchLaunch {
     uint64_t pass_tiles;  // tiles per wavefront pass
 };
 
-// One frame: for each pass of pass_tiles tiles, every sample's wavefronts;
-// then colour = max(pix_sum / samples, 0) for every pixel.  Capturable.
-hipError_t launch_batch(const BatchDev& b, const FrameDev& f, const TraceParams& p,
-                        const BatchLaunch& l, hipStream_t stream);
+// One frame, then colour = max(pix_sum / samples, 0) for every pixel.
+// Capturable.  With `fg` (per-path buffers for fg->group samples of every
+// pixel: path id pixel * group + j) and one pass over all tiles, the camera
+// rays are generated and traversed once and the samples are traced in
+// groups of fg->group, as the single-image tracer does; otherwise, for each
+// pass of pass_tiles tiles, every sample's wavefronts one sample at a time.
+// Both give the same images.
+hipError_t launch_batch(const BatchDev& b, const FrameDev& f, const FrameDev* fg,
+                        const TraceParams& p, const BatchLaunch& l, hipStream_t stream);
 
 }  // namespace lmb

@@ -203,8 +203,9 @@ __global__ __launch_bounds__(kPtBlock) void k_shade_plain_t(SceneDev s1, const S
             const uint32_t pid = group_camera ? pix * group + j : pix;
             const uint4 h = f.hits[qs];
             uint32_t seed = f.ps_seed[pid];
-            const float4 th = f.ps_through[pid];
-            const float4 ac = f.ps_accum[pid];
+            // a group's paths start here: throughput 1, nothing gathered yet
+            const float4 th = group_camera ? make_float4(1.f, 1.f, 1.f, -1.f) : f.ps_through[pid];
+            const float4 ac = group_camera ? make_float4(0.f, 0.f, 0.f, 0.f) : f.ps_accum[pid];
             V3 color = v3(th.x, th.y, th.z);
             float picked = th.w;
             V3 scol = v3(ac.x, ac.y, ac.z);
@@ -367,9 +368,7 @@ __global__ __launch_bounds__(kPtBlock) void k_generate_group(CameraDev cam, Fram
         const uint32_t j = (uint32_t)(pid - pix * group);
         uint32_t seed = first_start_seed(f, p, s0, pix);
         for (uint32_t k = 0; k < j; ++k) seed = rand_u32(seed);
-        fg.ps_seed[pid] = seed;
-        fg.ps_through[pid] = make_float4(1.f, 1.f, 1.f, -1.f);
-        fg.ps_accum[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        fg.ps_seed[pid] = seed;  // throughput and gathered colour start in the shading
     }
     if (s0 != 0) return;
     for (uint64_t slot = gtid; slot < f.npix; slot += stride) {
@@ -492,10 +491,12 @@ __device__ __forceinline__ uint32_t batch_item_of(const BatchDev& b, uint64_t ti
 // Primary rays of the tiles [t0, t1): consecutive threads walk a
 // tile_w x tile_h pixel tile, so each wavefront of the queue holds a compact
 // screen tile of one image.  Seeds as general.wesl:516 with the image's own
-// x, y; rays carry their image index in direction .w.
+// x, y; rays carry their image index in direction .w.  With `camera_only`
+// (sample groups) only the rays are written and pix_sum cleared: the
+// groups' seeds come from k_batch_group_start.
 __global__ __launch_bounds__(kPtBlock) void k_batch_generate(BatchDev b, FrameDev f, TraceParams p,
                                                              uint32_t sample, uint64_t t0,
-                                                             uint64_t t1) {
+                                                             uint64_t t1, uint32_t camera_only) {
     const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
     const uint32_t tile_px = b.tile_w * b.tile_h;
@@ -521,21 +522,19 @@ __global__ __launch_bounds__(kPtBlock) void k_batch_generate(BatchDev b, FrameDe
         float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4;
         if (valid) {
             const BatchItemDev& it = b.items[item];
-            uint32_t seed;
-            if (sample == 0) {
-                seed = rand_u32(x * y) + seed_offset + y;
-                f.pix_sum[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
-            } else {
-                seed = rand_u32(f.own_seed[pid]);
+            if (sample == 0) f.pix_sum[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (!camera_only) {
+                const uint32_t seed = sample == 0 ? rand_u32(x * y) + seed_offset + y
+                                                  : rand_u32(f.own_seed[pid]);
+                f.own_seed[pid] = seed;
+                f.ps_seed[pid] = seed;
+                f.ps_through[pid] = make_float4(1.f, 1.f, 1.f, -1.f);
+                f.ps_accum[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
-            f.own_seed[pid] = seed;
             float o[3], d[3];
             create_ray(it.cam, x, y, it.width, it.height, o, d);
             o4 = make_float4(o[0], o[1], o[2], __uint_as_float((uint32_t)pid));
             d4 = make_float4(d[0], d[1], d[2], __uint_as_float(item));
-            f.ps_seed[pid] = seed;
-            f.ps_through[pid] = make_float4(1.f, 1.f, 1.f, -1.f);
-            f.ps_accum[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         uint32_t idx[1];
         block_reserve<kPtBlock, 1>({valid}, {f.counters}, rl, idx);
@@ -543,6 +542,62 @@ __global__ __launch_bounds__(kPtBlock) void k_batch_generate(BatchDev b, FrameDe
             f.queue[0][2 * (uint64_t)idx[0]] = o4;
             f.queue[0][2 * (uint64_t)idx[0] + 1] = d4;
         }
+    }
+}
+
+__device__ __forceinline__ uint32_t batch_item_of_pixel(const BatchDev& b, uint64_t pix) {
+    uint32_t lo = 0, hi = b.num_items - 1;
+    while (lo < hi) {  // last item whose first pixel is <= pix
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (b.items[mid].pix_base <= pix)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Start seeds of samples s0 .. s0 + group - 1 of every pixel of the batch,
+// one thread per pixel (general.wesl:516, 544, as k_batch_generate draws
+// them one sample at a time): path pixel * group + j starts sample s0 + j,
+// and own_seed keeps the start seed of the group's last sample.
+__global__ __launch_bounds__(kPtBlock) void k_batch_group_start(BatchDev b, FrameDev f,
+                                                                FrameDev fg, uint32_t s0) {
+    const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
+    const uint32_t group = fg.group;
+    const uint32_t seed_offset = *b.seed;
+    for (uint64_t pix = gtid; pix < f.npix; pix += stride) {
+        uint32_t seed;
+        if (s0 == 0) {
+            const BatchItemDev& it = b.items[batch_item_of_pixel(b, pix)];
+            const uint32_t local = (uint32_t)(pix - it.pix_base);
+            const uint32_t x = local % it.width, y = local / it.width;
+            seed = rand_u32(x * y) + seed_offset + y;
+        } else {
+            seed = rand_u32(f.own_seed[pix]);
+        }
+        for (uint32_t j = 0; j < group; ++j) {
+            if (j > 0) seed = rand_u32(seed);
+            fg.ps_seed[pix * group + j] = seed;  // the rest of the path state starts in the shading
+        }
+        f.own_seed[pix] = seed;
+    }
+}
+
+// pix_sum += the group's sample colours, in sample order.
+__global__ __launch_bounds__(kPtBlock) void k_batch_group_sum(FrameDev f,
+                                                              const float4* __restrict__ colour,
+                                                              uint32_t group) {
+    const uint64_t gtid = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kPtBlock;
+    for (uint64_t pix = gtid; pix < f.npix; pix += stride) {
+        float4 ps = f.pix_sum[pix];
+        for (uint32_t j = 0; j < group; ++j) {
+            const float4 c = colour[pix * group + j];
+            ps = make_float4(ps.x + c.x, ps.y + c.y, ps.z + c.z, 0.f);
+        }
+        f.pix_sum[pix] = ps;
     }
 }
 
@@ -614,8 +669,44 @@ __global__ __launch_bounds__(kPtBlock) void k_batch_extend(const SceneDev* table
 }
 
 template <uint32_t Q>
-static hipError_t launch_batch_q(const BatchDev& b, const FrameDev& f, const TraceParams& p,
-                                 const BatchLaunch& l, hipStream_t stream) {
+static hipError_t launch_batch_grouped(const BatchDev& b, const FrameDev& f, const FrameDev& fg,
+                                       const TraceParams& p, const BatchLaunch& l,
+                                       hipStream_t stream) {
+    const dim3 grid(l.blocks), block(kPtBlock);
+    const SceneDev none{};
+    for (uint32_t s0 = 0; s0 < p.samples; s0 += fg.group) {
+        FrameDev g = fg;
+        g.group = std::min(fg.group, p.samples - s0);
+        hipLaunchKernelGGL(k_batch_begin, dim3(1), block, 0, stream, f, (uint32_t)(s0 == 0));
+        if (s0 == 0) {
+            // camera rays and their hits, shared by every sample (no jitter)
+            hipLaunchKernelGGL(k_batch_generate, grid, block, 0, stream, b, f, p, 0u, (uint64_t)0,
+                               b.tiles_total, 1u);
+            hipLaunchKernelGGL(k_batch_extend, grid, block, 0, stream, b.scenes, f, p, 0u, l.refill);
+        }
+        hipLaunchKernelGGL(k_batch_group_start, grid, block, 0, stream, b, f, g, s0);
+        FrameDev g0 = g;  // bounce 0 reads the camera queue and hits
+        g0.queue[0] = f.queue[0];
+        g0.hits = f.hits;
+        for (uint32_t bn = 0; bn < p.bounces; ++bn) {
+            if (bn > 0)
+                hipLaunchKernelGGL(k_batch_extend, grid, block, 0, stream, b.scenes, g, p, bn,
+                                   l.refill);
+            hipLaunchKernelGGL((k_shade_plain_t<Q, true, true>), grid, block, 0, stream, none,
+                               b.scenes, bn == 0 ? g0 : g, p, bn);
+        }
+        hipLaunchKernelGGL(k_batch_group_sum, grid, block, 0, stream, f, (const float4*)g.pix_sum,
+                           g.group);
+    }
+    hipLaunchKernelGGL(k_resolve, grid, block, 0, stream, f, p);
+    return hipGetLastError();
+}
+
+template <uint32_t Q>
+static hipError_t launch_batch_q(const BatchDev& b, const FrameDev& f, const FrameDev* fg,
+                                 const TraceParams& p, const BatchLaunch& l, hipStream_t stream) {
+    if (fg && fg->group > 1 && p.samples > 1 && l.pass_tiles >= b.tiles_total)
+        return launch_batch_grouped<Q>(b, f, *fg, p, l, stream);
     const dim3 grid(l.blocks), block(kPtBlock);
     const SceneDev none{};
     for (uint64_t t0 = 0; t0 < b.tiles_total; t0 += l.pass_tiles) {
@@ -623,7 +714,8 @@ static hipError_t launch_batch_q(const BatchDev& b, const FrameDev& f, const Tra
         for (uint32_t sample = 0; sample < p.samples; ++sample) {
             hipLaunchKernelGGL(k_batch_begin, dim3(1), block, 0, stream, f,
                                (uint32_t)(t0 == 0 && sample == 0));
-            hipLaunchKernelGGL(k_batch_generate, grid, block, 0, stream, b, f, p, sample, t0, t1);
+            hipLaunchKernelGGL(k_batch_generate, grid, block, 0, stream, b, f, p, sample, t0, t1,
+                               0u);
             // camera rays carry no jitter: later samples reuse the first's hits
             const bool reuse = p.samples > 1 && f.hits_primary != nullptr;
             if (reuse && sample > 0)
@@ -643,19 +735,19 @@ static hipError_t launch_batch_q(const BatchDev& b, const FrameDev& f, const Tra
     return hipGetLastError();
 }
 
-hipError_t launch_batch(const BatchDev& b, const FrameDev& f, const TraceParams& p,
-                        const BatchLaunch& l, hipStream_t stream) {
+hipError_t launch_batch(const BatchDev& b, const FrameDev& f, const FrameDev* fg,
+                        const TraceParams& p, const BatchLaunch& l, hipStream_t stream) {
     if (b.num_items == 0 || b.tiles_total == 0 || l.pass_tiles == 0 || l.blocks == 0)
         return hipSuccess;
     switch (p.quality) {
         case kQualityLow:
-            return launch_batch_q<kQualityLow>(b, f, p, l, stream);
+            return launch_batch_q<kQualityLow>(b, f, fg, p, l, stream);
         case kQualityHigh:
-            return launch_batch_q<kQualityHigh>(b, f, p, l, stream);
+            return launch_batch_q<kQualityHigh>(b, f, fg, p, l, stream);
         case kQualitySpectral:
-            return launch_batch_q<kQualitySpectral>(b, f, p, l, stream);
+            return launch_batch_q<kQualitySpectral>(b, f, fg, p, l, stream);
         default:
-            return launch_batch_q<kQualityMedium>(b, f, p, l, stream);
+            return launch_batch_q<kQualityMedium>(b, f, fg, p, l, stream);
     }
 }
 

@@ -101,3 +101,26 @@ def test_autotune_picks_a_candidate_and_keeps_the_images(items):
     assert batch.config == best
     batch.trace(2)
     check(batch, ref)
+
+
+def test_sample_groups_with_a_remainder(items, monkeypatch):
+    """5 samples traced in groups of 2, 2 and 1 (the group size capped by
+    LMB_BATCH_GROUP_PATHS): the start seeds carry over from group to group
+    as from sample to sample, so the images and the path-vertex count equal
+    the single-scene tracer's."""
+    opts = api.PathTracerOpts(samples=5, bounces=5, rr_start=2, guiding=api.Guiding.OFF,
+                              restir_temporal=api.RestirTemporalMode.Disabled)
+    npix = sum(w * h for _, _, w, h in items)
+    pipe = api.RayTracer(api.Medium).create_pipeline(opts)
+    ref, verts = [], 0
+    for scene, cam, w, h in items:
+        b = api.DataBuffers(w, h, 1, api.BufferType(0))
+        pipe.dispatch(scene, cam, b, 4)
+        ref.append(b.output(api.Output.COLOR))
+        verts += b.path_vertices()
+    for cap in (2 * npix, 0):  # groups of 2; one sample at a time
+        monkeypatch.setenv("LMB_BATCH_GROUP_PATHS", str(cap))
+        batch = api.Batch(items, opts, api.Medium)
+        batch.trace(4)
+        check(batch, ref)
+        assert batch.path_vertices() == verts
