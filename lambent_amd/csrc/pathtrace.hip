@@ -509,13 +509,16 @@ __global__ __launch_bounds__(kPtBlock) void k_batch_generate(BatchDev b, FrameDe
         uint32_t item = 0, x = 0, y = 0;
         uint64_t pid = 0;
         if (valid) {
-            const uint64_t tile = t0 + g / tile_px;
-            const uint32_t within = (uint32_t)(g % tile_px);
+            // a pass holds under 2^32 pixels and an image under 2^32 tiles:
+            // 32-bit division suffices
+            const uint32_t g32 = (uint32_t)g;
+            const uint64_t tile = t0 + g32 / tile_px;
+            const uint32_t within = g32 % tile_px;
             item = batch_item_of(b, tile);
             const BatchItemDev& it = b.items[item];
-            const uint64_t lt = tile - it.tile_base;
-            x = (uint32_t)(lt % it.tiles_x) * b.tile_w + within % b.tile_w;
-            y = (uint32_t)(lt / it.tiles_x) * b.tile_h + within / b.tile_w;
+            const uint32_t lt = (uint32_t)(tile - it.tile_base);
+            x = (lt % it.tiles_x) * b.tile_w + within % b.tile_w;
+            y = (lt / it.tiles_x) * b.tile_h + within / b.tile_w;
             valid = x < it.width && y < it.height;
             pid = it.pix_base + (uint64_t)y * it.width + x;
         }
