@@ -92,6 +92,7 @@ __global__ __launch_bounds__(kPtBlock) void k_generate(CameraDev cam, FrameDev f
         f.queue[0][2 * slot] = make_float4(o[0], o[1], o[2], __uint_as_float((uint32_t)pix));
         f.queue[0][2 * slot + 1] = make_float4(d[0], d[1], d[2], 0.f);
         f.ps_seed[pix] = seed;
+        // the cached tracer's shading (nrc_trace.hip) reads these back
         f.ps_through[pix] = make_float4(1.f, 1.f, 1.f, -1.f);
         f.ps_accum[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
@@ -203,9 +204,9 @@ __global__ __launch_bounds__(kPtBlock) void k_shade_plain_t(SceneDev s1, const S
             const uint32_t pid = group_camera ? pix * group + j : pix;
             const uint4 h = f.hits[qs];
             uint32_t seed = f.ps_seed[pid];
-            // a group's paths start here: throughput 1, nothing gathered yet
-            const float4 th = group_camera ? make_float4(1.f, 1.f, 1.f, -1.f) : f.ps_through[pid];
-            const float4 ac = group_camera ? make_float4(0.f, 0.f, 0.f, 0.f) : f.ps_accum[pid];
+            // every path starts here: throughput 1, nothing gathered yet
+            const float4 th = bounce == 0 ? make_float4(1.f, 1.f, 1.f, -1.f) : f.ps_through[pid];
+            const float4 ac = bounce == 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : f.ps_accum[pid];
             V3 color = v3(th.x, th.y, th.z);
             float picked = th.w;
             V3 scol = v3(ac.x, ac.y, ac.z);
@@ -530,9 +531,7 @@ __global__ __launch_bounds__(kPtBlock) void k_batch_generate(BatchDev b, FrameDe
                 const uint32_t seed = sample == 0 ? rand_u32(x * y) + seed_offset + y
                                                   : rand_u32(f.own_seed[pid]);
                 f.own_seed[pid] = seed;
-                f.ps_seed[pid] = seed;
-                f.ps_through[pid] = make_float4(1.f, 1.f, 1.f, -1.f);
-                f.ps_accum[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+                f.ps_seed[pid] = seed;  // throughput and colour start in the shading
             }
             float o[3], d[3];
             create_ray(it.cam, x, y, it.width, it.height, o, d);
